@@ -1,0 +1,251 @@
+"""eirene_amd — MI355X-native transactional B+tree batch path (Python host layer).
+
+The product is the C ABI of ``include/stmtree.h`` implemented by ``libstmtree.so``
+(hand-written HIP kernels for gfx950).  This module binds it with ctypes and mirrors the
+reference's batch interface (``GPU_BTree`` in the reference's src/tree.h:28-43):
+
+    prepareGPU()            -> StmTree(...)
+    prepareInput + gpu_work -> StmTree.run_batch(keys, vals, ops)
+    first "-i" insert file  -> StmTree.bulk_load(keys, vals)
+    test()                  -> the (values, found) arrays returned by run_batch
+
+Op codes are the reference's (src/input-helper.h:14-17): INSERT=0, GET=1, DELETE=2,
+RANGE=3; codes other than INSERT run with GET semantics, as in the reference's merge.
+There is no CPU fallback: importing works without a GPU, but creating a tree needs the
+HIP library and a device, and fails with StmError otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+__all__ = [
+    "StmTree", "StmError", "StmConfig", "load_library", "LIB_PATH",
+    "OP_INSERT", "OP_GET", "OP_DELETE", "OP_RANGE", "KEY_RESERVED",
+    "ROUTE_DESCENT", "ROUTE_LEARNED", "PHASES",
+]
+
+LIB_PATH = Path(__file__).resolve().parent / "libstmtree.so"
+
+OP_INSERT, OP_GET, OP_DELETE, OP_RANGE = 0, 1, 2, 3
+KEY_RESERVED = 0xFFFFFFFFFFFFFFFF
+ROUTE_DESCENT, ROUTE_LEARNED = 0, 1
+PHASES = ("sort", "merge", "lookup", "upsert", "deferred", "route", "train", "other")
+
+STATUS = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "STATE"}
+
+
+class StmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS.get(code, code)}: {message}")
+        self.code = code
+
+
+class _Config(C.Structure):
+    _fields_ = [
+        ("fanout", C.c_uint32), ("leaf_fill", C.c_uint32), ("max_keys", C.c_uint64),
+        ("max_batch", C.c_uint64), ("device", C.c_int32), ("lanes_per_op", C.c_uint32),
+        ("ops_per_group", C.c_uint32), ("upsert_ops_per_wave", C.c_uint32),
+        ("max_attempts", C.c_uint32), ("route", C.c_uint32), ("timing", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
+    ]
+
+
+class _Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "n_keys", "n_nodes", "capacity_nodes", "height", "batches", "ops", "unique_ops",
+        "upserts", "commits", "aborts", "leaf_splits", "inner_splits", "root_splits",
+        "deferred", "route_hits", "route_misses")] + [
+        ("phase_ms", C.c_double * 8), ("phase_launches", C.c_uint64 * 8)]
+
+
+_lib: Optional[C.CDLL] = None
+_P = C.c_void_p
+_U64P = C.POINTER(C.c_uint64)
+_U8P = C.POINTER(C.c_uint8)
+
+
+def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
+    """Loads libstmtree.so (built by ``python -m eirene_amd.build``); raises if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise StmError(3, f"native library {p} is missing; run `python -m eirene_amd.build`")
+    lib = C.CDLL(str(p))
+    sig = {
+        "stm_config_init": (C.c_int, [C.POINTER(_Config)]),
+        "stm_create": (C.c_int, [C.POINTER(_Config), C.POINTER(_P)]),
+        "stm_destroy": (C.c_int, [_P]),
+        "stm_bulk_load": (C.c_int, [_P, _P, _P, C.c_size_t]),
+        "stm_bulk_load_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
+        "stm_run_batch": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
+        "stm_run_batch_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
+        "stm_size": (C.c_int, [_P, _U64P]),
+        "stm_dump": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "stm_check": (C.c_int, [_P, _U64P]),
+        "stm_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
+        "stm_reset_stats": (C.c_int, [_P]),
+        "stm_get_stream": (C.c_int, [_P, C.POINTER(_P)]),
+        "stm_last_error": (C.c_char_p, []),
+        # workload generators (host only)
+        "stm_splitmix64": (C.c_uint64, [C.c_uint64]),
+        "stm_gen_values": (None, [_P, C.c_size_t, _P]),
+        "stm_gen_keys": (C.c_int, [C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double, _P]),
+        "stm_gen_mixed_batch": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.c_uint64, C.c_double,
+                                          C.c_double, C.c_int, C.c_double, C.c_double, _P, _P, _P]),
+        "stm_gen_zipf_batch": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.c_uint64, C.c_double,
+                                         C.c_double, _P, _P, _P]),
+        "stm_gen_batch_sizes": (None, [C.c_size_t, C.c_uint64, C.c_double, C.c_double, _P]),
+        "stm_sort_u64": (None, [_P, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise StmError(rc, load_library().stm_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _u8(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+@dataclass
+class StmConfig:
+    fanout: int = 64
+    leaf_fill: int = 0
+    max_keys: int = 1 << 24
+    max_batch: int = 1 << 22
+    device: int = 0
+    lanes_per_op: int = 16
+    ops_per_group: int = 16
+    upsert_ops_per_wave: int = 1
+    max_attempts: int = 64
+    route: int = ROUTE_DESCENT
+    timing: bool = False
+
+    def to_c(self) -> _Config:
+        c = _Config()
+        _check(load_library().stm_config_init(C.byref(c)))
+        for f in ("fanout", "leaf_fill", "max_keys", "max_batch", "device", "lanes_per_op",
+                  "ops_per_group", "upsert_ops_per_wave", "max_attempts", "route"):
+            setattr(c, f, int(getattr(self, f)))
+        c.timing = 1 if self.timing else 0
+        return c
+
+
+class StmTree:
+    """One device-resident tree.  Not thread-safe: drive it from one thread."""
+
+    def __init__(self, config: Optional[StmConfig] = None, **kw):
+        cfg = config or StmConfig(**kw)
+        self.config = cfg
+        self._lib = load_library()
+        h = C.c_void_p()
+        _check(self._lib.stm_create(C.byref(cfg.to_c()), C.byref(h)))
+        self._h = h
+
+    # -- lifetime
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.stm_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- operations
+    def bulk_load(self, keys, vals) -> None:
+        k, v = _u64(keys), _u64(vals)
+        if k.shape != v.shape:
+            raise ValueError("keys and vals differ in length")
+        _check(self._lib.stm_bulk_load(self._h, _ptr(k), _ptr(v), k.size))
+
+    def bulk_load_device(self, keys_t, vals_t) -> None:
+        _check(self._lib.stm_bulk_load_device(self._h, keys_t.data_ptr(), vals_t.data_ptr(),
+                                              keys_t.numel()))
+
+    def run_batch(self, keys, vals, ops):
+        """Runs one batch; returns (values u64[n], found u8[n])."""
+        k, o = _u64(keys), _u8(ops)
+        v = _u64(vals) if vals is not None else None
+        n = k.size
+        if o.size != n or (v is not None and v.size != n):
+            raise ValueError("keys, vals and ops differ in length")
+        out_v = np.zeros(n, dtype=np.uint64)
+        out_f = np.zeros(n, dtype=np.uint8)
+        _check(self._lib.stm_run_batch(self._h, _ptr(k), _ptr(v) if v is not None else None,
+                                       _ptr(o), n, _ptr(out_v), _ptr(out_f)))
+        return out_v, out_f
+
+    def run_batch_device(self, keys_t, vals_t, ops_t, out_vals_t, out_found_t) -> None:
+        """Device-resident batch (torch tensors on the tree's device)."""
+        _check(self._lib.stm_run_batch_device(
+            self._h, keys_t.data_ptr(), vals_t.data_ptr() if vals_t is not None else None,
+            ops_t.data_ptr(), keys_t.numel(), out_vals_t.data_ptr(), out_found_t.data_ptr()))
+
+    def size(self) -> int:
+        n = C.c_uint64()
+        _check(self._lib.stm_size(self._h, C.byref(n)))
+        return n.value
+
+    def dump(self):
+        """In-order (keys, values) of the whole tree."""
+        n = C.c_size_t()
+        _check(self._lib.stm_dump(self._h, None, None, 0, C.byref(n)))
+        keys = np.empty(n.value, dtype=np.uint64)
+        vals = np.empty(n.value, dtype=np.uint64)
+        _check(self._lib.stm_dump(self._h, _ptr(keys), _ptr(vals), n.value, C.byref(n)))
+        return keys[: n.value], vals[: n.value]
+
+    def check(self) -> int:
+        """Number of violated structural invariants (0 = well formed)."""
+        bad = C.c_uint64()
+        _check(self._lib.stm_check(self._h, C.byref(bad)))
+        return bad.value
+
+    def stats(self) -> dict:
+        s = _Stats()
+        _check(self._lib.stm_get_stats(self._h, C.byref(s)))
+        d = {name: getattr(s, name) for name, _ in _Stats._fields_[:16]}
+        d["phase_ms"] = {p: s.phase_ms[i] for i, p in enumerate(PHASES)}
+        d["phase_launches"] = {p: s.phase_launches[i] for i, p in enumerate(PHASES)}
+        return d
+
+    def reset_stats(self) -> None:
+        _check(self._lib.stm_reset_stats(self._h))
+
+    def stream(self) -> int:
+        p = C.c_void_p()
+        _check(self._lib.stm_get_stream(self._h, C.byref(p)))
+        return p.value or 0

@@ -1,0 +1,714 @@
+// stm_api.hip — the C ABI of include/stmtree.h: tree object, batch pipeline, bulk build,
+// dump, structural check and statistics.
+//
+// Batch pipeline on the tree's stream (reference: GPU_STM_Tree::gpu_work_main,
+// src/gpu-stm-tree-v38/bpt-gpustm.cpp:77-167, and launchKernelBefore/launchKernel/
+// launchKernelAfter, src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:264-332):
+//   sort (key, position)  ->  per-key fold  ->  read-only lookups  ->  transactional upserts
+// Batches longer than max_batch are cut into consecutive chunks, as the reference cuts
+// its input file into batch_size pieces; each chunk completes before the next starts,
+// so results do not depend on the chunking.  No host round trip happens inside a chunk
+// (the reference copies the run count to the host between RLE and scan, :303).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/stmtree.h"
+#include "stm_common.hpp"
+#include "stm_internal.hpp"
+#include "stm_tree.hpp"
+
+using namespace stm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(e_ == hipErrorOutOfMemory ? STM_ERR_NOMEM : STM_ERR_DEVICE, "%s: %s (%s:%d)", \
+                  #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, count * sizeof(T));
+}
+
+struct Scratch {
+  uint64_t n = 0;
+  uint64_t *in_keys = nullptr, *in_vals = nullptr, *out_vals = nullptr;
+  uint8_t *in_ops = nullptr, *out_found = nullptr;
+  uint64_t *skeys_tmp = nullptr, *skeys = nullptr;
+  uint32_t *sidx_tmp = nullptr, *sidx = nullptr;
+  uint32_t* hist = nullptr;
+  void* fold_tiles = nullptr;
+  uint64_t *ukey = nullptr, *uval = nullptr;
+  uint8_t* uop = nullptr;
+  uint32_t *gstart = nullptr, *npend = nullptr, *ulist = nullptr, *dlist = nullptr;
+  BatchCounters* counters = nullptr;
+
+  hipError_t alloc(uint64_t cap, bool staging) {
+    n = cap;
+    hipError_t e;
+#define A_(p, c) if ((e = dalloc(&p, c)) != hipSuccess) return e
+    if (staging) {
+      A_(in_keys, cap); A_(in_vals, cap); A_(in_ops, cap); A_(out_vals, cap); A_(out_found, cap);
+    }
+    A_(skeys_tmp, cap); A_(skeys, cap); A_(sidx_tmp, cap); A_(sidx, cap);
+    A_(hist, sort_hist_words(cap));
+    uint8_t* ft = nullptr;
+    A_(ft, fold_scratch_bytes(cap));
+    fold_tiles = ft;
+    A_(ukey, cap); A_(uval, cap); A_(uop, cap); A_(gstart, cap); A_(npend, cap);
+    A_(ulist, cap); A_(dlist, cap);
+    A_(counters, 1);
+#undef A_
+    return hipSuccess;
+  }
+  void release() {
+    void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, ulist, dlist, counters};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    *this = Scratch();
+  }
+};
+
+}  // namespace
+
+struct stm_tree {
+  stm_config cfg{};
+  hipStream_t stream = nullptr;
+  TreeView view{};
+  uint64_t capacity = 0;
+  Scratch scr;
+  stm_stats st{};
+  hipEvent_t ev_a[STM_NUM_PHASES] = {};
+  hipEvent_t ev_b[STM_NUM_PHASES] = {};
+  bool phase_used[STM_NUM_PHASES] = {};
+};
+
+namespace {
+
+uint64_t capacity_for(uint64_t max_keys, uint32_t F) {
+  const uint64_t half = F / 2;
+  uint64_t leaves = (max_keys + half - 1) / half + 1;
+  uint64_t cap = leaves + leaves / (half - 1) + 64;
+  cap += cap / 10 + 1024;
+  return cap;
+}
+
+int set_device(stm_tree* t) {
+  HIP_TRY(hipSetDevice(t->cfg.device));
+  return STM_OK;
+}
+
+void phase_begin(stm_tree* t, int p) {
+  if (!t->cfg.timing) return;
+  (void)hipEventRecord(t->ev_a[p], t->stream);
+}
+void phase_end(stm_tree* t, int p) {
+  if (!t->cfg.timing) return;
+  (void)hipEventRecord(t->ev_b[p], t->stream);
+  t->phase_used[p] = true;
+}
+void phase_collect(stm_tree* t) {
+  if (!t->cfg.timing) return;
+  for (int p = 0; p < STM_NUM_PHASES; ++p) {
+    if (!t->phase_used[p]) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t->ev_a[p], t->ev_b[p]) == hipSuccess) {
+      t->st.phase_ms[p] += ms;
+      t->st.phase_launches[p] += 1;
+    }
+    t->phase_used[p] = false;
+  }
+}
+
+int read_header(stm_tree* t, TreeHeader* h) {
+  HIP_TRY(hipMemcpyAsync(h, t->view.hdr, sizeof(TreeHeader), hipMemcpyDeviceToHost, t->stream));
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  return STM_OK;
+}
+
+int check_device_error(stm_tree* t) {
+  TreeHeader h;
+  int rc = read_header(t, &h);
+  if (rc) return rc;
+  if (h.error & ERR_POOL_FULL)
+    return fail(STM_ERR_NOMEM, "node pool exhausted (capacity %llu nodes); raise max_keys",
+                (unsigned long long)t->capacity);
+  if (h.error & ERR_LIVELOCK) return fail(STM_ERR_DEVICE, "serial upsert pass could not commit");
+  return STM_OK;
+}
+
+// Writes an empty tree: one empty leaf as root.
+int init_empty(stm_tree* t) {
+  NodeMeta m{};
+  m.lock = 0;
+  m.info = 0;
+  m.low = 0;
+  m.high = KEY_EMPTY;
+  m.next = NODE_NONE;
+  HIP_TRY(hipMemsetAsync(t->view.keys, 0xff, sizeof(uint64_t) * t->cfg.fanout, t->stream));
+  HIP_TRY(hipMemsetAsync(t->view.vals, 0, sizeof(uint64_t) * t->cfg.fanout, t->stream));
+  HIP_TRY(hipMemcpyAsync(t->view.meta, &m, sizeof(m), hipMemcpyHostToDevice, t->stream));
+  TreeHeader h{};
+  h.lock = 0;
+  h.root = 0;
+  h.height = 1;
+  h.n_nodes = 1;
+  h.capacity = t->capacity;
+  h.error = 0;
+  h.n_keys = 0;
+  HIP_TRY(hipMemcpyAsync(t->view.hdr, &h, sizeof(h), hipMemcpyHostToDevice, t->stream));
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  return STM_OK;
+}
+
+// One chunk of at most max_batch ops, all pointers on the device.
+int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+              uint32_t n, uint64_t* out_vals, uint8_t* out_found) {
+  Scratch& s = t->scr;
+  hipStream_t st = t->stream;
+  phase_begin(t, STM_PHASE_SORT);
+  HIP_TRY(radix_sort_pairs(keys, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, n, 0, 64, s.hist, st));
+  phase_end(t, STM_PHASE_SORT);
+
+  phase_begin(t, STM_PHASE_MERGE);
+  FoldArgs fa{};
+  fa.skeys = s.skeys;
+  fa.sidx = s.sidx;
+  fa.vals = vals;
+  fa.ops = ops;
+  fa.n = n;
+  fa.out_vals = out_vals;
+  fa.out_found = out_found;
+  fa.ukey = s.ukey;
+  fa.uval = s.uval;
+  fa.uop = s.uop;
+  fa.gstart = s.gstart;
+  fa.npend = s.npend;
+  fa.ulist = s.ulist;
+  fa.counters = s.counters;
+  fa.tile_scratch = s.fold_tiles;
+  HIP_TRY(fold_batch(fa, st));
+  phase_end(t, STM_PHASE_MERGE);
+
+  phase_begin(t, STM_PHASE_LOOKUP);
+  LookupArgs la{};
+  la.ukey = s.ukey;
+  la.uop = s.uop;
+  la.gstart = s.gstart;
+  la.npend = s.npend;
+  la.sidx = s.sidx;
+  la.counters = s.counters;
+  la.n_max = n;
+  la.chunk = t->cfg.ops_per_group;
+  la.lanes = t->cfg.lanes_per_op;
+  la.out_vals = out_vals;
+  la.out_found = out_found;
+  HIP_TRY(launch_lookup(t->view, la, st));
+  phase_end(t, STM_PHASE_LOOKUP);
+
+  phase_begin(t, STM_PHASE_UPSERT);
+  UpsertArgs ua{};
+  ua.ukey = s.ukey;
+  ua.uval = s.uval;
+  ua.gstart = s.gstart;
+  ua.npend = s.npend;
+  ua.sidx = s.sidx;
+  ua.ulist = s.ulist;
+  ua.counters = s.counters;
+  ua.dlist = s.dlist;
+  ua.max_attempts = t->cfg.max_attempts;
+  ua.per_wave = t->cfg.upsert_ops_per_wave;
+  // Enough wavefronts to cover every CU several times; each drains the shared queue.
+  const uint64_t waves_needed = (n + ua.per_wave - 1) / ua.per_wave;
+  ua.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves_needed + 3) / 4, 2048));
+  ua.out_vals = out_vals;
+  ua.out_found = out_found;
+  HIP_TRY(launch_upsert(t->view, ua, st));
+  phase_end(t, STM_PHASE_UPSERT);
+
+  BatchCounters bc;
+  HIP_TRY(hipMemcpyAsync(&bc, s.counters, sizeof(bc), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  phase_collect(t);
+  t->st.batches += 1;
+  t->st.ops += n;
+  t->st.unique_ops += bc.n_unique;
+  if (bc.n_bad) return fail(STM_ERR_INVALID, "batch holds the reserved key 0xffffffffffffffff");
+  return check_device_error(t);
+}
+
+int validate_host_batch(const uint64_t* keys, const uint8_t* ops, size_t n, bool need_vals,
+                        const uint64_t* vals) {
+  bool any_insert = false;
+  for (size_t i = 0; i < n; ++i) {
+    if (keys[i] == STM_KEY_RESERVED)
+      return fail(STM_ERR_INVALID, "op %zu uses the reserved key 0xffffffffffffffff", i);
+    if (ops) {
+      if (ops[i] > STM_OP_RANGE) return fail(STM_ERR_INVALID, "op %zu has op code %u", i, ops[i]);
+      any_insert |= ops[i] == STM_OP_INSERT;
+    }
+  }
+  if (need_vals && (any_insert || !ops) && !vals)
+    return fail(STM_ERR_INVALID, "inserts need a value array");
+  return STM_OK;
+}
+
+int run_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+               size_t n, uint64_t* out_vals, uint8_t* out_found) {
+  const uint64_t mb = t->cfg.max_batch;
+  for (size_t off = 0; off < n; off += mb) {
+    const uint32_t m = (uint32_t)std::min<uint64_t>(mb, n - off);
+    int rc = run_chunk(t, keys + off, vals ? vals + off : nullptr, ops + off, m, out_vals + off,
+                       out_found + off);
+    if (rc) return rc;
+  }
+  return STM_OK;
+}
+
+// Bottom-up build of an empty tree from n device-resident pairs.
+int build_from_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, size_t n) {
+  hipStream_t st = t->stream;
+  const uint32_t fill = t->cfg.leaf_fill;
+  if (n > 0x7fffffffull) return fail(STM_ERR_INVALID, "bulk load of %zu keys exceeds 2^31-1", n);
+  Scratch tmp;
+  hipError_t e = tmp.alloc(std::max<uint64_t>(n, 1), false);
+  if (e != hipSuccess) {
+    tmp.release();
+    return fail(STM_ERR_NOMEM, "bulk-load scratch for %zu keys: %s", n, hipGetErrorString(e));
+  }
+  int rc = STM_OK;
+  do {
+    uint32_t U = 0;
+    if (n > 0) {
+      hipError_t r1 = radix_sort_pairs(keys, tmp.skeys_tmp, tmp.skeys, tmp.sidx_tmp, tmp.sidx,
+                                       (uint32_t)n, 0, 64, tmp.hist, st);
+      if (r1 != hipSuccess) { rc = fail(STM_ERR_DEVICE, "sort: %s", hipGetErrorString(r1)); break; }
+      FoldArgs fa{};
+      fa.skeys = tmp.skeys;
+      fa.sidx = tmp.sidx;
+      fa.vals = vals;
+      fa.ops = nullptr;
+      fa.n = (uint32_t)n;
+      fa.ukey = tmp.ukey;
+      fa.uval = tmp.uval;
+      fa.uop = tmp.uop;
+      fa.counters = tmp.counters;
+      fa.tile_scratch = tmp.fold_tiles;
+      r1 = fold_batch(fa, st);
+      if (r1 != hipSuccess) { rc = fail(STM_ERR_DEVICE, "fold: %s", hipGetErrorString(r1)); break; }
+      BatchCounters bc;
+      if (hipMemcpyAsync(&bc, tmp.counters, sizeof(bc), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        rc = fail(STM_ERR_DEVICE, "bulk-load counters");
+        break;
+      }
+      if (bc.n_bad) { rc = fail(STM_ERR_INVALID, "bulk load holds the reserved key"); break; }
+      U = bc.n_unique;
+    }
+    // Level sizes.
+    std::vector<uint32_t> level_n;
+    uint64_t L = U == 0 ? 1 : (U + fill - 1) / fill;
+    level_n.push_back((uint32_t)L);
+    while (level_n.back() > 1) level_n.push_back((level_n.back() + fill - 1) / fill);
+    uint64_t total = 0;
+    for (uint32_t c : level_n) total += c;
+    if (total > t->capacity) {
+      rc = fail(STM_ERR_NOMEM, "bulk load needs %llu nodes, pool holds %llu; raise max_keys",
+                (unsigned long long)total, (unsigned long long)t->capacity);
+      break;
+    }
+    uint64_t *lows_a = nullptr, *lows_b = nullptr;
+    if (dalloc(&lows_a, L) != hipSuccess || dalloc(&lows_b, L) != hipSuccess) {
+      if (lows_a) (void)hipFree(lows_a);
+      rc = fail(STM_ERR_NOMEM, "bulk-load fence arrays");
+      break;
+    }
+    hipError_t r2 = hipMemsetAsync(t->view.meta, 0, sizeof(NodeMeta) * t->capacity, st);
+    if (r2 == hipSuccess)
+      r2 = launch_build_leaves(t->view, tmp.ukey, tmp.uval, U, fill, (uint32_t)L, lows_a, st);
+    uint32_t first_prev = 0, first_this = (uint32_t)L;
+    uint64_t* lp = lows_a;
+    uint64_t* lt = lows_b;
+    for (size_t lv = 1; lv < level_n.size() && r2 == hipSuccess; ++lv) {
+      r2 = launch_build_inner(t->view, lp, level_n[lv - 1], first_prev, fill, level_n[lv], first_this,
+                              (uint32_t)lv, lt, st);
+      first_prev = first_this;
+      first_this += level_n[lv];
+      std::swap(lp, lt);
+    }
+    if (r2 == hipSuccess) {
+      TreeHeader h{};
+      h.lock = 0;
+      h.root = first_prev + level_n.back() - 1;
+      if (level_n.size() == 1) h.root = 0;
+      h.height = level_n.size();
+      h.n_nodes = total;
+      h.capacity = t->capacity;
+      h.error = 0;
+      h.n_keys = U;
+      r2 = hipMemcpyAsync(t->view.hdr, &h, sizeof(h), hipMemcpyHostToDevice, st);
+      if (r2 == hipSuccess) r2 = hipStreamSynchronize(st);
+    }
+    (void)hipFree(lows_a);
+    (void)hipFree(lows_b);
+    if (r2 != hipSuccess) rc = fail(STM_ERR_DEVICE, "bulk build: %s", hipGetErrorString(r2));
+  } while (0);
+  (void)hipStreamSynchronize(st);
+  tmp.release();
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* stm_last_error(void) { return g_last_error.c_str(); }
+
+int stm_config_init(stm_config* cfg) {
+  if (!cfg) return fail(STM_ERR_INVALID, "null config");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->fanout = 64;
+  cfg->leaf_fill = 0;
+  cfg->max_keys = 1ull << 24;
+  cfg->max_batch = 1ull << 22;
+  cfg->device = 0;
+  cfg->lanes_per_op = 16;
+  cfg->ops_per_group = 16;
+  cfg->upsert_ops_per_wave = 1;
+  cfg->max_attempts = 64;
+  cfg->route = STM_ROUTE_DESCENT;
+  cfg->timing = 0;
+  return STM_OK;
+}
+
+int stm_create(const stm_config* cfg_in, stm_tree** out) {
+  if (!out) return fail(STM_ERR_INVALID, "null output handle");
+  *out = nullptr;
+  stm_config cfg;
+  if (cfg_in) cfg = *cfg_in;
+  else stm_config_init(&cfg);
+  if (cfg.fanout != 32 && cfg.fanout != 64 && cfg.fanout != 128)
+    return fail(STM_ERR_INVALID, "fanout %u not in {32, 64, 128}", cfg.fanout);
+  if (cfg.leaf_fill == 0) cfg.leaf_fill = cfg.fanout - cfg.fanout / 4;
+  if (cfg.leaf_fill < 2 || cfg.leaf_fill > cfg.fanout)
+    return fail(STM_ERR_INVALID, "leaf_fill %u not in [2, fanout]", cfg.leaf_fill);
+  if (cfg.max_batch == 0 || cfg.max_batch > (1ull << 30))
+    return fail(STM_ERR_INVALID, "max_batch %llu not in [1, 2^30]", (unsigned long long)cfg.max_batch);
+  if (cfg.max_keys == 0) return fail(STM_ERR_INVALID, "max_keys must be positive");
+  const uint32_t lp = cfg.lanes_per_op;
+  if (!(lp == 8 || lp == 16 || lp == 32 || lp == 64) || lp > cfg.fanout)
+    return fail(STM_ERR_INVALID, "lanes_per_op %u not in {8,16,32,64} or above fanout", lp);
+  if (cfg.ops_per_group == 0) cfg.ops_per_group = 16;
+  if (cfg.upsert_ops_per_wave == 0) cfg.upsert_ops_per_wave = 1;
+  if (cfg.max_attempts == 0) cfg.max_attempts = 64;
+  if (cfg.route > STM_ROUTE_LEARNED) return fail(STM_ERR_INVALID, "route %u unknown", cfg.route);
+  const uint64_t cap = capacity_for(cfg.max_keys, cfg.fanout);
+  if (cap >= NODE_NONE) return fail(STM_ERR_INVALID, "max_keys too large for 32-bit node ids");
+
+  stm_tree* t = new stm_tree();
+  t->cfg = cfg;
+  t->capacity = cap;
+  int rc = set_device(t);
+  if (rc) {
+    delete t;
+    return rc;
+  }
+  auto bail = [&](int code) {
+    stm_destroy(t);
+    return code;
+  };
+  if (hipStreamCreate(&t->stream) != hipSuccess) return bail(fail(STM_ERR_DEVICE, "hipStreamCreate"));
+  const uint32_t F = cfg.fanout;
+  hipError_t e = hipSuccess;
+  if ((e = dalloc(&t->view.keys, cap * F)) != hipSuccess ||
+      (e = dalloc(&t->view.vals, cap * F)) != hipSuccess ||
+      (e = dalloc(&t->view.meta, cap)) != hipSuccess ||
+      (e = dalloc(&t->view.hdr, 1)) != hipSuccess ||
+      (e = dalloc(&t->view.stats, (size_t)ST_NUM)) != hipSuccess)
+    return bail(fail(STM_ERR_NOMEM, "node pool of %llu nodes: %s", (unsigned long long)cap,
+                     hipGetErrorString(e)));
+  t->view.capacity = (uint32_t)cap;
+  t->view.fanout = F;
+  if ((e = t->scr.alloc(cfg.max_batch, true)) != hipSuccess)
+    return bail(fail(STM_ERR_NOMEM, "batch scratch for %llu ops: %s",
+                     (unsigned long long)cfg.max_batch, hipGetErrorString(e)));
+  if (hipMemsetAsync(t->view.meta, 0, sizeof(NodeMeta) * cap, t->stream) != hipSuccess ||
+      hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * ST_NUM, t->stream) != hipSuccess)
+    return bail(fail(STM_ERR_DEVICE, "pool init"));
+  for (int p = 0; p < STM_NUM_PHASES; ++p) {
+    if (hipEventCreate(&t->ev_a[p]) != hipSuccess || hipEventCreate(&t->ev_b[p]) != hipSuccess)
+      return bail(fail(STM_ERR_DEVICE, "hipEventCreate"));
+  }
+  rc = init_empty(t);
+  if (rc) return bail(rc);
+  *out = t;
+  return STM_OK;
+}
+
+int stm_destroy(stm_tree* t) {
+  if (!t) return STM_OK;
+  (void)hipSetDevice(t->cfg.device);
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  t->scr.release();
+  void* ps[] = {t->view.keys, t->view.vals, t->view.meta, t->view.hdr, t->view.stats};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  for (int p = 0; p < STM_NUM_PHASES; ++p) {
+    if (t->ev_a[p]) (void)hipEventDestroy(t->ev_a[p]);
+    if (t->ev_b[p]) (void)hipEventDestroy(t->ev_b[p]);
+  }
+  if (t->stream) (void)hipStreamDestroy(t->stream);
+  delete t;
+  return STM_OK;
+}
+
+int stm_size(stm_tree* t, uint64_t* n_keys) {
+  if (!t || !n_keys) return fail(STM_ERR_INVALID, "null argument");
+  int rc = set_device(t);
+  if (rc) return rc;
+  TreeHeader h;
+  rc = read_header(t, &h);
+  if (rc) return rc;
+  *n_keys = h.n_keys;
+  return STM_OK;
+}
+
+int stm_bulk_load_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_vals, size_t n) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n && (!d_keys || !d_vals)) return fail(STM_ERR_INVALID, "null key or value array");
+  int rc = set_device(t);
+  if (rc) return rc;
+  uint64_t have = 0;
+  rc = stm_size(t, &have);
+  if (rc) return rc;
+  if (have == 0) return build_from_device(t, d_keys, d_vals, n);
+  // Non-empty tree: a batch of inserts, outputs discarded into the staging buffers.
+  const uint64_t mb = t->cfg.max_batch;
+  HIP_TRY(hipMemsetAsync(t->scr.in_ops, STM_OP_INSERT, mb, t->stream));
+  for (size_t off = 0; off < n; off += mb) {
+    const uint32_t m = (uint32_t)std::min<uint64_t>(mb, n - off);
+    rc = run_chunk(t, d_keys + off, d_vals + off, t->scr.in_ops, m, t->scr.out_vals, t->scr.out_found);
+    if (rc) return rc;
+  }
+  return STM_OK;
+}
+
+int stm_bulk_load(stm_tree* t, const uint64_t* keys, const uint64_t* vals, size_t n) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n && (!keys || !vals)) return fail(STM_ERR_INVALID, "null key or value array");
+  int rc = set_device(t);
+  if (rc) return rc;
+  rc = validate_host_batch(keys, nullptr, n, false, vals);
+  if (rc) return rc;
+  uint64_t *dk = nullptr, *dv = nullptr;
+  if (dalloc(&dk, n) != hipSuccess || dalloc(&dv, n) != hipSuccess) {
+    if (dk) (void)hipFree(dk);
+    return fail(STM_ERR_NOMEM, "bulk-load input of %zu keys", n);
+  }
+  rc = STM_OK;
+  if (n) {
+    if (hipMemcpyAsync(dk, keys, n * 8, hipMemcpyHostToDevice, t->stream) != hipSuccess ||
+        hipMemcpyAsync(dv, vals, n * 8, hipMemcpyHostToDevice, t->stream) != hipSuccess)
+      rc = fail(STM_ERR_DEVICE, "bulk-load copy");
+  }
+  if (!rc) rc = stm_bulk_load_device(t, dk, dv, n);
+  (void)hipStreamSynchronize(t->stream);
+  (void)hipFree(dk);
+  (void)hipFree(dv);
+  return rc;
+}
+
+int stm_run_batch_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_vals,
+                         const uint8_t* d_ops, size_t n, uint64_t* d_out_vals, uint8_t* d_out_found) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n == 0) return STM_OK;
+  if (!d_keys || !d_vals || !d_ops || !d_out_vals || !d_out_found)
+    return fail(STM_ERR_INVALID, "null device array (device batches always carry values)");
+  int rc = set_device(t);
+  if (rc) return rc;
+  return run_device(t, d_keys, d_vals, d_ops, n, d_out_vals, d_out_found);
+}
+
+int stm_run_batch(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+                  size_t n, uint64_t* out_vals, uint8_t* out_found) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n == 0) return STM_OK;
+  if (!keys || !ops || !out_vals || !out_found) return fail(STM_ERR_INVALID, "null array");
+  int rc = set_device(t);
+  if (rc) return rc;
+  rc = validate_host_batch(keys, ops, n, true, vals);
+  if (rc) return rc;
+  Scratch& s = t->scr;
+  const uint64_t mb = t->cfg.max_batch;
+  for (size_t off = 0; off < n; off += mb) {
+    const size_t m = (size_t)std::min<uint64_t>(mb, n - off);
+    HIP_TRY(hipMemcpyAsync(s.in_keys, keys + off, m * 8, hipMemcpyHostToDevice, t->stream));
+    if (vals) HIP_TRY(hipMemcpyAsync(s.in_vals, vals + off, m * 8, hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(hipMemcpyAsync(s.in_ops, ops + off, m, hipMemcpyHostToDevice, t->stream));
+    rc = run_chunk(t, s.in_keys, vals ? s.in_vals : nullptr, s.in_ops, (uint32_t)m, s.out_vals,
+                   s.out_found);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out_vals + off, s.out_vals, m * 8, hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipMemcpyAsync(out_found + off, s.out_found, m, hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+  }
+  return STM_OK;
+}
+
+int stm_dump(stm_tree* t, uint64_t* keys, uint64_t* vals, size_t cap, size_t* n) {
+  if (!t || !n) return fail(STM_ERR_INVALID, "null argument");
+  int rc = set_device(t);
+  if (rc) return rc;
+  TreeHeader h;
+  rc = read_header(t, &h);
+  if (rc) return rc;
+  const uint32_t nn = (uint32_t)std::min<uint64_t>(h.n_nodes, t->capacity);
+  hipStream_t st = t->stream;
+  uint64_t *lows = nullptr, *slows = nullptr, *slows_tmp = nullptr, *offs = nullptr;
+  uint32_t *ids = nullptr, *order = nullptr, *order_tmp = nullptr, *sorted_ids = nullptr,
+           *counts = nullptr, *counter = nullptr, *hist = nullptr;
+  uint64_t *ok = nullptr, *ov = nullptr;
+  auto cleanup = [&]() {
+    void* ps[] = {lows, slows, slows_tmp, offs, ids, order, order_tmp, sorted_ids, counts, counter,
+                  hist, ok, ov};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  };
+  auto err = [&](int code, const char* what) {
+    (void)hipStreamSynchronize(st);
+    cleanup();
+    return fail(code, "dump: %s", what);
+  };
+  if (dalloc(&lows, nn) || dalloc(&slows, nn) || dalloc(&slows_tmp, nn) || dalloc(&ids, nn) ||
+      dalloc(&order, nn) || dalloc(&order_tmp, nn) || dalloc(&sorted_ids, nn) ||
+      dalloc(&counts, nn) || dalloc(&counter, 1) || dalloc(&offs, (size_t)nn + 1) ||
+      dalloc(&hist, sort_hist_words(nn)))
+    return err(STM_ERR_NOMEM, "scratch");
+  if (hipMemsetAsync(counter, 0, 4, st) != hipSuccess) return err(STM_ERR_DEVICE, "memset");
+  if (launch_collect_leaves(t->view, nn, lows, ids, counter, st) != hipSuccess)
+    return err(STM_ERR_DEVICE, "collect");
+  uint32_t n_leaves = 0;
+  if (hipMemcpyAsync(&n_leaves, counter, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return err(STM_ERR_DEVICE, "leaf count");
+  if (radix_sort_pairs(lows, slows_tmp, slows, order_tmp, order, n_leaves, 0, 64, hist, st) != hipSuccess ||
+      launch_leaf_counts(t->view, order, ids, n_leaves, sorted_ids, counts, st) != hipSuccess)
+    return err(STM_ERR_DEVICE, "leaf order");
+  std::vector<uint32_t> hc(n_leaves);
+  if (n_leaves && (hipMemcpyAsync(hc.data(), counts, 4ull * n_leaves, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                   hipStreamSynchronize(st) != hipSuccess))
+    return err(STM_ERR_DEVICE, "counts");
+  std::vector<uint64_t> ho(n_leaves + 1, 0);
+  for (uint32_t r = 0; r < n_leaves; ++r) ho[r + 1] = ho[r] + hc[r];
+  const uint64_t total = ho[n_leaves];
+  *n = total;
+  if (!keys || !vals || cap < total) {
+    (void)hipStreamSynchronize(st);
+    cleanup();
+    if (!keys && !vals && cap == 0) return STM_OK;  // size query
+    return fail(STM_ERR_INVALID, "dump needs room for %llu pairs, got %zu", (unsigned long long)total, cap);
+  }
+  if (dalloc(&ok, total) || dalloc(&ov, total)) return err(STM_ERR_NOMEM, "output");
+  if (hipMemcpyAsync(offs, ho.data(), 8ull * (n_leaves + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+      launch_dump_copy(t->view, sorted_ids, offs, n_leaves, ok, ov, st) != hipSuccess)
+    return err(STM_ERR_DEVICE, "copy");
+  if (total && (hipMemcpyAsync(keys, ok, total * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(vals, ov, total * 8, hipMemcpyDeviceToHost, st) != hipSuccess))
+    return err(STM_ERR_DEVICE, "download");
+  if (hipStreamSynchronize(st) != hipSuccess) return err(STM_ERR_DEVICE, "sync");
+  cleanup();
+  return STM_OK;
+}
+
+int stm_check(stm_tree* t, uint64_t* bad) {
+  if (!t || !bad) return fail(STM_ERR_INVALID, "null argument");
+  int rc = set_device(t);
+  if (rc) return rc;
+  TreeHeader h;
+  rc = read_header(t, &h);
+  if (rc) return rc;
+  unsigned long long* d = nullptr;
+  HIP_TRY(dalloc(&d, 1));
+  unsigned long long v = 0;
+  hipError_t e = hipMemsetAsync(d, 0, 8, t->stream);
+  const uint32_t nn = (uint32_t)std::min<uint64_t>(h.n_nodes, t->capacity);
+  if (e == hipSuccess) e = launch_check(t->view, nn, d, t->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, t->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(STM_ERR_DEVICE, "check: %s", hipGetErrorString(e));
+  // Root and height must describe a node of the right level.
+  NodeMeta rm;
+  HIP_TRY(hipMemcpy(&rm, t->view.meta + h.root, sizeof(rm), hipMemcpyDeviceToHost));
+  if (meta_level(rm.info) + 1 != h.height || rm.low != 0 || rm.high != KEY_EMPTY) v++;
+  *bad = v;
+  return STM_OK;
+}
+
+int stm_get_stats(stm_tree* t, stm_stats* out) {
+  if (!t || !out) return fail(STM_ERR_INVALID, "null argument");
+  int rc = set_device(t);
+  if (rc) return rc;
+  TreeHeader h;
+  rc = read_header(t, &h);
+  if (rc) return rc;
+  unsigned long long c[ST_NUM];
+  HIP_TRY(hipMemcpy(c, t->view.stats, sizeof(c), hipMemcpyDeviceToHost));
+  *out = t->st;
+  out->n_keys = h.n_keys;
+  out->n_nodes = std::min<uint64_t>(h.n_nodes, t->capacity);
+  out->capacity_nodes = t->capacity;
+  out->height = h.height;
+  out->upserts = c[ST_UPSERTS];
+  out->commits = c[ST_COMMITS];
+  out->aborts = c[ST_ABORTS];
+  out->leaf_splits = c[ST_LEAF_SPLITS];
+  out->inner_splits = c[ST_INNER_SPLITS];
+  out->root_splits = c[ST_ROOT_SPLITS];
+  out->deferred = c[ST_DEFERRED];
+  out->route_hits = c[ST_ROUTE_HITS];
+  out->route_misses = c[ST_ROUTE_MISSES];
+  return STM_OK;
+}
+
+int stm_reset_stats(stm_tree* t) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  int rc = set_device(t);
+  if (rc) return rc;
+  t->st = stm_stats{};
+  HIP_TRY(hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * ST_NUM, t->stream));
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  return STM_OK;
+}
+
+int stm_get_stream(stm_tree* t, void** stream) {
+  if (!t || !stream) return fail(STM_ERR_INVALID, "null argument");
+  *stream = (void*)t->stream;
+  return STM_OK;
+}
+
+}  // extern "C"

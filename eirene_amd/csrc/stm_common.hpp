@@ -1,0 +1,145 @@
+// stm_common.hpp — HBM layout of the tree and CDNA4 wavefront helpers shared by all kernels.
+//
+// Node pool (structure of arrays, node id = index):
+//   keys[id * F + s]  u64 slot keys, sorted, empty slots hold KEY_EMPTY
+//   vals[id * F + s]  u64 leaf values, or child node ids in inner nodes
+//   meta[id]          64 B: version/lock word, count|level, low/high fences, right sibling
+// Inner nodes use the min-key convention: keys[s] is the low fence of child vals[s], so
+// keys[0] equals the node's own low fence and the child for key k is the last slot with
+// keys[s] <= k.  Leaves hold (keys[s], vals[s]) pairs.  Every node covers [low, high); the
+// high fence of a node equals the low fence of its right sibling (KEY_EMPTY = +inf for
+// the rightmost node of a level).  The reference keeps the node header in key slot 31 and
+// the leaf's right fence in value slot 31 (src/gpu-stm-tree-v38/bpt-gpustm-kernel-tree.cuh:25-56,
+// 139-141); here they live in the separate 64-byte meta record so every key slot is usable.
+//
+// Lock word (reference: 32-bit version_t with bit 0 = lock, src/gpu-stm-tree-v38/bpt-gpustm-kernel-tx.cuh:16-18):
+//   64-bit, bit 0 = locked, bits 63..1 = version.  Unlock stores word + 1 (version + 1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define STM_DEV __device__ __forceinline__
+
+namespace stm {
+
+constexpr uint64_t KEY_EMPTY = ~0ull;
+constexpr uint32_t NODE_NONE = 0xffffffffu;
+constexpr uint8_t OP_INSERT = 0;
+constexpr uint8_t OP_GET = 1;
+constexpr int WAVE = 64;
+
+struct alignas(64) NodeMeta {
+  uint64_t lock;   // bit 0 locked, version above
+  uint64_t info;   // count (bits 0..31) | level (bits 32..63); level 0 = leaf
+  uint64_t low;    // inclusive low fence
+  uint64_t high;   // exclusive high fence, KEY_EMPTY = +inf
+  uint64_t next;   // right sibling id at the same level, NODE_NONE at the right edge
+  uint64_t pad[3];
+};
+static_assert(sizeof(NodeMeta) == 64, "meta record must be one 64-byte sector");
+
+struct alignas(64) TreeHeader {
+  uint64_t lock;      // version/lock word guarding root and height
+  uint64_t root;      // root node id
+  uint64_t height;    // number of levels, 1 = the root is a leaf
+  uint64_t n_nodes;   // bump allocator
+  uint64_t capacity;  // node-pool capacity
+  uint64_t error;     // sticky error bits (ERR_*) raised by kernels
+  uint64_t n_keys;    // keys in the tree (maintained by the batch pipeline)
+  uint64_t pad;
+};
+
+// Sticky error bits in TreeHeader::error.
+constexpr uint64_t ERR_POOL_FULL = 1;     // an insert needed a node beyond capacity
+constexpr uint64_t ERR_LIVELOCK = 2;      // the serial pass could not commit an op
+constexpr uint64_t ERR_BAD_KEY = 4;       // a batch carried the reserved key
+constexpr uint64_t ERR_BAD_OP = 8;        // a batch carried an op code > 3
+
+// Device counters (u64 each) accumulated by kernels.
+enum StatSlot : int {
+  ST_COMMITS = 0,
+  ST_ABORTS,
+  ST_LEAF_SPLITS,
+  ST_INNER_SPLITS,
+  ST_ROOT_SPLITS,
+  ST_DEFERRED,
+  ST_NEW_KEYS,
+  ST_UPSERTS,
+  ST_ROUTE_HITS,
+  ST_ROUTE_MISSES,
+  ST_NUM
+};
+
+struct TreeView {
+  uint64_t* keys;
+  uint64_t* vals;
+  NodeMeta* meta;
+  TreeHeader* hdr;
+  unsigned long long* stats;
+  uint32_t capacity;
+  uint32_t fanout;
+};
+
+// ---------------------------------------------------------------- memory-model helpers
+// Agent-scope atomics: the node pool is written by transactions on any XCD while other
+// XCDs read it, so the transactional path uses the HIP/LLVM agent-scope memory model
+// (relaxed atomic loads/stores of 64-bit words, acquire/release fences around the
+// version checks).  The read-only lookup kernel runs when no writer is active and uses
+// plain loads; kernel boundaries order it against the transactional kernels.
+STM_DEV uint64_t ld_acquire(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+STM_DEV uint64_t ld_relaxed(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+STM_DEV void st_relaxed(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+STM_DEV void st_release(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+STM_DEV bool cas_u64(uint64_t* p, uint64_t expect, uint64_t desired) {
+  return __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+STM_DEV void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+STM_DEV void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
+// ---------------------------------------------------------------- wavefront helpers
+STM_DEV int lane_id() { return __lane_id(); }
+STM_DEV uint64_t ballot(bool p) { return __ballot(p); }
+STM_DEV uint32_t uniform32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+STM_DEV uint64_t uniform64(uint64_t v) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+STM_DEV uint64_t shfl64(uint64_t v, int src) {
+  // ds_bpermute moves 32 bits; two of them move a u64.
+  int lo = __shfl((int)(uint32_t)v, src, WAVE);
+  int hi = __shfl((int)(uint32_t)(v >> 32), src, WAVE);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+STM_DEV uint64_t readlane64(uint64_t v, int src) {
+  uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, src);
+  uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+STM_DEV uint64_t lanemask_lt() {
+  int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__host__ __device__ inline uint32_t meta_count(uint64_t info) { return (uint32_t)info; }
+__host__ __device__ inline uint32_t meta_level(uint64_t info) { return (uint32_t)(info >> 32); }
+__host__ __device__ inline uint64_t make_info(uint32_t count, uint32_t level) {
+  return (uint64_t)count | ((uint64_t)level << 32);
+}
+
+// Wave-aggregated counter bump (one atomic per wave).
+STM_DEV void stat_add(unsigned long long* stats, int slot, uint64_t v) {
+  const uint64_t exec = __builtin_amdgcn_read_exec();
+  if (lane_id() == __builtin_ctzll(exec) && v) atomicAdd(stats + slot, (unsigned long long)v);
+}
+
+}  // namespace stm

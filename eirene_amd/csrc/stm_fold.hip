@@ -1,0 +1,221 @@
+// stm_fold.hip — per-key fold of a sorted batch into one merged op per unique key.
+//
+// Reference pipeline (src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:289-332):
+//   cub::DeviceRunLengthEncode::Encode (:296) -> run count copied to the host (:303) ->
+//   cub::DeviceScan::ExclusiveSum (:305) -> mergeMission (bpt-gpustm-kernel-aux.cuh:16-44)
+//   -> tree kernel -> writebackAns (bpt-gpustm-kernel-aux.cuh:46-56).
+// mergeMission walks each key group in input order: an op is answered by the value of
+// the latest earlier INSERT of the group, otherwise by the tree; the merged op is
+// INSERT(last inserted value) if the group has an insert, GET otherwise.
+//
+// Here one segmented scan replaces RLE + scan + the thread-per-group loop, so skewed
+// (Zipf) groups are spread over many threads and no run count travels to the host.
+// The scan state of a prefix is (has_head, last head position, first and last insert
+// position since that head, number of heads).  Three kernels: tile reduce, scan of the
+// tile aggregates, apply.  Ops that the fold answers (an earlier insert of the same key
+// exists) are written to the outputs here; the first op of each group up to and
+// including its first insert waits for the tree ("npend" ops from "gstart").
+#include "stm_common.hpp"
+#include "stm_internal.hpp"
+
+namespace stm {
+
+constexpr int FOLD_BLOCK = 256;
+constexpr int FOLD_ITEMS = 16;
+constexpr int FOLD_TILE = FOLD_BLOCK * FOLD_ITEMS;
+constexpr int FOLD_WAVES = FOLD_BLOCK / WAVE;
+
+struct Seg {
+  int32_t h;      // 1 if the range contains a group head
+  int32_t head;   // position of the last head in the range (-1 if none)
+  int32_t first;  // first insert after the last head (or in the whole range if h == 0)
+  int32_t last;   // last insert after the last head (or in the whole range if h == 0)
+  uint32_t c;     // heads in the range
+};
+
+STM_DEV Seg seg_identity() { return Seg{0, -1, -1, -1, 0u}; }
+
+STM_DEV Seg seg_combine(const Seg& a, const Seg& b) {
+  Seg r;
+  r.h = a.h | b.h;
+  r.head = b.h ? b.head : a.head;
+  r.first = b.h ? b.first : (a.first >= 0 ? a.first : b.first);
+  r.last = b.h ? b.last : max(a.last, b.last);
+  r.c = a.c + b.c;
+  return r;
+}
+
+STM_DEV Seg seg_shfl_up(const Seg& s, int off) {
+  Seg r;
+  r.h = __shfl_up(s.h, off, WAVE);
+  r.head = __shfl_up(s.head, off, WAVE);
+  r.first = __shfl_up(s.first, off, WAVE);
+  r.last = __shfl_up(s.last, off, WAVE);
+  r.c = __shfl_up(s.c, off, WAVE);
+  return r;
+}
+
+// Exclusive scan of one Seg per thread over the block; also returns the block total.
+STM_DEV Seg block_exclusive_seg(const Seg& v, Seg* s_w /*[FOLD_WAVES]*/, Seg& total) {
+  const int lane = lane_id();
+  const int wave = threadIdx.x / WAVE;
+  Seg x = v;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    Seg y = seg_shfl_up(x, off);
+    if (lane >= off) x = seg_combine(y, x);
+  }
+  Seg ex = seg_shfl_up(x, 1);
+  if (lane == 0) ex = seg_identity();
+  if (lane == WAVE - 1) s_w[wave] = x;
+  __syncthreads();
+  Seg wp = seg_identity();
+  total = seg_identity();
+#pragma unroll
+  for (int w = 0; w < FOLD_WAVES; ++w) {
+    Seg t = s_w[w];
+    if (w < wave) wp = seg_combine(wp, t);
+    total = seg_combine(total, t);
+  }
+  __syncthreads();
+  return seg_combine(wp, ex);
+}
+
+STM_DEV Seg seg_element(const FoldArgs& a, uint32_t i, uint64_t key, uint64_t prev_key) {
+  const bool head = (i == 0) || key != prev_key;
+  const bool ins = a.ops ? (a.ops[a.sidx[i]] == OP_INSERT) : true;
+  Seg e;
+  e.h = head ? 1 : 0;
+  e.head = head ? (int32_t)i : -1;
+  e.first = ins ? (int32_t)i : -1;
+  e.last = e.first;
+  e.c = head ? 1u : 0u;
+  return e;
+}
+
+// Aggregate of the FOLD_ITEMS consecutive elements owned by this thread.
+STM_DEV Seg thread_aggregate(const FoldArgs& a, uint32_t i0) {
+  Seg s = seg_identity();
+  if (i0 >= a.n) return s;
+  uint64_t prev = (i0 > 0) ? a.skeys[i0 - 1] : 0;
+  const uint32_t i1 = min(i0 + FOLD_ITEMS, a.n);
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint64_t k = a.skeys[i];
+    s = seg_combine(s, seg_element(a, i, k, prev));
+    prev = k;
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_reduce(FoldArgs a, Seg* tile_agg) {
+  __shared__ Seg s_w[FOLD_WAVES];
+  const uint32_t i0 = blockIdx.x * FOLD_TILE + threadIdx.x * FOLD_ITEMS;
+  Seg t = thread_aggregate(a, i0);
+  Seg total;
+  (void)block_exclusive_seg(t, s_w, total);
+  if (threadIdx.x == 0) tile_agg[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, uint32_t ntiles,
+                                                                BatchCounters* counters) {
+  __shared__ Seg s_w[FOLD_WAVES];
+  const uint32_t per = (ntiles + FOLD_BLOCK - 1) / FOLD_BLOCK;
+  const uint32_t b0 = threadIdx.x * per;
+  const uint32_t b1 = min(b0 + per, ntiles);
+  Seg s = seg_identity();
+  for (uint32_t b = b0; b < b1; ++b) s = seg_combine(s, tile_agg[b]);
+  Seg total;
+  Seg run = block_exclusive_seg(s, s_w, total);
+  for (uint32_t b = b0; b < b1; ++b) {
+    Seg t = tile_agg[b];
+    tile_agg[b] = run;  // becomes the exclusive prefix of tile b
+    run = seg_combine(run, t);
+  }
+  if (threadIdx.x == 0) {
+    counters->n_unique = total.c;
+    counters->n_upserts = 0;
+    counters->n_deferred = 0;
+    counters->queue_head = 0;
+    counters->n_bad = 0;
+  }
+}
+
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg* tile_prefix) {
+  __shared__ Seg s_w[FOLD_WAVES];
+  const uint32_t i0 = blockIdx.x * FOLD_TILE + threadIdx.x * FOLD_ITEMS;
+  Seg t = thread_aggregate(a, i0);
+  Seg total;
+  Seg bex = block_exclusive_seg(t, s_w, total);
+  if (blockIdx.x * FOLD_TILE >= a.n) return;  // whole block out of range (uniform)
+  Seg x = seg_combine(tile_prefix[blockIdx.x], bex);
+  uint64_t prev = (i0 > 0 && i0 < a.n) ? a.skeys[i0 - 1] : 0;
+  uint64_t k = (i0 < a.n) ? a.skeys[i0] : 0;
+  // Uniform trip count so the upsert-list append can be aggregated per wavefront.
+  for (uint32_t j = 0; j < FOLD_ITEMS; ++j) {
+    const uint32_t i = i0 + j;
+    const bool valid = i < a.n;
+    bool append = false;
+    uint32_t g = 0;
+    uint64_t next = k;
+    if (valid) {
+      next = (i + 1 < a.n) ? a.skeys[i + 1] : k;
+      const Seg e = seg_element(a, i, k, prev);
+      const int32_t last_before = e.h ? -1 : x.last;  // latest earlier insert of this key
+      if (a.out_vals && last_before >= 0) {
+        const uint32_t pos = a.sidx[i];
+        a.out_vals[pos] = a.vals[a.sidx[last_before]];
+        a.out_found[pos] = 1;
+      }
+      x = seg_combine(x, e);
+      const bool group_end = (i + 1 == a.n) || next != k;
+      if (group_end) {
+        g = x.c - 1;
+        a.ukey[g] = k;
+        if (a.gstart) {
+          a.gstart[g] = (uint32_t)x.head;
+          a.npend[g] = (uint32_t)((x.first >= 0 ? x.first : (int32_t)i) - x.head + 1);
+        }
+        uint8_t op;
+        if (k == KEY_EMPTY) {
+          op = UOP_SKIP;
+          atomicAdd(&a.counters->n_bad, (uint32_t)(i - x.head + 1));
+        } else if (x.last >= 0) {
+          op = OP_INSERT;
+          a.uval[g] = a.vals[a.sidx[x.last]];
+          append = a.ulist != nullptr;
+        } else {
+          op = OP_GET;
+        }
+        a.uop[g] = op;
+      }
+    }
+    const uint64_t want = ballot(append);
+    if (want) {
+      uint32_t base = 0;
+      const int leader = __ffsll((unsigned long long)want) - 1;
+      if (lane_id() == leader) base = atomicAdd(&a.counters->n_upserts, (uint32_t)__popcll(want));
+      base = __shfl(base, leader, WAVE);
+      if (append) a.ulist[base + (uint32_t)__popcll(want & lanemask_lt())] = g;
+    }
+    prev = k;
+    k = next;
+  }
+}
+
+size_t fold_scratch_bytes(uint64_t n) {
+  uint64_t ntiles = (n + FOLD_TILE - 1) / FOLD_TILE;
+  if (ntiles == 0) ntiles = 1;
+  return (size_t)ntiles * sizeof(Seg);
+}
+
+hipError_t fold_batch(const FoldArgs& a, hipStream_t stream) {
+  const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
+  if (a.n == 0) return hipSuccess;
+  Seg* tiles = (Seg*)a.tile_scratch;
+  k_fold_reduce<<<ntiles, FOLD_BLOCK, 0, stream>>>(a, tiles);
+  k_fold_scan_tiles<<<1, FOLD_BLOCK, 0, stream>>>(tiles, ntiles, a.counters);
+  k_fold_apply<<<ntiles, FOLD_BLOCK, 0, stream>>>(a, tiles);
+  return hipGetLastError();
+}
+
+}  // namespace stm

@@ -1,0 +1,51 @@
+// stm_internal.hpp — host-side interfaces shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "stm_common.hpp"
+
+namespace stm {
+
+// ------------------------------------------------------------------ radix sort (stm_sort.hip)
+// Words of scratch (u32) the sort needs for n elements.
+size_t sort_hist_words(uint64_t n);
+// Stable ascending sort of keys[begin_bit, end_bit) with the original index as payload.
+// keys_in is not modified; results land in keys_out / idx_out.
+hipError_t radix_sort_pairs(const uint64_t* keys_in, uint64_t* keys_tmp, uint64_t* keys_out,
+                            uint32_t* idx_tmp, uint32_t* idx_out, uint32_t n, int begin_bit,
+                            int end_bit, uint32_t* hist_words, hipStream_t stream);
+
+// ------------------------------------------------------------------ per-key fold (stm_fold.hip)
+// Per-batch counters written by the fold and the tree kernels (one 64-byte block).
+struct BatchCounters {
+  uint32_t n_unique;    // groups (unique keys)
+  uint32_t n_upserts;   // groups whose merged op is INSERT
+  uint32_t n_deferred;  // upserts moved to the serial pass
+  uint32_t queue_head;  // work-queue cursor of the upsert kernel
+  uint32_t n_bad;       // ops rejected (reserved key)
+  uint32_t pad[11];
+};
+
+struct FoldArgs {
+  const uint64_t* skeys;  // sorted keys
+  const uint32_t* sidx;   // original positions, stable
+  const uint64_t* vals;   // original values (may be null if no inserts)
+  const uint8_t* ops;     // original op codes, null = every op is an insert
+  uint32_t n;
+  // outputs
+  uint64_t* out_vals;     // per original op (may be null for bulk load)
+  uint8_t* out_found;
+  uint64_t* ukey;         // per group
+  uint64_t* uval;
+  uint8_t* uop;           // OP_GET, OP_INSERT or UOP_SKIP
+  uint32_t* gstart;       // sorted position of the group's first op
+  uint32_t* npend;        // ops answered by the tree (prefix of the group)
+  uint32_t* ulist;        // groups with merged INSERT (unordered)
+  BatchCounters* counters;
+  void* tile_scratch;     // fold_scratch_bytes(n)
+};
+constexpr uint8_t UOP_SKIP = 0xff;
+size_t fold_scratch_bytes(uint64_t n);
+hipError_t fold_batch(const FoldArgs& a, hipStream_t stream);
+
+}  // namespace stm

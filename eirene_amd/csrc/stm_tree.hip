@@ -1,0 +1,925 @@
+// stm_tree.hip — B+tree kernels for gfx950: bottom-up build, read-only batched lookup,
+// transactional upsert (STM), dump and structural check.
+//
+// Reference counterpart: tx_insert_kernel (src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:15-258)
+// with its device library (bpt-gpustm-kernel-tree.cuh, bpt-gpustm-kernel-tx.cuh).  The
+// reference runs every unique key of a batch as a transaction on one 32-lane warp.  Here
+// the merged ops of a batch are split by type (the merged ops touch distinct keys, so
+// they commute):
+//   * merged GETs run first in k_lookup on a tree no one writes, walking the sorted keys
+//     in chunks with the last inner node and leaf cached in registers;
+//   * merged INSERTs then run in k_upsert as transactions on 64-lane wavefronts:
+//     begin (root + version), optimistic reads validated by fences and lock versions,
+//     try-lock commit of the write set with 64-bit CAS, abort and retry with backoff,
+//     preemptive split of full inner nodes on the way down
+//     (reference: src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:162-181).
+#include "stm_common.hpp"
+#include "stm_internal.hpp"
+#include "stm_tree.hpp"
+
+namespace stm {
+
+// ================================================================== bulk build
+template <int F>
+__global__ void k_build_leaves(TreeView t, const uint64_t* __restrict__ ukey,
+                               const uint64_t* __restrict__ uval, uint32_t U, uint32_t fill,
+                               uint32_t L, uint64_t* __restrict__ lows_out) {
+  const uint64_t gi = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (gi >= (uint64_t)L * F) return;
+  const uint32_t leaf = (uint32_t)(gi / F);
+  const uint32_t s = (uint32_t)(gi % F);
+  const uint64_t first = (uint64_t)leaf * fill;
+  const uint64_t src = first + s;
+  const bool valid = s < fill && src < U;
+  t.keys[gi] = valid ? ukey[src] : KEY_EMPTY;
+  t.vals[gi] = valid ? uval[src] : 0;
+  if (s == 0) {
+    const uint32_t cnt = U > first ? (uint32_t)min<uint64_t>(fill, U - first) : 0u;
+    NodeMeta m{};
+    m.lock = 0;
+    m.info = make_info(cnt, 0);
+    m.low = leaf == 0 ? 0 : ukey[first];
+    m.high = (leaf + 1 < L) ? ukey[first + fill] : KEY_EMPTY;
+    m.next = (leaf + 1 < L) ? (uint64_t)(leaf + 1) : (uint64_t)NODE_NONE;
+    t.meta[leaf] = m;
+    lows_out[leaf] = m.low;
+  }
+}
+
+template <int F>
+__global__ void k_build_inner(TreeView t, const uint64_t* __restrict__ lows_prev, uint32_t n_prev,
+                              uint32_t first_prev, uint32_t fill, uint32_t n_this,
+                              uint32_t first_this, uint32_t level, uint64_t* __restrict__ lows_out) {
+  const uint64_t gi = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (gi >= (uint64_t)n_this * F) return;
+  const uint32_t j = (uint32_t)(gi / F);
+  const uint32_t s = (uint32_t)(gi % F);
+  const uint64_t c = (uint64_t)j * fill + s;
+  const bool valid = s < fill && c < n_prev;
+  const uint64_t node = (uint64_t)first_this + j;
+  t.keys[node * F + s] = valid ? lows_prev[c] : KEY_EMPTY;
+  t.vals[node * F + s] = valid ? (uint64_t)(first_prev + c) : 0;
+  if (s == 0) {
+    const uint64_t c0 = (uint64_t)j * fill;
+    NodeMeta m{};
+    m.lock = 0;
+    m.info = make_info((uint32_t)min<uint64_t>(fill, n_prev - c0), level);
+    m.low = lows_prev[c0];
+    m.high = (j + 1 < n_this) ? lows_prev[c0 + fill] : KEY_EMPTY;
+    m.next = (j + 1 < n_this) ? (uint64_t)(node + 1) : (uint64_t)NODE_NONE;
+    t.meta[node] = m;
+    lows_out[j] = m.low;
+  }
+}
+
+// ================================================================== read-only lookup
+// G lanes serve one op; lane g of the group holds slots [g*K, g*K+K) of a node.
+template <int G>
+STM_DEV uint32_t group_sum(uint32_t v) {
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
+// x[e] for a runtime e, written as a compile-time select tree so the array stays in
+// registers (a loop here lets the compiler turn it into an indexed scratch access).
+template <int K, int I = 0>
+STM_DEV uint64_t select_k(const uint64_t (&x)[K], int e) {
+  if constexpr (I == K - 1) {
+    return x[I];
+  } else {
+    return e == I ? x[I] : select_k<K, I + 1>(x, e);
+  }
+}
+
+// Slot s of the node held by the group (s group-uniform, 0 <= s < F).
+template <int F, int G>
+STM_DEV uint64_t group_at(const uint64_t (&x)[F / G], int s, int gbase) {
+  constexpr int K = F / G;
+  return shfl64(select_k<K>(x, s % K), gbase + s / K);
+}
+
+template <int F, int G>
+STM_DEV int group_count_le(const uint64_t (&x)[F / G], uint64_t key) {
+  constexpr int K = F / G;
+  uint32_t c = 0;
+#pragma unroll
+  for (int e = 0; e < K; ++e) c += (x[e] <= key) ? 1u : 0u;
+  return (int)group_sum<G>(c);
+}
+
+template <int F, int G>
+STM_DEV void load_slots(const uint64_t* __restrict__ base, uint32_t node, int g,
+                        uint64_t (&x)[F / G]) {
+  constexpr int K = F / G;
+  const uint64_t* p = base + (size_t)node * F + (size_t)g * K;
+#pragma unroll
+  for (int e = 0; e < K; ++e) x[e] = p[e];
+}
+
+template <int F, int G>
+__global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __restrict__ ukey,
+                                                const uint8_t* __restrict__ uop,
+                                                const uint32_t* __restrict__ gstart,
+                                                const uint32_t* __restrict__ npend,
+                                                const uint32_t* __restrict__ sidx,
+                                                const BatchCounters* __restrict__ cnt,
+                                                uint32_t chunk, uint64_t* __restrict__ out_vals,
+                                                uint8_t* __restrict__ out_found) {
+  constexpr int K = F / G;
+  const int lane = lane_id();
+  const int g = lane & (G - 1);
+  const int gbase = lane - g;
+  const uint64_t grp = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G;
+  const uint32_t U = cnt->n_unique;
+  const uint64_t i0 = grp * chunk;
+  if (i0 >= U) return;
+  const uint32_t i1 = (uint32_t)min<uint64_t>(i0 + chunk, U);
+  const uint32_t cap = t.capacity;
+  const uint32_t root = (uint32_t)min<uint64_t>(t.hdr->root, cap - 1);
+  const uint32_t height = (uint32_t)min<uint64_t>(t.hdr->height, 64);
+
+  uint32_t leaf = NODE_NONE;
+  uint64_t leaf_hi = 0;
+  uint64_t lk[K];
+  uint32_t par = NODE_NONE;
+  uint64_t par_hi = 0;
+  uint64_t pk[K], pv[K];
+
+  for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
+    if (uop[i] != OP_GET) continue;
+    const uint64_t key = ukey[i];
+    if (leaf == NODE_NONE || key >= leaf_hi) {
+      if (height <= 1) {
+        leaf = root;
+        leaf_hi = KEY_EMPTY;
+      } else {
+        if (par == NODE_NONE || key >= par_hi) {
+          uint32_t node = root;
+          uint64_t hi = KEY_EMPTY;
+          for (uint32_t lvl = height - 1; lvl > 1; --lvl) {
+            uint64_t nk[K], nv[K];
+            load_slots<F, G>(t.keys, node, g, nk);
+            load_slots<F, G>(t.vals, node, g, nv);
+            const int s = max(group_count_le<F, G>(nk, key) - 1, 0);
+            const uint64_t child = group_at<F, G>(nv, s, gbase);
+            const uint64_t nx = (s + 1 < F) ? group_at<F, G>(nk, s + 1, gbase) : KEY_EMPTY;
+            hi = min(hi, nx);
+            node = (uint32_t)min<uint64_t>(child, cap - 1);
+          }
+          load_slots<F, G>(t.keys, node, g, pk);
+          load_slots<F, G>(t.vals, node, g, pv);
+          par = node;
+          par_hi = hi;
+        }
+        const int s = max(group_count_le<F, G>(pk, key) - 1, 0);
+        leaf = (uint32_t)min<uint64_t>(group_at<F, G>(pv, s, gbase), cap - 1);
+        const uint64_t nx = (s + 1 < F) ? group_at<F, G>(pk, s + 1, gbase) : KEY_EMPTY;
+        leaf_hi = min(par_hi, nx);
+      }
+      load_slots<F, G>(t.keys, leaf, g, lk);
+    }
+    const int s = group_count_le<F, G>(lk, key) - 1;
+    const uint64_t at = group_at<F, G>(lk, max(s, 0), gbase);
+    const bool found = s >= 0 && at == key;
+    uint64_t val = 0;
+    if (found) val = t.vals[(size_t)leaf * F + s];
+    const uint32_t gs = gstart[i];
+    const uint32_t np = npend[i];
+    for (uint32_t p = g; p < np; p += G) {
+      const uint32_t pos = sidx[gs + p];
+      out_vals[pos] = val;
+      out_found[pos] = found ? 1 : 0;
+    }
+  }
+}
+
+// ================================================================== transactional upsert
+// A node held by one wavefront: slot s = j*64 + lane.
+template <int F>
+struct WNode {
+  static constexpr int SPL = (F + WAVE - 1) / WAVE;
+  uint64_t k[SPL];
+  uint64_t v[SPL];
+  uint64_t ver;
+  uint64_t low, high;
+  uint32_t id, count, level, next;
+};
+
+template <int F>
+STM_DEV uint64_t wave_at(const uint64_t (&x)[WNode<F>::SPL], uint32_t s) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j)
+    if ((s >> 6) == (uint32_t)j) r = readlane64(x[j], (int)(s & 63));
+  return r;
+}
+
+// Per-lane gather of slot src (any lane, any j); dflt when src is outside [0, F).
+template <int F>
+STM_DEV uint64_t wgather(const uint64_t (&x)[WNode<F>::SPL], int src, uint64_t dflt) {
+  uint64_t r = dflt;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) {
+    const uint64_t tv = shfl64(x[j], src & 63);
+    if (src >= 0 && src < F && (src >> 6) == j) r = tv;
+  }
+  return r;
+}
+
+template <int F>
+STM_DEV uint32_t wave_count_le(const uint64_t (&x)[WNode<F>::SPL], uint64_t key) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) c += (uint32_t)__popcll(ballot(x[j] <= key));
+  return c;
+}
+
+// Slot of parent p that routes key, checked to point at child; NODE_NONE if it does not.
+template <int F>
+STM_DEV uint32_t parent_slot(const WNode<F>& p, uint64_t key, uint32_t child) {
+  const uint32_t cle = wave_count_le<F>(p.k, key);
+  if (cle == 0 || cle > p.count) return NODE_NONE;
+  if (wave_at<F>(p.v, cle - 1) != (uint64_t)child) return NODE_NONE;
+  return cle - 1;
+}
+
+// Optimistic read of a node: version word (acquire), then content.  The content is only
+// trusted after the version is re-validated: by the commit CAS for nodes the transaction
+// writes, by fences/levels for the nodes it only passes through.
+template <int F>
+STM_DEV bool tx_read(const TreeView& t, uint32_t id, WNode<F>& n) {
+  if (id >= t.capacity) return false;
+  NodeMeta* m = t.meta + id;
+  const uint64_t w = uniform64(ld_acquire(&m->lock));
+  if (w & 1) return false;
+  const uint64_t info = uniform64(ld_relaxed(&m->info));
+  n.low = uniform64(ld_relaxed(&m->low));
+  n.high = uniform64(ld_relaxed(&m->high));
+  n.next = (uint32_t)uniform64(ld_relaxed(&m->next));
+  const uint64_t* kp = t.keys + (size_t)id * F;
+  const uint64_t* vp = t.vals + (size_t)id * F;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) {
+    const int s = j * WAVE + lane_id();
+    if (s < F) {
+      n.k[j] = ld_relaxed(kp + s);
+      n.v[j] = ld_relaxed(vp + s);
+    } else {
+      n.k[j] = KEY_EMPTY;
+      n.v[j] = 0;
+    }
+  }
+  n.ver = w;
+  n.id = id;
+  n.count = meta_count(info);
+  n.level = meta_level(info);
+  return n.count <= (uint32_t)F;
+}
+
+template <int F>
+STM_DEV void tx_write_node(const TreeView& t, uint32_t id, const uint64_t (&k)[WNode<F>::SPL],
+                           const uint64_t (&v)[WNode<F>::SPL], uint32_t count, uint32_t level,
+                           uint64_t low, uint64_t high, uint32_t next) {
+  uint64_t* kp = t.keys + (size_t)id * F;
+  uint64_t* vp = t.vals + (size_t)id * F;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) {
+    const int s = j * WAVE + lane_id();
+    if (s < F) {
+      st_relaxed(kp + s, k[j]);
+      st_relaxed(vp + s, v[j]);
+    }
+  }
+  NodeMeta* m = t.meta + id;
+  const int l = lane_id();
+  if (l == 1) st_relaxed(&m->info, make_info(count, level));
+  if (l == 2) st_relaxed(&m->low, low);
+  if (l == 3) st_relaxed(&m->high, high);
+  if (l == 4) st_relaxed(&m->next, (uint64_t)next);
+}
+
+// Try-lock of up to three version words (two nodes and optionally the tree header).
+// Each word must still hold the version the transaction read.  On any failure the words
+// already taken are restored to their old value (nothing was written) and false returns.
+STM_DEV bool tx_lock(uint64_t* w0, uint64_t v0, uint64_t* w1, uint64_t v1, uint64_t* w2,
+                     uint64_t v2) {
+  const int l = lane_id();
+  uint64_t* w = l == 0 ? w0 : (l == 1 ? w1 : (l == 2 ? w2 : nullptr));
+  const uint64_t v = l == 0 ? v0 : (l == 1 ? v1 : v2);
+  bool ok = true;
+  if (w) ok = cas_u64(w, v, v | 1ull);
+  const uint64_t fail = ballot(!ok);
+  if (fail) {
+    if (w && ok) st_relaxed(w, v);
+    return false;
+  }
+  fence_release();  // the lock bits are visible before any content store
+  return true;
+}
+
+STM_DEV void tx_unlock(uint64_t* w0, uint64_t v0, uint64_t* w1, uint64_t v1, uint64_t* w2,
+                       uint64_t v2) {
+  fence_release();  // content stores are visible before the new versions
+  const int l = lane_id();
+  uint64_t* w = l == 0 ? w0 : (l == 1 ? w1 : (l == 2 ? w2 : nullptr));
+  const uint64_t v = l == 0 ? v0 : (l == 1 ? v1 : v2);
+  if (w) st_relaxed(w, v + 2);  // clears the lock bit and bumps the version
+}
+
+// Allocates n fresh node ids; NODE_NONE when the pool is exhausted.
+STM_DEV uint32_t tx_alloc(const TreeView& t, uint32_t n) {
+  uint64_t base = 0;
+  if (lane_id() == 0) base = atomicAdd((unsigned long long*)&t.hdr->n_nodes, (unsigned long long)n);
+  base = readlane64(base, 0);
+  if (base + n > t.capacity) {
+    if (lane_id() == 0) atomicOr((unsigned long long*)&t.hdr->error, (unsigned long long)ERR_POOL_FULL);
+    return NODE_NONE;
+  }
+  return (uint32_t)base;
+}
+
+// Marks fresh nodes locked (version 0) before they become reachable.
+STM_DEV void tx_claim_new(const TreeView& t, uint32_t a, uint32_t b) {
+  const int l = lane_id();
+  if (l == 0 && a != NODE_NONE) st_relaxed(&t.meta[a].lock, 1ull);
+  if (l == 1 && b != NODE_NONE) st_relaxed(&t.meta[b].lock, 1ull);
+  fence_release();
+}
+
+enum TxResult { TX_DONE = 0, TX_ABORT = 1, TX_RESTART = 2, TX_FAIL = 3 };
+
+struct TxCounters {
+  uint32_t commits = 0, aborts = 0, leaf_splits = 0, inner_splits = 0, root_splits = 0;
+  uint32_t new_keys = 0;
+};
+
+// Inserts separator (sep -> rid) right after slot ps of parent p (which has room).
+template <int F>
+STM_DEV void parent_with_child(const WNode<F>& p, uint32_t ps, uint64_t sep, uint32_t rid,
+                               uint64_t (&nk)[WNode<F>::SPL], uint64_t (&nv)[WNode<F>::SPL]) {
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) {
+    const int s = j * WAVE + lane_id();
+    const uint64_t pk = wgather<F>(p.k, s - 1, KEY_EMPTY);
+    const uint64_t pv = wgather<F>(p.v, s - 1, 0);
+    const bool keep = s <= (int)ps;
+    const bool mine = s == (int)ps + 1;
+    nk[j] = keep ? p.k[j] : (mine ? sep : pk);
+    nv[j] = keep ? p.v[j] : (mine ? (uint64_t)rid : pv);
+  }
+}
+
+// Splits the full inner node n (preemptively, its own transaction).  pn is its parent when
+// has_par; otherwise n is the root, a new root is made and the header is written.
+template <int F>
+STM_DEV TxResult tx_split_inner(const TreeView& t, const WNode<F>& n, const WNode<F>& pn,
+                                bool has_par, uint64_t hver, uint64_t key, TxCounters& c) {
+  constexpr int SPL = WNode<F>::SPL;
+  const uint32_t h = F / 2;
+  const uint64_t sep = wave_at<F>(n.k, h);
+  uint32_t ps = 0;
+  if (has_par) {
+    if (pn.count >= (uint32_t)F) return TX_ABORT;  // parent filled up meanwhile
+    ps = parent_slot<F>(pn, key, n.id);
+    if (ps == NODE_NONE) return TX_ABORT;
+  }
+  uint64_t* hw = has_par ? nullptr : &t.hdr->lock;
+  if (!tx_lock(&t.meta[n.id].lock, n.ver, has_par ? &t.meta[pn.id].lock : nullptr,
+               has_par ? pn.ver : 0, hw, hver))
+    return TX_ABORT;
+  const uint32_t nnew = has_par ? 1u : 2u;
+  const uint32_t base = tx_alloc(t, nnew);
+  if (base == NODE_NONE) {
+    tx_unlock(&t.meta[n.id].lock, n.ver - 2, has_par ? &t.meta[pn.id].lock : nullptr,
+              has_par ? pn.ver - 2 : 0, hw, hver - 2);
+    return TX_FAIL;
+  }
+  const uint32_t rid = base;
+  const uint32_t qid = has_par ? NODE_NONE : base + 1;
+  tx_claim_new(t, rid, qid);
+  uint64_t lk[SPL], lv[SPL], rk[SPL], rv[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const int s = j * WAVE + lane_id();
+    lk[j] = s < (int)h ? n.k[j] : KEY_EMPTY;
+    lv[j] = s < (int)h ? n.v[j] : 0;
+    const uint64_t gk = wgather<F>(n.k, s + (int)h, KEY_EMPTY);
+    const uint64_t gv = wgather<F>(n.v, s + (int)h, 0);
+    const bool r_ok = s < (int)(n.count - h);
+    rk[j] = r_ok ? gk : KEY_EMPTY;
+    rv[j] = r_ok ? gv : 0;
+  }
+  tx_write_node<F>(t, rid, rk, rv, n.count - h, n.level, sep, n.high, n.next);
+  tx_write_node<F>(t, n.id, lk, lv, h, n.level, n.low, sep, rid);
+  if (has_par) {
+    uint64_t pk[SPL], pv[SPL];
+    parent_with_child<F>(pn, ps, sep, rid, pk, pv);
+    tx_write_node<F>(t, pn.id, pk, pv, pn.count + 1, pn.level, pn.low, pn.high,
+                     pn.next);
+  } else {
+    uint64_t qk[SPL], qv[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      const int s = j * WAVE + lane_id();
+      qk[j] = s == 0 ? n.low : (s == 1 ? sep : KEY_EMPTY);
+      qv[j] = s == 0 ? (uint64_t)n.id : (s == 1 ? (uint64_t)rid : 0);
+    }
+    tx_write_node<F>(t, qid, qk, qv, 2, n.level + 1, n.low, n.high, NODE_NONE);
+    if (lane_id() == 0) {
+      st_relaxed(&t.hdr->root, (uint64_t)qid);
+      st_relaxed(&t.hdr->height, (uint64_t)(n.level + 2));
+    }
+  }
+  tx_unlock(&t.meta[n.id].lock, n.ver, has_par ? &t.meta[pn.id].lock : &t.meta[rid].lock,
+            has_par ? pn.ver : 0ull, has_par ? &t.meta[rid].lock : &t.hdr->lock, has_par ? 0ull : hver);
+  if (!has_par) {
+    if (lane_id() == 0) st_relaxed(&t.meta[qid].lock, 2ull);
+    c.root_splits++;
+  }
+  c.inner_splits++;
+  c.commits++;
+  return TX_RESTART;
+}
+
+// Upsert of (key, val) into leaf n.  pn is its parent when has_par (else n is the root).
+template <int F>
+STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNode<F>& pn,
+                                bool has_par, uint64_t hver, uint64_t key, uint64_t val,
+                                bool& found, uint64_t& old, TxCounters& c) {
+  constexpr int SPL = WNode<F>::SPL;
+  const uint32_t cle = wave_count_le<F>(n.k, key);
+  const uint32_t s = cle ? cle - 1 : 0;
+  found = cle > 0 && wave_at<F>(n.k, s) == key;
+  old = found ? wave_at<F>(n.v, s) : 0;
+  uint64_t* lw = &t.meta[n.id].lock;
+  if (found) {
+    if (!tx_lock(lw, n.ver, nullptr, 0, nullptr, 0)) return TX_ABORT;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j)
+      if (j * WAVE + lane_id() == (int)s) st_relaxed(t.vals + (size_t)n.id * F + s, val);
+    tx_unlock(lw, n.ver, nullptr, 0, nullptr, 0);
+    c.commits++;
+    return TX_DONE;
+  }
+  const uint32_t pos = cle;  // insert position
+  if (n.count < (uint32_t)F) {
+    if (!tx_lock(lw, n.ver, nullptr, 0, nullptr, 0)) return TX_ABORT;
+    uint64_t nk[SPL], nv[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      const int sl = j * WAVE + lane_id();
+      const uint64_t gk = wgather<F>(n.k, sl - 1, KEY_EMPTY);
+      const uint64_t gv = wgather<F>(n.v, sl - 1, 0);
+      nk[j] = sl < (int)pos ? n.k[j] : (sl == (int)pos ? key : gk);
+      nv[j] = sl < (int)pos ? n.v[j] : (sl == (int)pos ? val : gv);
+    }
+    tx_write_node<F>(t, n.id, nk, nv, n.count + 1, 0, n.low, n.high, n.next);
+    tx_unlock(lw, n.ver, nullptr, 0, nullptr, 0);
+    c.commits++;
+    c.new_keys++;
+    return TX_DONE;
+  }
+  // Full leaf: split F+1 entries into h (left) and F+1-h (right).
+  const uint32_t h = (F + 1) / 2;
+  uint32_t ps = 0;
+  if (has_par) {
+    if (pn.count >= (uint32_t)F) return TX_ABORT;
+    ps = parent_slot<F>(pn, key, n.id);
+    if (ps == NODE_NONE) return TX_ABORT;
+  }
+  uint64_t* pw = has_par ? &t.meta[pn.id].lock : nullptr;
+  uint64_t* hw = has_par ? nullptr : &t.hdr->lock;
+  if (!tx_lock(lw, n.ver, pw, has_par ? pn.ver : 0, hw, hver)) return TX_ABORT;
+  const uint32_t base = tx_alloc(t, has_par ? 1u : 2u);
+  if (base == NODE_NONE) {
+    tx_unlock(lw, n.ver - 2, pw, has_par ? pn.ver - 2 : 0, hw, hver - 2);
+    return TX_FAIL;
+  }
+  const uint32_t rid = base;
+  const uint32_t qid = has_par ? NODE_NONE : base + 1;
+  tx_claim_new(t, rid, qid);
+  // A(x) = merged sequence: old[x] before pos, the new pair at pos, old[x-1] after.
+  uint64_t lk[SPL], lv[SPL], rk[SPL], rv[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const int sl = j * WAVE + lane_id();
+    // left slot sl < h holds A(sl)
+    const uint64_t km1 = wgather<F>(n.k, sl - 1, KEY_EMPTY);
+    const uint64_t vm1 = wgather<F>(n.v, sl - 1, 0);
+    const uint64_t ak = sl < (int)pos ? n.k[j] : (sl == (int)pos ? key : km1);
+    const uint64_t av = sl < (int)pos ? n.v[j] : (sl == (int)pos ? val : vm1);
+    lk[j] = sl < (int)h ? ak : KEY_EMPTY;
+    lv[j] = sl < (int)h ? av : 0;
+    // right slot sl < F+1-h holds A(sl + h)
+    const int x = sl + (int)h;
+    const uint64_t kx = wgather<F>(n.k, x, KEY_EMPTY);
+    const uint64_t vx = wgather<F>(n.v, x, 0);
+    const uint64_t kx1 = wgather<F>(n.k, x - 1, KEY_EMPTY);
+    const uint64_t vx1 = wgather<F>(n.v, x - 1, 0);
+    const uint64_t bk = x < (int)pos ? kx : (x == (int)pos ? key : kx1);
+    const uint64_t bv = x < (int)pos ? vx : (x == (int)pos ? val : vx1);
+    const bool r_ok = sl < (int)(F + 1 - h);
+    rk[j] = r_ok ? bk : KEY_EMPTY;
+    rv[j] = r_ok ? bv : 0;
+  }
+  const uint64_t sep = wave_at<F>(rk, 0);
+  tx_write_node<F>(t, rid, rk, rv, F + 1 - h, 0, sep, n.high, n.next);
+  tx_write_node<F>(t, n.id, lk, lv, h, 0, n.low, sep, rid);
+  if (has_par) {
+    uint64_t pk[SPL], pv[SPL];
+    parent_with_child<F>(pn, ps, sep, rid, pk, pv);
+    tx_write_node<F>(t, pn.id, pk, pv, pn.count + 1, pn.level, pn.low, pn.high,
+                     pn.next);
+    tx_unlock(lw, n.ver, pw, pn.ver, &t.meta[rid].lock, 0ull);
+  } else {
+    uint64_t qk[SPL], qv[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      const int sl = j * WAVE + lane_id();
+      qk[j] = sl == 0 ? n.low : (sl == 1 ? sep : KEY_EMPTY);
+      qv[j] = sl == 0 ? (uint64_t)n.id : (sl == 1 ? (uint64_t)rid : 0);
+    }
+    tx_write_node<F>(t, qid, qk, qv, 2, 1, n.low, n.high, NODE_NONE);
+    if (lane_id() == 0) {
+      st_relaxed(&t.hdr->root, (uint64_t)qid);
+      st_relaxed(&t.hdr->height, 2ull);
+    }
+    tx_unlock(lw, n.ver, &t.meta[rid].lock, 0ull, &t.hdr->lock, hver);
+    if (lane_id() == 0) st_relaxed(&t.meta[qid].lock, 2ull);
+    c.root_splits++;
+  }
+  c.leaf_splits++;
+  c.commits++;
+  c.new_keys++;
+  return TX_DONE;
+}
+
+// One attempt of the upsert transaction for (key, val).
+template <int F>
+STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val, bool& found,
+                                   uint64_t& old, TxCounters& c) {
+  const uint64_t hver = uniform64(ld_acquire(&t.hdr->lock));
+  if (hver & 1) return TX_ABORT;
+  const uint32_t root = (uint32_t)uniform64(ld_relaxed(&t.hdr->root));
+  const uint32_t height = (uint32_t)uniform64(ld_relaxed(&t.hdr->height));
+  if (height == 0 || height > 64) return TX_ABORT;
+  WNode<F> par, cur;
+  bool has_par = false;
+  uint32_t id = root;
+  uint32_t lvl = height - 1;
+  for (uint32_t step = 0; step < height; ++step) {
+    if (!tx_read<F>(t, id, cur)) return TX_ABORT;
+    if (cur.level != lvl || key < cur.low || key >= cur.high) return TX_ABORT;
+    if (cur.level == 0) {
+      return tx_leaf_upsert<F>(t, cur, par, has_par, hver, key, val, found, old, c);
+    }
+    if (cur.count >= (uint32_t)F) return tx_split_inner<F>(t, cur, par, has_par, hver, key, c);
+    if (cur.count == 0) return TX_ABORT;
+    const uint32_t cle = wave_count_le<F>(cur.k, key);
+    if (cle == 0) return TX_ABORT;
+    const uint64_t child = wave_at<F>(cur.v, cle - 1);
+    if (child >= t.capacity) return TX_ABORT;
+    par = cur;
+    has_par = true;
+    id = (uint32_t)child;
+    --lvl;
+  }
+  return TX_ABORT;
+}
+
+STM_DEV void backoff(uint32_t attempt) {
+  const uint32_t spins = min(attempt, 32u);
+  for (uint32_t i = 0; i < spins; ++i) __builtin_amdgcn_s_sleep(4);
+}
+
+// Runs merged upsert op g to completion (or gives up after max_attempts).
+template <int F>
+STM_DEV bool run_upsert(const TreeView& t, uint32_t g, const uint64_t* __restrict__ ukey,
+                        const uint64_t* __restrict__ uval, const uint32_t* __restrict__ gstart,
+                        const uint32_t* __restrict__ npend, const uint32_t* __restrict__ sidx,
+                        uint64_t* __restrict__ out_vals, uint8_t* __restrict__ out_found,
+                        uint32_t max_attempts, TxCounters& c) {
+  const uint64_t key = ukey[g];
+  const uint64_t val = uval[g];
+  bool found = false;
+  uint64_t old = 0;
+  uint32_t aborts = 0, restarts = 0;
+  for (;;) {
+    const TxResult r = tx_upsert_attempt<F>(t, key, val, found, old, c);
+    if (r == TX_DONE) break;
+    if (r == TX_FAIL) return true;  // pool exhausted: error bit raised, op dropped
+    if (r == TX_ABORT) {
+      c.aborts++;
+      if (++aborts >= max_attempts) return false;
+      backoff(aborts);
+    } else if (++restarts >= 4 * max_attempts) {  // TX_RESTART after a preemptive split
+      return false;
+    }
+  }
+  if (out_vals) {
+    const uint32_t gs = gstart[g];
+    const uint32_t np = npend[g];
+    for (uint32_t p = lane_id(); p < np; p += WAVE) {
+      const uint32_t pos = sidx[gs + p];
+      out_vals[pos] = old;
+      out_found[pos] = found ? 1 : 0;
+    }
+  }
+  return true;
+}
+
+STM_DEV void flush_counters(const TreeView& t, const TxCounters& c) {
+  if (lane_id() == 0) {
+    if (c.commits) atomicAdd(t.stats + ST_COMMITS, (unsigned long long)c.commits);
+    if (c.aborts) atomicAdd(t.stats + ST_ABORTS, (unsigned long long)c.aborts);
+    if (c.leaf_splits) atomicAdd(t.stats + ST_LEAF_SPLITS, (unsigned long long)c.leaf_splits);
+    if (c.inner_splits) atomicAdd(t.stats + ST_INNER_SPLITS, (unsigned long long)c.inner_splits);
+    if (c.root_splits) atomicAdd(t.stats + ST_ROOT_SPLITS, (unsigned long long)c.root_splits);
+    if (c.new_keys) {
+      atomicAdd(t.stats + ST_NEW_KEYS, (unsigned long long)c.new_keys);
+      atomicAdd((unsigned long long*)&t.hdr->n_keys, (unsigned long long)c.new_keys);
+    }
+  }
+}
+
+// Work-queue kernel: each wavefront takes merged upserts from ulist until it is drained.
+template <int F>
+__global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __restrict__ ukey,
+                                                const uint64_t* __restrict__ uval,
+                                                const uint32_t* __restrict__ gstart,
+                                                const uint32_t* __restrict__ npend,
+                                                const uint32_t* __restrict__ sidx,
+                                                const uint32_t* __restrict__ ulist,
+                                                BatchCounters* cnt, uint32_t* __restrict__ dlist,
+                                                uint32_t max_attempts, uint32_t per_wave,
+                                                uint64_t* __restrict__ out_vals,
+                                                uint8_t* __restrict__ out_found) {
+  const uint32_t n_up = cnt->n_upserts;
+  TxCounters c;
+  uint32_t upserts = 0, deferred = 0;
+  for (;;) {
+    uint32_t q = 0;
+    if (lane_id() == 0) q = atomicAdd(&cnt->queue_head, per_wave);
+    q = uniform32(__shfl(q, 0, WAVE));
+    if (q >= n_up) break;
+    const uint32_t qe = min(q + per_wave, n_up);
+    for (uint32_t e = q; e < qe; ++e) {
+      const uint32_t g = ulist[e];
+      upserts++;
+      if (!run_upsert<F>(t, g, ukey, uval, gstart, npend, sidx, out_vals, out_found,
+                         max_attempts, c)) {
+        uint32_t slot = 0;
+        if (lane_id() == 0) slot = atomicAdd(&cnt->n_deferred, 1u);
+        slot = uniform32(__shfl(slot, 0, WAVE));
+        if (lane_id() == 0) dlist[slot] = g;
+        deferred++;
+      }
+    }
+  }
+  flush_counters(t, c);
+  if (lane_id() == 0) {
+    if (upserts) atomicAdd(t.stats + ST_UPSERTS, (unsigned long long)upserts);
+    if (deferred) atomicAdd(t.stats + ST_DEFERRED, (unsigned long long)deferred);
+  }
+}
+
+// Serial pass over the ops that exhausted their attempts: one wavefront, no competitors,
+// so every transaction commits on its first attempt.
+template <int F>
+__global__ __launch_bounds__(64) void k_upsert_serial(TreeView t, const uint64_t* __restrict__ ukey,
+                                                      const uint64_t* __restrict__ uval,
+                                                      const uint32_t* __restrict__ gstart,
+                                                      const uint32_t* __restrict__ npend,
+                                                      const uint32_t* __restrict__ sidx,
+                                                      const uint32_t* __restrict__ dlist,
+                                                      const BatchCounters* cnt,
+                                                      uint64_t* __restrict__ out_vals,
+                                                      uint8_t* __restrict__ out_found) {
+  const uint32_t n = cnt->n_deferred;
+  TxCounters c;
+  for (uint32_t e = 0; e < n; ++e) {
+    if (!run_upsert<F>(t, dlist[e], ukey, uval, gstart, npend, sidx, out_vals, out_found, 1024u, c)) {
+      if (lane_id() == 0) atomicOr((unsigned long long*)&t.hdr->error, (unsigned long long)ERR_LIVELOCK);
+    }
+  }
+  flush_counters(t, c);
+}
+
+// ================================================================== dump / check
+__global__ void k_collect_leaves(TreeView t, uint32_t n_nodes, uint64_t* __restrict__ lows,
+                                 uint32_t* __restrict__ ids, uint32_t* __restrict__ counter) {
+  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n_nodes) return;
+  const NodeMeta m = t.meta[id];
+  if (meta_level(m.info) != 0) return;
+  const uint32_t slot = atomicAdd(counter, 1u);
+  lows[slot] = m.low;
+  ids[slot] = id;
+}
+
+__global__ void k_leaf_counts(TreeView t, const uint32_t* __restrict__ order,
+                              const uint32_t* __restrict__ ids_unsorted, uint32_t n_leaves,
+                              uint32_t* __restrict__ sorted_ids, uint32_t* __restrict__ counts) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_leaves) return;
+  const uint32_t id = ids_unsorted[order[r]];
+  sorted_ids[r] = id;
+  counts[r] = meta_count(t.meta[id].info);
+}
+
+template <int F>
+__global__ void k_dump_copy(TreeView t, const uint32_t* __restrict__ sorted_ids,
+                            const uint64_t* __restrict__ offsets, uint32_t n_leaves,
+                            uint64_t* __restrict__ out_k, uint64_t* __restrict__ out_v) {
+  const uint64_t gi = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (gi >= (uint64_t)n_leaves * F) return;
+  const uint32_t r = (uint32_t)(gi / F);
+  const uint32_t s = (uint32_t)(gi % F);
+  const uint64_t o0 = offsets[r];
+  const uint64_t o1 = offsets[r + 1];
+  if (o0 + s >= o1) return;
+  const uint32_t id = sorted_ids[r];
+  out_k[o0 + s] = t.keys[(size_t)id * F + s];
+  out_v[o0 + s] = t.vals[(size_t)id * F + s];
+}
+
+// One wavefront per node; counts violated invariants into *bad.
+template <int F>
+__global__ __launch_bounds__(256) void k_check(TreeView t, uint32_t n_nodes,
+                                               unsigned long long* bad) {
+  constexpr int SPL = WNode<F>::SPL;
+  const uint32_t id = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (id >= n_nodes) return;
+  const NodeMeta m = t.meta[id];
+  const uint32_t count = meta_count(m.info);
+  const uint32_t level = meta_level(m.info);
+  uint32_t errs = 0;
+  if (m.lock & 1) errs++;
+  if (count > (uint32_t)F) {
+    if (lane_id() == 0) atomicAdd(bad, 1ull);
+    return;
+  }
+  if (m.low >= m.high) errs++;
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const int s = j * WAVE + lane_id();
+    if (s >= F) continue;
+    const uint64_t k = t.keys[(size_t)id * F + s];
+    const uint64_t v = t.vals[(size_t)id * F + s];
+    if ((uint32_t)s >= count) {
+      if (k != KEY_EMPTY) errs++;
+      continue;
+    }
+    if (s > 0 && t.keys[(size_t)id * F + s - 1] >= k) errs++;
+    if (k < m.low || k >= m.high) errs++;
+    if (level > 0) {
+      if (s == 0 && k != m.low) errs++;
+      if (v >= n_nodes) {
+        errs++;
+        continue;
+      }
+      const NodeMeta cm = t.meta[v];
+      if (meta_level(cm.info) + 1 != level) errs++;
+      if (cm.low != k) errs++;
+      const uint64_t hi = ((uint32_t)s + 1 < count) ? t.keys[(size_t)id * F + s + 1] : m.high;
+      if (cm.high != hi) errs++;
+    }
+  }
+  if (lane_id() == 0) {
+    if (m.next != NODE_NONE) {
+      if (m.next >= n_nodes) errs++;
+      else {
+        const NodeMeta nm = t.meta[m.next];
+        if (nm.low != m.high || meta_level(nm.info) != level) errs++;
+      }
+    } else if (m.high != KEY_EMPTY) {
+      errs++;
+    }
+    if (level > 0 && count == 0) errs++;
+  }
+  for (int off = 32; off >= 1; off >>= 1) errs += __shfl_xor(errs, off, WAVE);
+  if (lane_id() == 0 && errs) atomicAdd(bad, (unsigned long long)errs);
+}
+
+// ================================================================== launchers
+template <int F>
+static hipError_t launch_build_leaves_f(const TreeView& t, const uint64_t* ukey, const uint64_t* uval,
+                                        uint32_t U, uint32_t fill, uint32_t L, uint64_t* lows,
+                                        hipStream_t s) {
+  const uint64_t n = (uint64_t)L * F;
+  k_build_leaves<F><<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(t, ukey, uval, U, fill, L, lows);
+  return hipGetLastError();
+}
+template <int F>
+static hipError_t launch_build_inner_f(const TreeView& t, const uint64_t* lows_prev, uint32_t n_prev,
+                                       uint32_t first_prev, uint32_t fill, uint32_t n_this,
+                                       uint32_t first_this, uint32_t level, uint64_t* lows_out,
+                                       hipStream_t s) {
+  const uint64_t n = (uint64_t)n_this * F;
+  k_build_inner<F><<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(t, lows_prev, n_prev, first_prev, fill,
+                                                               n_this, first_this, level, lows_out);
+  return hipGetLastError();
+}
+
+#define STM_DISPATCH_F(F_, CALL)   \
+  switch (F_) {                    \
+    case 32: { constexpr int F = 32; return CALL; } \
+    case 64: { constexpr int F = 64; return CALL; } \
+    case 128: { constexpr int F = 128; return CALL; } \
+    default: return hipErrorInvalidValue; \
+  }
+
+hipError_t launch_build_leaves(const TreeView& t, const uint64_t* ukey, const uint64_t* uval,
+                               uint32_t U, uint32_t fill, uint32_t L, uint64_t* lows, hipStream_t s) {
+  STM_DISPATCH_F(t.fanout, launch_build_leaves_f<F>(t, ukey, uval, U, fill, L, lows, s));
+}
+hipError_t launch_build_inner(const TreeView& t, const uint64_t* lows_prev, uint32_t n_prev,
+                              uint32_t first_prev, uint32_t fill, uint32_t n_this,
+                              uint32_t first_this, uint32_t level, uint64_t* lows_out,
+                              hipStream_t s) {
+  STM_DISPATCH_F(t.fanout, launch_build_inner_f<F>(t, lows_prev, n_prev, first_prev, fill, n_this,
+                                                   first_this, level, lows_out, s));
+}
+
+template <int F, int G>
+static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipStream_t s) {
+  const uint64_t groups = (a.n_max + a.chunk - 1) / a.chunk;
+  const uint64_t threads = groups * G;
+  const uint32_t blocks = (uint32_t)((threads + 255) / 256);
+  if (blocks == 0) return hipSuccess;
+  k_lookup<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx, a.counters,
+                                         a.chunk, a.out_vals, a.out_found);
+  return hipGetLastError();
+}
+
+template <int F>
+static hipError_t launch_lookup_f(const TreeView& t, const LookupArgs& a, hipStream_t s) {
+  switch (a.lanes) {
+    case 8: return launch_lookup_fg<F, 8>(t, a, s);
+    case 16: return launch_lookup_fg<F, 16>(t, a, s);
+    case 32: return launch_lookup_fg<F, (F >= 32 ? 32 : 16)>(t, a, s);
+    case 64: return launch_lookup_fg<F, (F >= 64 ? 64 : 32)>(t, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_lookup(const TreeView& t, const LookupArgs& a, hipStream_t s) {
+  STM_DISPATCH_F(t.fanout, launch_lookup_f<F>(t, a, s));
+}
+
+template <int F>
+static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
+  k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.ulist,
+                                        a.counters, a.dlist, a.max_attempts, a.per_wave,
+                                        a.out_vals, a.out_found);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_upsert_serial<F><<<1, 64, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.dlist,
+                                      a.counters, a.out_vals, a.out_found);
+  return hipGetLastError();
+}
+
+hipError_t launch_upsert(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
+  STM_DISPATCH_F(t.fanout, launch_upsert_f<F>(t, a, s));
+}
+
+hipError_t launch_collect_leaves(const TreeView& t, uint32_t n_nodes, uint64_t* lows, uint32_t* ids,
+                                 uint32_t* counter, hipStream_t s) {
+  if (n_nodes == 0) return hipSuccess;
+  k_collect_leaves<<<(n_nodes + 255) / 256, 256, 0, s>>>(t, n_nodes, lows, ids, counter);
+  return hipGetLastError();
+}
+hipError_t launch_leaf_counts(const TreeView& t, const uint32_t* order, const uint32_t* ids,
+                              uint32_t n_leaves, uint32_t* sorted_ids, uint32_t* counts,
+                              hipStream_t s) {
+  if (n_leaves == 0) return hipSuccess;
+  k_leaf_counts<<<(n_leaves + 255) / 256, 256, 0, s>>>(t, order, ids, n_leaves, sorted_ids, counts);
+  return hipGetLastError();
+}
+template <int F>
+static hipError_t launch_dump_copy_f(const TreeView& t, const uint32_t* sorted_ids,
+                                     const uint64_t* offsets, uint32_t n_leaves, uint64_t* out_k,
+                                     uint64_t* out_v, hipStream_t s) {
+  const uint64_t n = (uint64_t)n_leaves * F;
+  if (n == 0) return hipSuccess;
+  k_dump_copy<F><<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(t, sorted_ids, offsets, n_leaves, out_k, out_v);
+  return hipGetLastError();
+}
+hipError_t launch_dump_copy(const TreeView& t, const uint32_t* sorted_ids, const uint64_t* offsets,
+                            uint32_t n_leaves, uint64_t* out_k, uint64_t* out_v, hipStream_t s) {
+  STM_DISPATCH_F(t.fanout, launch_dump_copy_f<F>(t, sorted_ids, offsets, n_leaves, out_k, out_v, s));
+}
+template <int F>
+static hipError_t launch_check_f(const TreeView& t, uint32_t n_nodes, unsigned long long* bad,
+                                 hipStream_t s) {
+  if (n_nodes == 0) return hipSuccess;
+  const uint64_t threads = (uint64_t)n_nodes * WAVE;
+  k_check<F><<<(uint32_t)((threads + 255) / 256), 256, 0, s>>>(t, n_nodes, bad);
+  return hipGetLastError();
+}
+hipError_t launch_check(const TreeView& t, uint32_t n_nodes, unsigned long long* bad, hipStream_t s) {
+  STM_DISPATCH_F(t.fanout, launch_check_f<F>(t, n_nodes, bad, s));
+}
+
+}  // namespace stm

@@ -1,0 +1,55 @@
+// stm_tree.hpp — host launchers of the tree kernels (stm_tree.hip).
+#pragma once
+#include "stm_common.hpp"
+#include "stm_internal.hpp"
+
+namespace stm {
+
+struct LookupArgs {
+  const uint64_t* ukey;
+  const uint8_t* uop;
+  const uint32_t* gstart;
+  const uint32_t* npend;
+  const uint32_t* sidx;
+  const BatchCounters* counters;
+  uint64_t n_max;     // upper bound of the group count (grid sizing)
+  uint32_t chunk;     // sorted groups walked by one lane group
+  uint32_t lanes;     // lanes per op: 8, 16, 32, 64
+  uint64_t* out_vals;
+  uint8_t* out_found;
+};
+
+struct UpsertArgs {
+  const uint64_t* ukey;
+  const uint64_t* uval;
+  const uint32_t* gstart;
+  const uint32_t* npend;
+  const uint32_t* sidx;
+  const uint32_t* ulist;
+  BatchCounters* counters;
+  uint32_t* dlist;
+  uint32_t max_attempts;
+  uint32_t per_wave;
+  uint32_t blocks;     // 256-thread workgroups of the work-queue kernel
+  uint64_t* out_vals;
+  uint8_t* out_found;
+};
+
+hipError_t launch_build_leaves(const TreeView& t, const uint64_t* ukey, const uint64_t* uval,
+                               uint32_t U, uint32_t fill, uint32_t L, uint64_t* lows, hipStream_t s);
+hipError_t launch_build_inner(const TreeView& t, const uint64_t* lows_prev, uint32_t n_prev,
+                              uint32_t first_prev, uint32_t fill, uint32_t n_this,
+                              uint32_t first_this, uint32_t level, uint64_t* lows_out,
+                              hipStream_t s);
+hipError_t launch_lookup(const TreeView& t, const LookupArgs& a, hipStream_t s);
+hipError_t launch_upsert(const TreeView& t, const UpsertArgs& a, hipStream_t s);
+hipError_t launch_collect_leaves(const TreeView& t, uint32_t n_nodes, uint64_t* lows, uint32_t* ids,
+                                 uint32_t* counter, hipStream_t s);
+hipError_t launch_leaf_counts(const TreeView& t, const uint32_t* order, const uint32_t* ids,
+                              uint32_t n_leaves, uint32_t* sorted_ids, uint32_t* counts,
+                              hipStream_t s);
+hipError_t launch_dump_copy(const TreeView& t, const uint32_t* sorted_ids, const uint64_t* offsets,
+                            uint32_t n_leaves, uint64_t* out_k, uint64_t* out_v, hipStream_t s);
+hipError_t launch_check(const TreeView& t, uint32_t n_nodes, unsigned long long* bad, hipStream_t s);
+
+}  // namespace stm

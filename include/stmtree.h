@@ -1,0 +1,159 @@
+/*
+ * stmtree.h — C ABI of the MI355X-native transactional B+tree batch path.
+ *
+ * The library keeps an ordered u64 -> u64 map resident in HBM and runs batches of
+ * mixed point lookups and inserts whose results equal a sequential replay of the
+ * batch in input order (the reference's sort + per-key fold linearisation).
+ *
+ * Every entry point returns an int status (STM_OK or an STM_ERR_* code) and never
+ * exits the process; stm_last_error() gives the message of the last failure on the
+ * calling thread.  A tree handle is not thread-safe: one host thread drives it.
+ *
+ * Reference interface each entry point replaces (paths relative to the reference repo):
+ *   GPU_BTree::prepareGPU          src/tree.h:32, src/gpu-stm-tree-v38/bpt-gpustm.cpp:14-28   -> stm_create
+ *   GPU_BTree::~GPU_BTree          src/tree.h:30, src/gpu-stm-tree-v38/bpt-gpustm.h:201         -> stm_destroy
+ *   first "-i" insert-only file    README.md:9, src/main.cpp:25-29                             -> stm_bulk_load
+ *   prepareInput + gpu_work        src/tree.h:33,35, src/gpu-stm-tree-v38/bpt-gpustm.cpp:77-174 -> stm_run_batch
+ *   test() answer un-permute       src/gpu-stm-tree-v38/bpt-gpustm.cpp:185-192                  -> out_vals/out_found
+ *   Time_Measure::getGPUTime       src/time_measure.cuh:30-42                                   -> stm_get_stats
+ *   ABORT_COUNT counters           src/gpu-stm-tree-v38/bpt-gpustm.cpp:142-164                  -> stm_get_stats
+ *   CUDA_ERROR_HANDLER (exit)      src/cuda_utils.h:3-12                                        -> status codes
+ */
+#ifndef STMTREE_H
+#define STMTREE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define STM_OK 0
+#define STM_ERR_INVALID 1 /* bad argument (null pointer, key == STM_KEY_RESERVED, bad op code, size) */
+#define STM_ERR_NOMEM 2   /* device allocation failed or the node pool is exhausted */
+#define STM_ERR_DEVICE 3  /* a HIP runtime call or a kernel failed */
+#define STM_ERR_STATE 4   /* call not valid in the tree's current state */
+
+/* Op codes: same values as the reference's mission_t (src/input-helper.h:14-17).
+ * INSERT is an upsert that returns the previous value; GET returns the value.
+ * As in the reference's merge (src/gpu-stm-tree-v38/bpt-gpustm-kernel-aux.cuh:34), every
+ * op code other than INSERT is executed with GET semantics by stm_run_batch. */
+#define STM_OP_INSERT 0
+#define STM_OP_GET 1
+#define STM_OP_DELETE 2
+#define STM_OP_RANGE 3
+
+/* The one key value that cannot be stored (it is the empty-slot sentinel). */
+#define STM_KEY_RESERVED 0xffffffffffffffffull
+
+/* Lookup routing methods (stm_config.route). */
+#define STM_ROUTE_DESCENT 0 /* root-to-leaf descent (the plain method) */
+#define STM_ROUTE_LEARNED 1 /* learned leaf-window prediction, verified, with descent fallback */
+
+typedef struct stm_config {
+    uint32_t fanout;      /* slots per node: 32, 64 or 128 (default 64) */
+    uint32_t leaf_fill;   /* keys per node written by bulk load (0 = 3/4 of fanout) */
+    uint64_t max_keys;    /* node-pool sizing: keys the tree may hold (default 2^27) */
+    uint64_t max_batch;   /* ops per device pass; longer batches are chunked (default 2^22) */
+    int32_t device;       /* HIP device ordinal (default 0) */
+    uint32_t lanes_per_op;/* lookup kernel lanes per op: 8, 16, 32 or 64 (default 16) */
+    uint32_t ops_per_group;/* sorted ops walked by one lane group (default 16) */
+    uint32_t upsert_ops_per_wave; /* upserts handled by one wavefront (default 1) */
+    uint32_t max_attempts;/* STM attempts before an op is deferred to the serial pass (default 64) */
+    uint32_t route;       /* STM_ROUTE_* */
+    uint32_t timing;      /* 1 = record HIP events around every phase (stm_get_stats) */
+    uint32_t reserved[5];
+} stm_config;
+
+/* Per-phase timing slots of stm_stats.phase_ms / phase_launches. */
+#define STM_PHASE_SORT 0
+#define STM_PHASE_MERGE 1
+#define STM_PHASE_LOOKUP 2
+#define STM_PHASE_UPSERT 3
+#define STM_PHASE_DEFERRED 4
+#define STM_PHASE_ROUTE 5
+#define STM_PHASE_TRAIN 6
+#define STM_PHASE_OTHER 7
+#define STM_NUM_PHASES 8
+
+typedef struct stm_stats {
+    uint64_t n_keys;          /* keys in the tree */
+    uint64_t n_nodes;         /* nodes allocated from the pool */
+    uint64_t capacity_nodes;  /* node-pool capacity */
+    uint64_t height;          /* levels (1 = root is a leaf) */
+    uint64_t batches;         /* device passes run */
+    uint64_t ops;             /* ops submitted */
+    uint64_t unique_ops;      /* ops after the per-key fold */
+    uint64_t upserts;         /* merged INSERT ops executed as transactions */
+    uint64_t commits;         /* committed transactions (upserts + preemptive splits) */
+    uint64_t aborts;          /* aborted transaction attempts */
+    uint64_t leaf_splits;
+    uint64_t inner_splits;
+    uint64_t root_splits;
+    uint64_t deferred;        /* ops that exhausted max_attempts and ran in the serial pass */
+    uint64_t route_hits;      /* learned routing: leaf found inside the predicted window */
+    uint64_t route_misses;    /* learned routing: fell back to descent */
+    double phase_ms[STM_NUM_PHASES];
+    uint64_t phase_launches[STM_NUM_PHASES];
+} stm_stats;
+
+typedef struct stm_tree stm_tree;
+
+/* Fill *cfg with defaults. */
+int stm_config_init(stm_config* cfg);
+
+/* Create an empty tree on cfg->device (cfg may be NULL for defaults). */
+int stm_create(const stm_config* cfg, stm_tree** out);
+
+/* Free every device and host resource of the tree. NULL is accepted. */
+int stm_destroy(stm_tree* tree);
+
+/* Load n (key, value) pairs from host memory.  On an empty tree the tree is built
+ * bottom-up; on a non-empty tree this is a batch of n inserts.  Duplicate keys keep
+ * the value of the last occurrence (sequential insert semantics). */
+int stm_bulk_load(stm_tree* tree, const uint64_t* keys, const uint64_t* vals, size_t n);
+
+/* Same as stm_bulk_load with device-resident inputs. */
+int stm_bulk_load_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals, size_t n);
+
+/* Run one batch of n ops held in host memory.  vals[i] is read only for inserts
+ * (vals may be NULL when the batch has no inserts).  For every op i:
+ *   out_found[i] = 1 and out_vals[i] = value  if the key was present before op i,
+ *   out_found[i] = 0 and out_vals[i] = 0      otherwise,
+ * where "before op i" means after every earlier op of the batch, in input order. */
+int stm_run_batch(stm_tree* tree, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+                  size_t n, uint64_t* out_vals, uint8_t* out_found);
+
+/* Same with device-resident inputs and outputs.  Work is queued on the tree's stream
+ * (stm_get_stream); the call returns after the batch has completed. */
+int stm_run_batch_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
+                         const uint8_t* d_ops, size_t n, uint64_t* d_out_vals, uint8_t* d_out_found);
+
+/* Number of keys in the tree. */
+int stm_size(stm_tree* tree, uint64_t* n_keys);
+
+/* In-order dump of all (key, value) pairs into host buffers of capacity cap.
+ * *n receives the number of keys; STM_ERR_INVALID if cap < *n. */
+int stm_dump(stm_tree* tree, uint64_t* keys, uint64_t* vals, size_t cap, size_t* n);
+
+/* Structural check of the device tree: sorted keys, fences, levels, sibling chain,
+ * child ranges, all locks released.  *bad receives the number of violations. */
+int stm_check(stm_tree* tree, uint64_t* bad);
+
+/* Counters and per-phase device times accumulated since create or stm_reset_stats. */
+int stm_get_stats(stm_tree* tree, stm_stats* out);
+int stm_reset_stats(stm_tree* tree);
+
+/* The hipStream_t every kernel of this tree runs on. */
+int stm_get_stream(stm_tree* tree, void** stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* stm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STMTREE_H */

@@ -1,0 +1,206 @@
+"""GPU parity of the HIP batch path against the CPU oracle (bit-exact).
+
+Every test calls the product through the C ABI (libstmtree.so via ctypes) and compares
+each op's (found, value) and the final in-order dump with the oracle's sequential replay
+on the same inputs.  The structural checker (stm_check) must report a well-formed tree
+with every lock released after each batch.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def tree(**kw):
+    from eirene_amd import StmConfig, StmTree
+    base = dict(max_keys=1 << 20, max_batch=1 << 16)
+    base.update(kw)
+    return StmTree(StmConfig(**base))
+
+
+def assert_same(got, want, what=""):
+    gv, gf = got
+    wv, wf = want
+    bad = np.nonzero((gf != wf) | (gv != wv))[0]
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5].tolist()}: " \
+        f"got {[(int(gf[i]), int(gv[i])) for i in bad[:3]]} want {[(int(wf[i]), int(wv[i])) for i in bad[:3]]}"
+
+
+def assert_dump(t, o):
+    k, v = t.dump()
+    ek, ev = o.dump()
+    assert k.size == ek.size, (k.size, ek.size)
+    np.testing.assert_array_equal(k, ek)
+    np.testing.assert_array_equal(v, ev)
+    assert t.size() == ek.size
+    assert t.check() == 0
+
+
+def test_known_answers():
+    with tree(fanout=64) as t:
+        keys = np.array([5, 5, 5, 5, 5, 7, 7], dtype=np.uint64)
+        vals = np.array([0, 10, 0, 11, 0, 70, 0], dtype=np.uint64)
+        ops = np.array([1, 0, 1, 0, 3, 0, 2], dtype=np.uint8)
+        v, f = t.run_batch(keys, vals, ops)
+        assert f.tolist() == [0, 0, 1, 1, 1, 0, 1]
+        assert v.tolist() == [0, 0, 10, 10, 11, 0, 70]
+        k, d = t.dump()
+        assert k.tolist() == [5, 7] and d.tolist() == [11, 70]
+        assert t.check() == 0
+
+
+@pytest.mark.parametrize("fanout", [32, 64, 128])
+@pytest.mark.parametrize("name", sorted(p.stem for p in GOLDEN.glob("*.npz")))
+def test_golden_fixtures(oracle_mod, name, fanout):
+    z = np.load(GOLDEN / f"{name}.npz")
+    with tree(fanout=fanout, lanes_per_op=16) as t:
+        t.bulk_load(z["load_keys"], z["load_vals"])
+        assert t.check() == 0
+        for b in range(int(z["n_batches"][0])):
+            got = t.run_batch(z[f"b{b}_keys"], z[f"b{b}_vals"], z[f"b{b}_ops"])
+            assert_same(got, (z[f"b{b}_out_vals"], z[f"b{b}_out_found"]), f"{name} batch {b}")
+            assert t.check() == 0
+        k, v = t.dump()
+        np.testing.assert_array_equal(k, z["dump_keys"])
+        np.testing.assert_array_equal(v, z["dump_vals"])
+
+
+@pytest.mark.parametrize("fanout,lanes", [(32, 8), (32, 32), (64, 8), (64, 16), (64, 64),
+                                          (128, 16), (128, 64)])
+def test_random_batches(oracle_mod, fanout, lanes):
+    rng = np.random.default_rng(fanout * 100 + lanes)
+    domain = rng.integers(0, 1 << 63, size=30000, dtype=np.uint64)
+    load = rng.choice(domain, 8000)
+    load_vals = rng.integers(0, 1 << 63, size=load.size, dtype=np.uint64)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, load_vals)
+    with tree(fanout=fanout, lanes_per_op=lanes, ops_per_group=int(rng.integers(1, 40))) as t:
+        t.bulk_load(load, load_vals)
+        for b in range(4):
+            n = int(rng.integers(1, 40000))
+            keys = rng.choice(domain, n)
+            vals = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+            ops = rng.integers(0, 4, size=n).astype(np.uint8)
+            assert_same(t.run_batch(keys, vals, ops), o.run_batch(keys, vals, ops), f"batch {b}")
+        assert_dump(t, o)
+
+
+@pytest.mark.parametrize("fanout", [32, 64, 128])
+def test_empty_tree_growth_with_root_splits(oracle_mod, fanout):
+    rng = np.random.default_rng(fanout)
+    o = oracle_mod.Oracle()
+    with tree(fanout=fanout) as t:
+        for b in range(3):
+            n = 60000
+            keys = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+            if b == 2:
+                keys[: n // 2] = rng.integers(0, 64, size=n // 2).astype(np.uint64)
+            vals = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+            ops = (rng.random(n) < 0.1).astype(np.uint8)  # 90 % inserts
+            assert_same(t.run_batch(keys, vals, ops), o.run_batch(keys, vals, ops), f"batch {b}")
+            assert t.check() == 0
+        st = t.stats()
+        assert st["leaf_splits"] > 0 and st["root_splits"] > 0 and st["height"] >= 3
+        assert_dump(t, o)
+
+
+def test_chunking_does_not_change_results(oracle_mod):
+    rng = np.random.default_rng(5)
+    load = rng.integers(0, 1 << 20, size=20000).astype(np.uint64)
+    vals0 = rng.integers(0, 1 << 63, size=load.size, dtype=np.uint64)
+    n = 50000
+    keys = rng.integers(0, 1 << 20, size=n).astype(np.uint64)
+    vals = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+    ops = (rng.random(n) < 0.6).astype(np.uint8)
+    outs = []
+    for mb in (997, 1 << 16):
+        with tree(max_batch=mb) as t:
+            t.bulk_load(load, vals0)
+            outs.append(t.run_batch(keys, vals, ops) + t.dump())
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, vals0)
+    assert_same(outs[0][:2], o.run_batch(keys, vals, ops))
+
+
+def test_contended_zipf(oracle_mod):
+    from eirene_amd import workloads as w
+    load = w.uniform_keys(1 << 18, 21)
+    lv = w.values_for(load)
+    srt = w.sorted_copy(load)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, lv)
+    with tree(max_keys=1 << 21, max_batch=1 << 18) as t:
+        t.bulk_load(load, lv)
+        for b in range(4):
+            keys, vals, ops = w.zipf_batch(srt, 1 << 18, 100 + b, 0.99, 0.5)
+            assert_same(t.run_batch(keys, vals, ops), o.run_batch(keys, vals, ops), f"zipf {b}")
+            assert t.check() == 0
+        st = t.stats()
+        assert st["upserts"] > 0 and st["commits"] >= st["upserts"]
+        assert_dump(t, o)
+
+
+def test_plain_uniform_reduced(oracle_mod):
+    from eirene_amd import workloads as w
+    load = w.uniform_keys(1 << 20, 1)
+    lv = w.values_for(load)
+    keys, vals, ops = w.mixed_batch(w.sorted_copy(load), 1 << 19, 2, 0.95, 0.9)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, lv)
+    with tree(max_keys=1 << 21, max_batch=1 << 19) as t:
+        t.bulk_load(load, lv)
+        assert_same(t.run_batch(keys, vals, ops), o.run_batch(keys, vals, ops))
+        assert_dump(t, o)
+
+
+def test_reserved_key_rejected():
+    from eirene_amd import StmError
+    with tree() as t:
+        keys = np.array([1, 0xFFFFFFFFFFFFFFFF], dtype=np.uint64)
+        with pytest.raises(StmError) as e:
+            t.run_batch(keys, np.zeros(2, dtype=np.uint64), np.zeros(2, dtype=np.uint8))
+        assert e.value.code == 1
+        with pytest.raises(StmError):
+            t.run_batch(keys[:1], np.zeros(1, dtype=np.uint64), np.array([9], dtype=np.uint8))
+        assert t.size() == 0 and t.check() == 0
+
+
+def test_pool_exhaustion_reports_nomem():
+    from eirene_amd import StmError
+    rng = np.random.default_rng(1)
+    with tree(max_keys=1000, fanout=32) as t:
+        keys = rng.integers(0, 1 << 63, size=200000, dtype=np.uint64)
+        with pytest.raises(StmError) as e:
+            t.run_batch(keys, keys, np.zeros(keys.size, dtype=np.uint8))
+        assert e.value.code == 2
+        assert t.check() == 0  # the tree stays well formed after refused inserts
+
+
+def test_device_api_matches_host_api(oracle_mod):
+    import torch
+    rng = np.random.default_rng(9)
+    load = rng.integers(0, 1 << 63, size=10000, dtype=np.uint64)
+    n = 30000
+    keys = np.concatenate([rng.choice(load, n // 2), rng.integers(0, 1 << 63, size=n // 2, dtype=np.uint64)])
+    vals = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+    ops = (rng.random(n) < 0.8).astype(np.uint8)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, load)
+    want = o.run_batch(keys, vals, ops)
+    dev = torch.device("cuda:0")
+    as_t = lambda a: torch.from_numpy(a.view(np.int64) if a.dtype == np.uint64 else a).to(dev)
+    with tree() as t:
+        t.bulk_load_device(as_t(load), as_t(load))
+        ov = torch.zeros(n, dtype=torch.int64, device=dev)
+        of = torch.zeros(n, dtype=torch.uint8, device=dev)
+        t.run_batch_device(as_t(keys), as_t(vals), as_t(ops), ov, of)
+        torch.cuda.synchronize()
+        got = (ov.cpu().numpy().view(np.uint64), of.cpu().numpy())
+        assert_same(got, want)
+        assert_dump(t, o)
