@@ -1,0 +1,189 @@
+"""Benchmark of the STM B+tree batch path on one MI355X.
+
+Workload (the 2^26-key, 95 % lookup / 5 % insert configuration the headline metric is
+quoted on): bulk-load 2^26 unique keys drawn from Gamma(k=2, theta=2^56), values
+splitmix64(key); then batches of 2^22 ops, 95 % point lookups (90 % of them hits sampled
+uniformly from the loaded keys, 10 % fresh misses) and 5 % inserts of fresh keys of the
+same distribution.  A step is one batch.  All batches are generated on the host, copied
+to HBM before the timed region, and run back to back through stm_run_batch_device.
+
+Prints one JSON line.  `value` is Mops/s over the K timed batches (wall clock from the
+first launch to the last batch's completion).  Per-kernel device times come from HIP
+events the library records on its own stream around every phase (stm_config.timing).
+
+    python bench.py [--steps K] [--warmup W] [--route descent|learned] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mops/s (95% lookup/5% insert, 2^26 keys): learned vs plain; p99 latency"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per op with a 64-slot u64 accounting node (key 8 + result 8 + leaf
+# keys 512 + value 8 for a lookup; key, value, old value, leaf read + write and the lock
+# word for an insert).
+BYTES_LOOKUP = 536
+BYTES_INSERT = 2088
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--load-log2", type=int, default=26)
+    ap.add_argument("--batch-log2", type=int, default=22)
+    ap.add_argument("--fanout", type=int, default=64)
+    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--group", type=int, default=16)
+    ap.add_argument("--route", choices=["descent", "learned"], default="descent")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="approximate CPU-baseline budget (batches run until it is spent)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; the batch path has no "
+                          "multi-GPU mode (key-range sharding is a design note)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import eirene_amd as ea
+    from eirene_amd import workloads as w
+
+    n_load = 1 << args.load_log2
+    n_batch = 1 << args.batch_log2
+    n_batches = args.warmup + args.steps
+    t0 = time.time()
+    load = w.gamma_keys(n_load, args.seed, 2.0, 2.0 ** 56)
+    load_vals = w.values_for(load)
+    load_sorted = w.sorted_copy(load)
+    log(f"generated {n_load} keys in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    keys = np.empty(n_batches * n_batch, dtype=np.uint64)
+    vals = np.empty_like(keys)
+    ops = np.empty(n_batches * n_batch, dtype=np.uint8)
+    for b in range(n_batches):
+        s = slice(b * n_batch, (b + 1) * n_batch)
+        keys[s], vals[s], ops[s] = w.mixed_batch(load_sorted, n_batch, args.seed * 1000 + b, 0.95,
+                                                 0.9, w.DIST_GAMMA, 2.0, 2.0 ** 56)
+    log(f"generated {n_batches} batches in {time.time() - t0:.1f}s")
+
+    cfg = ea.StmConfig(fanout=args.fanout, max_keys=n_load + n_batches * n_batch + (1 << 20),
+                       max_batch=n_batch, lanes_per_op=args.lanes, ops_per_group=args.group,
+                       route=ea.ROUTE_LEARNED if args.route == "learned" else ea.ROUTE_DESCENT,
+                       timing=True)
+    tree = ea.StmTree(cfg)
+    t0 = time.time()
+    dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(load_vals)
+    tree.bulk_load_device(dk, dv)
+    dk.free()
+    dv.free()
+    log(f"bulk load {time.time() - t0:.2f}s, stats {tree.stats()['height']} levels")
+    bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
+    out_v = ea.DeviceBuffer(keys.size, np.uint64)
+    out_f = ea.DeviceBuffer(keys.size, np.uint8)
+
+    for b in range(args.warmup):
+        tree.run_batch_device(bk, bv, bo, out_v, out_f, n=n_batch, offset=b * n_batch)
+    tree.sync()
+    tree.reset_stats()
+    lat = []
+    t_start = time.perf_counter()
+    for b in range(args.warmup, n_batches):
+        tb = time.perf_counter()
+        tree.run_batch_device(bk, bv, bo, out_v, out_f, n=n_batch, offset=b * n_batch)
+        lat.append(time.perf_counter() - tb)
+    tree.sync()
+    elapsed = time.perf_counter() - t_start
+    st = tree.stats()
+    total_ops = args.steps * n_batch
+    mops = total_ops / elapsed / 1e6
+    log(f"{args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s; phases {st['phase_ms']}")
+
+    # Roofline of the dominant tree kernel: the read-only lookup kernel.
+    lookups = st["unique_ops"] - st["upserts"]
+    lk_launch = max(st["phase_launches"]["lookup"], 1)
+    lk_ms = st["phase_ms"]["lookup"] / lk_launch
+    lk_bytes = lookups / lk_launch * BYTES_LOOKUP
+    achieved = lk_bytes / (lk_ms * 1e-3) / 1e9 if lk_ms > 0 else 0.0
+    mix_bytes = 0.95 * BYTES_LOOKUP + 0.05 * BYTES_INSERT
+    roofline = {
+        "kernel": "k_lookup<64,16>" if args.fanout == 64 and args.lanes == 16 else
+                  f"k_lookup<{args.fanout},{args.lanes}>",
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "avg_launch_ms": round(lk_ms, 4), "lookups_per_launch": round(lookups / lk_launch),
+        "bytes_per_lookup": BYTES_LOOKUP,
+        "whole_batch_gbs": round(total_ops * mix_bytes / elapsed / 1e9, 1),
+    }
+
+    cpu = None
+    agree = None
+    if not args.no_cpu:
+        import oracle
+        oracle.build()
+        threads = min(os.cpu_count() or 1, 16)
+        t0 = time.time()
+        ex = oracle.CpuExecutor(load_sorted, w.values_for(load_sorted), threads)
+        log(f"cpu executor built in {time.time() - t0:.1f}s with {threads} threads")
+        cpu_sec = 0.0
+        cpu_ops = 0
+        b = 0
+        while b < n_batches and (cpu_sec < args.cpu_seconds or b < 2):
+            s = slice(b * n_batch, (b + 1) * n_batch)
+            cv, cf, sec = ex.run_batch(keys[s], vals[s], ops[s])
+            if b == 0:  # first batch: CPU and GPU start from the same tree
+                gv = out_v.download(n_batch)
+                gf = out_f.download(n_batch)
+                agree = bool(np.array_equal(gv, cv) and np.array_equal(gf, cf))
+            cpu_sec += sec
+            cpu_ops += n_batch
+            b += 1
+        ex.close()
+        cpu = {"value": round(cpu_ops / cpu_sec / 1e6, 2), "unit": "Mops/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {b} batches of the same workload ({cpu_ops} ops) on a "
+                         f"{threads}-thread key-range-partitioned CPU B+tree"}
+
+    out = {
+        "metric": METRIC, "value": round(mops, 2), "unit": "Mops/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"learned_gamma: bulk-load 2^{args.load_log2} Gamma(k=2, theta=2^56) "
+                               f"keys, {args.steps} batches of 2^{args.batch_log2} ops, 95% lookup "
+                               f"(90% hits) / 5% insert", "global_batch": n_batch,
+                   "route": args.route, "fanout": args.fanout, "lanes_per_op": args.lanes,
+                   "ops_per_group": args.group, "parallelism": "single GPU"},
+        "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+        "phase_ms_per_batch": {k: round(v / max(st["phase_launches"][k], 1), 4)
+                               for k, v in st["phase_ms"].items() if st["phase_launches"][k]},
+        "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
+                                   "root_splits", "deferred", "n_keys", "height")},
+        "roofline": roofline, "cpu_baseline": cpu, "cpu_gpu_first_batch_agree": agree,
+    }
+    print(json.dumps(out), flush=True)
+    tree.close()
+
+
+if __name__ == "__main__":
+    main()

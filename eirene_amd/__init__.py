@@ -25,7 +25,7 @@ from typing import Optional
 import numpy as np
 
 __all__ = [
-    "StmTree", "StmError", "StmConfig", "load_library", "LIB_PATH",
+    "StmTree", "StmError", "StmConfig", "DeviceBuffer", "load_library", "LIB_PATH",
     "OP_INSERT", "OP_GET", "OP_DELETE", "OP_RANGE", "KEY_RESERVED",
     "ROUTE_DESCENT", "ROUTE_LEARNED", "PHASES",
 ]
@@ -94,6 +94,11 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_reset_stats": (C.c_int, [_P]),
         "stm_get_stream": (C.c_int, [_P, C.POINTER(_P)]),
         "stm_last_error": (C.c_char_p, []),
+        "stm_device_alloc": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(_P)]),
+        "stm_device_free": (C.c_int, [_P]),
+        "stm_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
+        "stm_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+        "stm_sync": (C.c_int, [_P]),
         # workload generators (host only)
         "stm_splitmix64": (C.c_uint64, [C.c_uint64]),
         "stm_gen_values": (None, [_P, C.c_size_t, _P]),
@@ -129,6 +134,51 @@ def _u64(a) -> np.ndarray:
 
 def _u8(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+class DeviceBuffer:
+    """A device allocation made through the library's own HIP runtime.
+
+    Keeps batches resident in HBM without bringing a second GPU runtime (such as the one
+    bundled with PyTorch) into the process."""
+
+    def __init__(self, n: int, dtype, device: int = 0):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(load_library().stm_device_alloc(device, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, device: int = 0) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.size, a.dtype, device)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size > self.n:
+            raise ValueError("array larger than the buffer")
+        _check(load_library().stm_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, n: Optional[int] = None) -> np.ndarray:
+        n = self.n if n is None else int(n)
+        out = np.empty(n, dtype=self.dtype)
+        _check(load_library().stm_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            load_library().stm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -191,9 +241,9 @@ class StmTree:
             raise ValueError("keys and vals differ in length")
         _check(self._lib.stm_bulk_load(self._h, _ptr(k), _ptr(v), k.size))
 
-    def bulk_load_device(self, keys_t, vals_t) -> None:
-        _check(self._lib.stm_bulk_load_device(self._h, keys_t.data_ptr(), vals_t.data_ptr(),
-                                              keys_t.numel()))
+    def bulk_load_device(self, keys: "DeviceBuffer", vals: "DeviceBuffer", n: Optional[int] = None) -> None:
+        n = keys.n if n is None else int(n)
+        _check(self._lib.stm_bulk_load_device(self._h, keys.ptr, vals.ptr, n))
 
     def run_batch(self, keys, vals, ops):
         """Runs one batch; returns (values u64[n], found u8[n])."""
@@ -208,11 +258,17 @@ class StmTree:
                                        _ptr(o), n, _ptr(out_v), _ptr(out_f)))
         return out_v, out_f
 
-    def run_batch_device(self, keys_t, vals_t, ops_t, out_vals_t, out_found_t) -> None:
-        """Device-resident batch (torch tensors on the tree's device)."""
+    def run_batch_device(self, keys: "DeviceBuffer", vals: "DeviceBuffer", ops: "DeviceBuffer",
+                         out_vals: "DeviceBuffer", out_found: "DeviceBuffer",
+                         n: Optional[int] = None, offset: int = 0) -> None:
+        """Device-resident batch of n ops starting at element `offset` of each buffer."""
+        n = keys.n - offset if n is None else int(n)
         _check(self._lib.stm_run_batch_device(
-            self._h, keys_t.data_ptr(), vals_t.data_ptr() if vals_t is not None else None,
-            ops_t.data_ptr(), keys_t.numel(), out_vals_t.data_ptr(), out_found_t.data_ptr()))
+            self._h, keys.ptr + 8 * offset, vals.ptr + 8 * offset, ops.ptr + offset, n,
+            out_vals.ptr + 8 * offset, out_found.ptr + offset))
+
+    def sync(self) -> None:
+        _check(self._lib.stm_sync(self._h))
 
     def size(self) -> int:
         n = C.c_uint64()

@@ -711,4 +711,37 @@ int stm_get_stream(stm_tree* t, void** stream) {
   return STM_OK;
 }
 
+int stm_device_alloc(int device, size_t bytes, void** ptr) {
+  if (!ptr) return fail(STM_ERR_INVALID, "null argument");
+  *ptr = nullptr;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+  return STM_OK;
+}
+
+int stm_device_free(void* ptr) {
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return STM_OK;
+}
+
+int stm_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return STM_OK;
+  if (!dst || !src) return fail(STM_ERR_INVALID, "null argument");
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return STM_OK;
+}
+
+int stm_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return STM_OK;
+  if (!dst || !src) return fail(STM_ERR_INVALID, "null argument");
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return STM_OK;
+}
+
+int stm_sync(stm_tree* t) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  return STM_OK;
+}
+
 }  // extern "C"

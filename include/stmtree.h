@@ -126,8 +126,8 @@ int stm_bulk_load_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t*
 int stm_run_batch(stm_tree* tree, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
                   size_t n, uint64_t* out_vals, uint8_t* out_found);
 
-/* Same with device-resident inputs and outputs.  Work is queued on the tree's stream
- * (stm_get_stream); the call returns after the batch has completed. */
+/* Same with device-resident inputs and outputs (d_vals is required).  Work is queued on
+ * the tree's stream (stm_get_stream); the call returns after the batch has completed. */
 int stm_run_batch_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
                          const uint8_t* d_ops, size_t n, uint64_t* d_out_vals, uint8_t* d_out_found);
 
@@ -151,6 +151,16 @@ int stm_get_stream(stm_tree* tree, void** stream);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* stm_last_error(void);
+
+/* Device-buffer plumbing for callers that keep batches resident in HBM (benchmarks,
+ * serving loops) without a second GPU runtime in the process.  No reference counterpart:
+ * the reference allocates its buffers inside prepareGPU (bpt-gpustm.cpp:196-311). */
+int stm_device_alloc(int device, size_t bytes, void** ptr);
+int stm_device_free(void* ptr);
+int stm_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int stm_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Blocks until every kernel queued on the tree's stream has finished. */
+int stm_sync(stm_tree* tree);
 
 #ifdef __cplusplus
 }

@@ -183,7 +183,7 @@ def test_pool_exhaustion_reports_nomem():
 
 
 def test_device_api_matches_host_api(oracle_mod):
-    import torch
+    from eirene_amd import DeviceBuffer
     rng = np.random.default_rng(9)
     load = rng.integers(0, 1 << 63, size=10000, dtype=np.uint64)
     n = 30000
@@ -193,14 +193,12 @@ def test_device_api_matches_host_api(oracle_mod):
     o = oracle_mod.Oracle()
     o.bulk_load(load, load)
     want = o.run_batch(keys, vals, ops)
-    dev = torch.device("cuda:0")
-    as_t = lambda a: torch.from_numpy(a.view(np.int64) if a.dtype == np.uint64 else a).to(dev)
     with tree() as t:
-        t.bulk_load_device(as_t(load), as_t(load))
-        ov = torch.zeros(n, dtype=torch.int64, device=dev)
-        of = torch.zeros(n, dtype=torch.uint8, device=dev)
-        t.run_batch_device(as_t(keys), as_t(vals), as_t(ops), ov, of)
-        torch.cuda.synchronize()
-        got = (ov.cpu().numpy().view(np.uint64), of.cpu().numpy())
-        assert_same(got, want)
+        t.bulk_load_device(DeviceBuffer.from_array(load), DeviceBuffer.from_array(load))
+        dk, dv, do = (DeviceBuffer.from_array(a) for a in (keys, vals, ops))
+        ov, of = DeviceBuffer(n, np.uint64), DeviceBuffer(n, np.uint8)
+        half = n // 2  # two device batches back to back over one set of buffers
+        t.run_batch_device(dk, dv, do, ov, of, n=half)
+        t.run_batch_device(dk, dv, do, ov, of, n=n - half, offset=half)
+        assert_same((ov.download(), of.download()), want)
         assert_dump(t, o)
