@@ -63,7 +63,7 @@ struct Scratch {
   void* fold_tiles = nullptr;
   uint64_t *ukey = nullptr, *uval = nullptr;
   uint8_t* uop = nullptr;
-  uint32_t *gstart = nullptr, *npend = nullptr, *ulist = nullptr, *dlist = nullptr;
+  uint32_t *gstart = nullptr, *npend = nullptr, *dlist = nullptr;
   BatchCounters* counters = nullptr;
 
   hipError_t alloc(uint64_t cap, bool staging) {
@@ -79,14 +79,14 @@ struct Scratch {
     A_(ft, fold_scratch_bytes(cap));
     fold_tiles = ft;
     A_(ukey, cap); A_(uval, cap); A_(uop, cap); A_(gstart, cap); A_(npend, cap);
-    A_(ulist, cap); A_(dlist, cap);
+    A_(dlist, cap);
     A_(counters, 1);
 #undef A_
     return hipSuccess;
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
-                  hist, fold_tiles, ukey, uval, uop, gstart, npend, ulist, dlist, counters};
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, counters};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = Scratch();
@@ -208,7 +208,6 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   fa.uop = s.uop;
   fa.gstart = s.gstart;
   fa.npend = s.npend;
-  fa.ulist = s.ulist;
   fa.counters = s.counters;
   fa.tile_scratch = s.fold_tiles;
   HIP_TRY(fold_batch(fa, st));
@@ -234,17 +233,15 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   UpsertArgs ua{};
   ua.ukey = s.ukey;
   ua.uval = s.uval;
+  ua.uop = s.uop;
   ua.gstart = s.gstart;
   ua.npend = s.npend;
   ua.sidx = s.sidx;
-  ua.ulist = s.ulist;
   ua.counters = s.counters;
   ua.dlist = s.dlist;
   ua.max_attempts = t->cfg.max_attempts;
-  ua.per_wave = t->cfg.upsert_ops_per_wave;
-  // Enough wavefronts to cover every CU several times; each drains the shared queue.
-  const uint64_t waves_needed = (n + ua.per_wave - 1) / ua.per_wave;
-  ua.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves_needed + 3) / 4, 2048));
+  // One wavefront per 512 groups, at most 8192 wavefronts (every SIMD fully occupied).
+  ua.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 2047) / 2048, 2048));
   ua.out_vals = out_vals;
   ua.out_found = out_found;
   HIP_TRY(launch_upsert(t->view, ua, st));

@@ -81,29 +81,32 @@ struct TreeView {
 };
 
 // ---------------------------------------------------------------- memory-model helpers
-// Agent-scope atomics: the node pool is written by transactions on any XCD while other
-// XCDs read it, so the transactional path uses the HIP/LLVM agent-scope memory model
-// (relaxed atomic loads/stores of 64-bit words, acquire/release fences around the
-// version checks).  The read-only lookup kernel runs when no writer is active and uses
-// plain loads; kernel boundaries order it against the transactional kernels.
-STM_DEV uint64_t ld_acquire(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
+// The node pool is written by transactions on any XCD while other XCDs read it.  Every
+// access of the transactional path to node words, lock words and the tree header is an
+// agent-scope atomic (gfx950: global_load/global_store with sc1, CAS in L2), so each
+// single access is coherent across the eight XCD L2 caches of the device-local pool.
+// Ordering between those accesses only needs the issuing wavefront to wait for
+// completion: wave_order() waits until every earlier load has returned and every earlier
+// store has been acknowledged at device scope before any later access issues.  The
+// agent-scope fences of the LLVM model would add an L2 write-back (buffer_wbl2 sc1) and
+// invalidate (buffer_inv sc1), which only matter for non-atomic accesses; none exist on
+// this path.  The read-only lookup kernel uses plain loads and runs only when no writer
+// is active; kernel boundaries order it against the transactional kernels.
 STM_DEV uint64_t ld_relaxed(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 STM_DEV void st_relaxed(uint64_t* p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-STM_DEV void st_release(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 STM_DEV bool cas_u64(uint64_t* p, uint64_t expect, uint64_t desired) {
   return __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED,
                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-STM_DEV void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
-STM_DEV void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+STM_DEV void wave_order() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);  // no compiler motion across this point
+  __builtin_amdgcn_s_waitcnt(0);            // vmcnt(0) expcnt(0) lgkmcnt(0)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 // ---------------------------------------------------------------- wavefront helpers
 STM_DEV int lane_id() { return __lane_id(); }

@@ -93,27 +93,24 @@ STM_DEV Seg seg_element(const FoldArgs& a, uint32_t i, uint64_t key, uint64_t pr
   return e;
 }
 
-// Aggregate of the FOLD_ITEMS consecutive elements owned by this thread.
-STM_DEV Seg thread_aggregate(const FoldArgs& a, uint32_t i0) {
-  Seg s = seg_identity();
-  if (i0 >= a.n) return s;
-  uint64_t prev = (i0 > 0) ? a.skeys[i0 - 1] : 0;
-  const uint32_t i1 = min(i0 + FOLD_ITEMS, a.n);
-  for (uint32_t i = i0; i < i1; ++i) {
-    const uint64_t k = a.skeys[i];
-    s = seg_combine(s, seg_element(a, i, k, prev));
-    prev = k;
-  }
-  return s;
-}
-
+// A tile is processed in FOLD_ITEMS rounds of FOLD_BLOCK consecutive elements: thread t
+// owns element round * FOLD_BLOCK + t, so every global access of a round is coalesced and
+// the group outputs of a round land in consecutive slots.
 __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_reduce(FoldArgs a, Seg* tile_agg) {
   __shared__ Seg s_w[FOLD_WAVES];
-  const uint32_t i0 = blockIdx.x * FOLD_TILE + threadIdx.x * FOLD_ITEMS;
-  Seg t = thread_aggregate(a, i0);
-  Seg total;
-  (void)block_exclusive_seg(t, s_w, total);
-  if (threadIdx.x == 0) tile_agg[blockIdx.x] = total;
+  const uint32_t tile0 = blockIdx.x * FOLD_TILE;
+  Seg acc = seg_identity();
+  for (int r = 0; r < FOLD_ITEMS; ++r) {
+    const uint32_t r0 = tile0 + r * FOLD_BLOCK;
+    if (r0 >= a.n) break;  // uniform over the block
+    const uint32_t i = r0 + threadIdx.x;
+    Seg e = seg_identity();
+    if (i < a.n) e = seg_element(a, i, a.skeys[i], i ? a.skeys[i - 1] : 0);
+    Seg total;
+    (void)block_exclusive_seg(e, s_w, total);
+    acc = seg_combine(acc, total);
+  }
+  if (threadIdx.x == 0) tile_agg[blockIdx.x] = acc;
 }
 
 __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, uint32_t ntiles,
@@ -142,31 +139,36 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, u
 
 __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg* tile_prefix) {
   __shared__ Seg s_w[FOLD_WAVES];
-  const uint32_t i0 = blockIdx.x * FOLD_TILE + threadIdx.x * FOLD_ITEMS;
-  Seg t = thread_aggregate(a, i0);
-  Seg total;
-  Seg bex = block_exclusive_seg(t, s_w, total);
-  if (blockIdx.x * FOLD_TILE >= a.n) return;  // whole block out of range (uniform)
-  Seg x = seg_combine(tile_prefix[blockIdx.x], bex);
-  uint64_t prev = (i0 > 0 && i0 < a.n) ? a.skeys[i0 - 1] : 0;
-  uint64_t k = (i0 < a.n) ? a.skeys[i0] : 0;
-  // Uniform trip count so the upsert-list append can be aggregated per wavefront.
-  for (uint32_t j = 0; j < FOLD_ITEMS; ++j) {
-    const uint32_t i = i0 + j;
+  __shared__ uint32_t s_up;
+  if (threadIdx.x == 0) s_up = 0;
+  uint32_t n_up = 0;  // merged INSERTs found by this wavefront (lane 0 keeps the count)
+  const uint32_t tile0 = blockIdx.x * FOLD_TILE;
+  Seg carry = tile_prefix[blockIdx.x];  // state of every element before this round
+  for (int r = 0; r < FOLD_ITEMS; ++r) {
+    const uint32_t r0 = tile0 + r * FOLD_BLOCK;
+    if (r0 >= a.n) break;  // uniform over the block
+    const uint32_t i = r0 + threadIdx.x;
     const bool valid = i < a.n;
+    uint64_t k = 0, next = 0;
+    Seg e = seg_identity();
+    if (valid) {
+      k = a.skeys[i];
+      next = (i + 1 < a.n) ? a.skeys[i + 1] : k;
+      e = seg_element(a, i, k, i ? a.skeys[i - 1] : 0);
+    }
+    Seg total;
+    const Seg before = seg_combine(carry, block_exclusive_seg(e, s_w, total));
+    carry = seg_combine(carry, total);
     bool append = false;
     uint32_t g = 0;
-    uint64_t next = k;
     if (valid) {
-      next = (i + 1 < a.n) ? a.skeys[i + 1] : k;
-      const Seg e = seg_element(a, i, k, prev);
-      const int32_t last_before = e.h ? -1 : x.last;  // latest earlier insert of this key
+      const int32_t last_before = e.h ? -1 : before.last;  // latest earlier insert of this key
       if (a.out_vals && last_before >= 0) {
         const uint32_t pos = a.sidx[i];
         a.out_vals[pos] = a.vals[a.sidx[last_before]];
         a.out_found[pos] = 1;
       }
-      x = seg_combine(x, e);
+      const Seg x = seg_combine(before, e);
       const bool group_end = (i + 1 == a.n) || next != k;
       if (group_end) {
         g = x.c - 1;
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
         } else if (x.last >= 0) {
           op = OP_INSERT;
           a.uval[g] = a.vals[a.sidx[x.last]];
-          append = a.ulist != nullptr;
+          append = true;
         } else {
           op = OP_GET;
         }
@@ -190,16 +192,13 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
       }
     }
     const uint64_t want = ballot(append);
-    if (want) {
-      uint32_t base = 0;
-      const int leader = __ffsll((unsigned long long)want) - 1;
-      if (lane_id() == leader) base = atomicAdd(&a.counters->n_upserts, (uint32_t)__popcll(want));
-      base = __shfl(base, leader, WAVE);
-      if (append) a.ulist[base + (uint32_t)__popcll(want & lanemask_lt())] = g;
-    }
-    prev = k;
-    k = next;
+    if (lane_id() == 0) n_up += (uint32_t)__popcll(want);
   }
+  // One counter update per workgroup: a single hot atomic per wavefront round would
+  // serialize the whole kernel on one L2 channel.
+  if (lane_id() == 0 && n_up) atomicAdd(&s_up, n_up);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_up) atomicAdd(&a.counters->n_upserts, s_up);
 }
 
 size_t fold_scratch_bytes(uint64_t n) {

@@ -40,7 +40,6 @@ struct FoldArgs {
   uint8_t* uop;           // OP_GET, OP_INSERT or UOP_SKIP
   uint32_t* gstart;       // sorted position of the group's first op
   uint32_t* npend;        // ops answered by the tree (prefix of the group)
-  uint32_t* ulist;        // groups with merged INSERT (unordered)
   BatchCounters* counters;
   void* tile_scratch;     // fold_scratch_bytes(n)
 };

@@ -244,15 +244,16 @@ STM_DEV uint32_t parent_slot(const WNode<F>& p, uint64_t key, uint32_t child) {
   return cle - 1;
 }
 
-// Optimistic read of a node: version word (acquire), then content.  The content is only
-// trusted after the version is re-validated: by the commit CAS for nodes the transaction
-// writes, by fences/levels for the nodes it only passes through.
+// Optimistic read of a node: version word, then (ordered after it) the content.  The
+// content is only trusted after the version is re-validated: by the commit CAS for nodes
+// the transaction writes, by fences and levels for the nodes it only passes through.
 template <int F>
 STM_DEV bool tx_read(const TreeView& t, uint32_t id, WNode<F>& n) {
   if (id >= t.capacity) return false;
   NodeMeta* m = t.meta + id;
-  const uint64_t w = uniform64(ld_acquire(&m->lock));
+  const uint64_t w = uniform64(ld_relaxed(&m->lock));
   if (w & 1) return false;
+  wave_order();  // content loads issue only after the version word has been read
   const uint64_t info = uniform64(ld_relaxed(&m->info));
   n.low = uniform64(ld_relaxed(&m->low));
   n.high = uniform64(ld_relaxed(&m->high));
@@ -314,13 +315,13 @@ STM_DEV bool tx_lock(uint64_t* w0, uint64_t v0, uint64_t* w1, uint64_t v1, uint6
     if (w && ok) st_relaxed(w, v);
     return false;
   }
-  fence_release();  // the lock bits are visible before any content store
+  wave_order();  // the lock bits are held before any content store issues
   return true;
 }
 
 STM_DEV void tx_unlock(uint64_t* w0, uint64_t v0, uint64_t* w1, uint64_t v1, uint64_t* w2,
                        uint64_t v2) {
-  fence_release();  // content stores are visible before the new versions
+  wave_order();  // every content store is acknowledged before the new versions
   const int l = lane_id();
   uint64_t* w = l == 0 ? w0 : (l == 1 ? w1 : (l == 2 ? w2 : nullptr));
   const uint64_t v = l == 0 ? v0 : (l == 1 ? v1 : v2);
@@ -344,14 +345,15 @@ STM_DEV void tx_claim_new(const TreeView& t, uint32_t a, uint32_t b) {
   const int l = lane_id();
   if (l == 0 && a != NODE_NONE) st_relaxed(&t.meta[a].lock, 1ull);
   if (l == 1 && b != NODE_NONE) st_relaxed(&t.meta[b].lock, 1ull);
-  fence_release();
+  wave_order();
 }
 
-enum TxResult { TX_DONE = 0, TX_ABORT = 1, TX_RESTART = 2, TX_FAIL = 3 };
+enum TxResult { TX_DONE = 0, TX_ABORT = 1, TX_RESTART = 2, TX_FAIL = 3, TX_NEED_PARENT = 4 };
 
 struct TxCounters {
   uint32_t commits = 0, aborts = 0, leaf_splits = 0, inner_splits = 0, root_splits = 0;
   uint32_t new_keys = 0;
+  bool split_happened = false;  // set by a committed leaf split (the parent changed)
 };
 
 // Inserts separator (sep -> rid) right after slot ps of parent p (which has room).
@@ -442,11 +444,51 @@ STM_DEV TxResult tx_split_inner(const TreeView& t, const WNode<F>& n, const WNod
   return TX_RESTART;
 }
 
-// Upsert of (key, val) into leaf n.  pn is its parent when has_par (else n is the root).
+// Test-and-set lock of a leaf, then its content.  A 64-bit fetch_or takes the lock bit
+// without knowing the version; the word it returns is the unlocked version.  Holding the
+// lock, the content cannot change, so no re-validation is needed.  On success the caller
+// owns the lock and must release it (leaf_release) or commit (version + 2).
+template <int F>
+STM_DEV bool tx_lock_leaf(const TreeView& t, uint32_t id, WNode<F>& n) {
+  if (id >= t.capacity) return false;
+  NodeMeta* m = t.meta + id;
+  uint64_t old = 0;
+  if (lane_id() == 0)
+    old = __hip_atomic_fetch_or(&m->lock, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  old = readlane64(old, 0);
+  if (old & 1) return false;  // held by another transaction; the word is unchanged
+  wave_order();               // content loads issue only after the lock is held
+  const uint64_t info = uniform64(ld_relaxed(&m->info));
+  n.low = uniform64(ld_relaxed(&m->low));
+  n.high = uniform64(ld_relaxed(&m->high));
+  n.next = (uint32_t)uniform64(ld_relaxed(&m->next));
+  const uint64_t* kp = t.keys + (size_t)id * F;
+  const uint64_t* vp = t.vals + (size_t)id * F;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) {
+    const int s = j * WAVE + lane_id();
+    n.k[j] = s < F ? ld_relaxed(kp + s) : KEY_EMPTY;
+    n.v[j] = s < F ? ld_relaxed(vp + s) : 0;
+  }
+  n.ver = old;
+  n.id = id;
+  n.count = meta_count(info);
+  n.level = meta_level(info);
+  return true;
+}
+
+// Gives a leaf lock back untouched (nothing was written under it).
+STM_DEV void leaf_release(uint64_t* lw, uint64_t ver) {
+  if (lane_id() == 0) st_relaxed(lw, ver);
+}
+
+// Upsert of (key, val) into leaf n, which this wavefront has locked (tx_lock_leaf).  pn
+// is its parent when has_par (else n is the root).  Every return other than TX_DONE and
+// TX_NEED_PARENT has released the leaf; TX_NEED_PARENT keeps it locked.
 template <int F>
 STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNode<F>& pn,
-                                bool has_par, uint64_t hver, uint64_t key, uint64_t val,
-                                bool& found, uint64_t& old, TxCounters& c) {
+                                bool has_par, bool par_fresh, uint64_t hver, uint64_t key,
+                                uint64_t val, bool& found, uint64_t& old, TxCounters& c) {
   constexpr int SPL = WNode<F>::SPL;
   const uint32_t cle = wave_count_le<F>(n.k, key);
   const uint32_t s = cle ? cle - 1 : 0;
@@ -454,7 +496,6 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
   old = found ? wave_at<F>(n.v, s) : 0;
   uint64_t* lw = &t.meta[n.id].lock;
   if (found) {
-    if (!tx_lock(lw, n.ver, nullptr, 0, nullptr, 0)) return TX_ABORT;
 #pragma unroll
     for (int j = 0; j < SPL; ++j)
       if (j * WAVE + lane_id() == (int)s) st_relaxed(t.vals + (size_t)n.id * F + s, val);
@@ -464,7 +505,6 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
   }
   const uint32_t pos = cle;  // insert position
   if (n.count < (uint32_t)F) {
-    if (!tx_lock(lw, n.ver, nullptr, 0, nullptr, 0)) return TX_ABORT;
     uint64_t nk[SPL], nv[SPL];
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
@@ -481,16 +521,26 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
     return TX_DONE;
   }
   // Full leaf: split F+1 entries into h (left) and F+1-h (right).
+  if (has_par && !par_fresh) return TX_NEED_PARENT;
   const uint32_t h = (F + 1) / 2;
   uint32_t ps = 0;
   if (has_par) {
-    if (pn.count >= (uint32_t)F) return TX_ABORT;
+    if (pn.count >= (uint32_t)F) {
+      leaf_release(lw, n.ver);
+      return TX_ABORT;
+    }
     ps = parent_slot<F>(pn, key, n.id);
-    if (ps == NODE_NONE) return TX_ABORT;
+    if (ps == NODE_NONE) {
+      leaf_release(lw, n.ver);
+      return TX_ABORT;
+    }
   }
   uint64_t* pw = has_par ? &t.meta[pn.id].lock : nullptr;
   uint64_t* hw = has_par ? nullptr : &t.hdr->lock;
-  if (!tx_lock(lw, n.ver, pw, has_par ? pn.ver : 0, hw, hver)) return TX_ABORT;
+  if (!tx_lock(pw, has_par ? pn.ver : 0, hw, hver, nullptr, 0)) {
+    leaf_release(lw, n.ver);
+    return TX_ABORT;
+  }
   const uint32_t base = tx_alloc(t, has_par ? 1u : 2u);
   if (base == NODE_NONE) {
     tx_unlock(lw, n.ver - 2, pw, has_par ? pn.ver - 2 : 0, hw, hver - 2);
@@ -552,40 +602,90 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
   c.leaf_splits++;
   c.commits++;
   c.new_keys++;
+  c.split_happened = true;
   return TX_DONE;
 }
+
+// The inner node of level 1 that routed the wavefront's previous upsert.  Its content may
+// be stale by the time it is used again; a stale route is caught by the leaf's fence check
+// (and, for a split, by re-reading the parent with its version).
+template <int F>
+struct ParentCache {
+  WNode<F> n;
+  bool valid = false;
+};
 
 // One attempt of the upsert transaction for (key, val).
 template <int F>
 STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val, bool& found,
-                                   uint64_t& old, TxCounters& c) {
-  const uint64_t hver = uniform64(ld_acquire(&t.hdr->lock));
-  if (hver & 1) return TX_ABORT;
-  const uint32_t root = (uint32_t)uniform64(ld_relaxed(&t.hdr->root));
-  const uint32_t height = (uint32_t)uniform64(ld_relaxed(&t.hdr->height));
-  if (height == 0 || height > 64) return TX_ABORT;
+                                   uint64_t& old, TxCounters& c, ParentCache<F>& pc) {
   WNode<F> par, cur;
   bool has_par = false;
-  uint32_t id = root;
-  uint32_t lvl = height - 1;
-  for (uint32_t step = 0; step < height; ++step) {
-    if (!tx_read<F>(t, id, cur)) return TX_ABORT;
-    if (cur.level != lvl || key < cur.low || key >= cur.high) return TX_ABORT;
-    if (cur.level == 0) {
-      return tx_leaf_upsert<F>(t, cur, par, has_par, hver, key, val, found, old, c);
+  bool par_fresh = false;  // par was read with its version during this attempt
+  uint64_t hver = 0;
+  uint32_t leaf_id;
+  if (pc.valid && key >= pc.n.low && key < pc.n.high) {
+    // Route through the cached level-1 node; the tree never loses a level, so no header
+    // read is needed.  The leaf's fences validate the route.
+    const uint32_t cle = wave_count_le<F>(pc.n.k, key);
+    if (cle == 0 || cle > pc.n.count) {
+      pc.valid = false;
+      return TX_ABORT;
     }
-    if (cur.count >= (uint32_t)F) return tx_split_inner<F>(t, cur, par, has_par, hver, key, c);
-    if (cur.count == 0) return TX_ABORT;
-    const uint32_t cle = wave_count_le<F>(cur.k, key);
-    if (cle == 0) return TX_ABORT;
-    const uint64_t child = wave_at<F>(cur.v, cle - 1);
-    if (child >= t.capacity) return TX_ABORT;
-    par = cur;
+    leaf_id = (uint32_t)wave_at<F>(pc.n.v, cle - 1);
+    par = pc.n;
     has_par = true;
-    id = (uint32_t)child;
-    --lvl;
+  } else {
+    hver = uniform64(ld_relaxed(&t.hdr->lock));
+    if (hver & 1) return TX_ABORT;
+    wave_order();
+    const uint32_t root = (uint32_t)uniform64(ld_relaxed(&t.hdr->root));
+    const uint32_t height = (uint32_t)uniform64(ld_relaxed(&t.hdr->height));
+    if (height == 0 || height > 64) return TX_ABORT;
+    leaf_id = root;
+    if (height >= 2) {
+      uint32_t id = root;
+      for (uint32_t lvl = height - 1; lvl >= 1; --lvl) {
+        if (!tx_read<F>(t, id, cur)) return TX_ABORT;
+        if (cur.level != lvl || key < cur.low || key >= cur.high) return TX_ABORT;
+        if (cur.count >= (uint32_t)F) {
+          pc.valid = false;
+          return tx_split_inner<F>(t, cur, par, has_par, hver, key, c);
+        }
+        const uint32_t cle = wave_count_le<F>(cur.k, key);
+        if (cle == 0 || cle > cur.count) return TX_ABORT;
+        const uint64_t child = wave_at<F>(cur.v, cle - 1);
+        if (child >= t.capacity) return TX_ABORT;
+        par = cur;
+        has_par = true;
+        id = (uint32_t)child;
+      }
+      pc.n = par;
+      pc.valid = true;
+      par_fresh = true;
+      leaf_id = id;
+    }
   }
-  return TX_ABORT;
+  if (!tx_lock_leaf<F>(t, leaf_id, cur)) return TX_ABORT;
+  if (cur.level != 0 || cur.count > (uint32_t)F || key < cur.low || key >= cur.high) {
+    leaf_release(&t.meta[leaf_id].lock, cur.ver);
+    pc.valid = false;
+    return TX_ABORT;
+  }
+  TxResult r = tx_leaf_upsert<F>(t, cur, par, has_par, par_fresh, hver, key, val, found, old, c);
+  if (r == TX_NEED_PARENT) {
+    // The leaf splits: the parent's current version and content are needed (the leaf
+    // stays locked meanwhile).
+    if (!tx_read<F>(t, par.id, par) || par.level != 1 || key < par.low || key >= par.high) {
+      leaf_release(&t.meta[leaf_id].lock, cur.ver);
+      pc.valid = false;
+      return TX_ABORT;
+    }
+    r = tx_leaf_upsert<F>(t, cur, par, true, true, hver, key, val, found, old, c);
+  }
+  if (r != TX_DONE || c.split_happened) pc.valid = false;
+  c.split_happened = false;
+  return r;
 }
 
 STM_DEV void backoff(uint32_t attempt) {
@@ -593,20 +693,18 @@ STM_DEV void backoff(uint32_t attempt) {
   for (uint32_t i = 0; i < spins; ++i) __builtin_amdgcn_s_sleep(4);
 }
 
-// Runs merged upsert op g to completion (or gives up after max_attempts).
+// Runs one merged upsert to completion (or gives up after max_attempts) and answers the
+// np ops of its group that wait for the tree (sorted positions gs .. gs+np-1).
 template <int F>
-STM_DEV bool run_upsert(const TreeView& t, uint32_t g, const uint64_t* __restrict__ ukey,
-                        const uint64_t* __restrict__ uval, const uint32_t* __restrict__ gstart,
-                        const uint32_t* __restrict__ npend, const uint32_t* __restrict__ sidx,
-                        uint64_t* __restrict__ out_vals, uint8_t* __restrict__ out_found,
-                        uint32_t max_attempts, TxCounters& c) {
-  const uint64_t key = ukey[g];
-  const uint64_t val = uval[g];
+STM_DEV bool run_upsert(const TreeView& t, uint64_t key, uint64_t val, uint32_t gs, uint32_t np,
+                        const uint32_t* __restrict__ sidx, uint64_t* __restrict__ out_vals,
+                        uint8_t* __restrict__ out_found, uint32_t max_attempts, TxCounters& c,
+                        ParentCache<F>& pc) {
   bool found = false;
   uint64_t old = 0;
   uint32_t aborts = 0, restarts = 0;
   for (;;) {
-    const TxResult r = tx_upsert_attempt<F>(t, key, val, found, old, c);
+    const TxResult r = tx_upsert_attempt<F>(t, key, val, found, old, c, pc);
     if (r == TX_DONE) break;
     if (r == TX_FAIL) return true;  // pool exhausted: error bit raised, op dropped
     if (r == TX_ABORT) {
@@ -618,8 +716,6 @@ STM_DEV bool run_upsert(const TreeView& t, uint32_t g, const uint64_t* __restric
     }
   }
   if (out_vals) {
-    const uint32_t gs = gstart[g];
-    const uint32_t np = npend[g];
     for (uint32_t p = lane_id(); p < np; p += WAVE) {
       const uint32_t pos = sidx[gs + p];
       out_vals[pos] = old;
@@ -643,36 +739,51 @@ STM_DEV void flush_counters(const TreeView& t, const TxCounters& c) {
   }
 }
 
-// Work-queue kernel: each wavefront takes merged upserts from ulist until it is drained.
+// Each wavefront owns a contiguous range of the batch's key-sorted groups and runs the
+// merged INSERTs in it in key order, so consecutive upserts mostly reuse the cached
+// parent and different wavefronts work on disjoint key ranges.
 template <int F>
 __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __restrict__ ukey,
                                                 const uint64_t* __restrict__ uval,
+                                                const uint8_t* __restrict__ uop,
                                                 const uint32_t* __restrict__ gstart,
                                                 const uint32_t* __restrict__ npend,
                                                 const uint32_t* __restrict__ sidx,
-                                                const uint32_t* __restrict__ ulist,
                                                 BatchCounters* cnt, uint32_t* __restrict__ dlist,
-                                                uint32_t max_attempts, uint32_t per_wave,
+                                                uint32_t max_attempts,
                                                 uint64_t* __restrict__ out_vals,
                                                 uint8_t* __restrict__ out_found) {
-  const uint32_t n_up = cnt->n_upserts;
+  const uint32_t U = cnt->n_unique;
+  const uint32_t nw = gridDim.x * (blockDim.x / WAVE);
+  const uint32_t w = uniform32((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const uint32_t per = (U + nw - 1) / nw;
+  const uint32_t g0 = min(w * per, U);
+  const uint32_t g1 = min(g0 + per, U);
   TxCounters c;
+  ParentCache<F> pc;
   uint32_t upserts = 0, deferred = 0;
-  for (;;) {
-    uint32_t q = 0;
-    if (lane_id() == 0) q = atomicAdd(&cnt->queue_head, per_wave);
-    q = uniform32(__shfl(q, 0, WAVE));
-    if (q >= n_up) break;
-    const uint32_t qe = min(q + per_wave, n_up);
-    for (uint32_t e = q; e < qe; ++e) {
-      const uint32_t g = ulist[e];
+  for (uint32_t base = g0; base < g1; base += WAVE) {
+    // Op data of the round's upserts, loaded together (coalesced) and read lane by lane.
+    const uint32_t gi = base + lane_id();
+    const bool mine = gi < g1 && uop[gi] == OP_INSERT;
+    uint64_t todo = ballot(mine);
+    const uint64_t k_l = mine ? ukey[gi] : 0;
+    const uint64_t v_l = mine ? uval[gi] : 0;
+    const uint32_t gs_l = mine ? gstart[gi] : 0;
+    const uint32_t np_l = mine ? npend[gi] : 0;
+    while (todo) {
+      const int l = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const uint64_t key = readlane64(k_l, l);
+      const uint64_t val = readlane64(v_l, l);
+      const uint32_t gs = __builtin_amdgcn_readlane(gs_l, l);
+      const uint32_t np = __builtin_amdgcn_readlane(np_l, l);
       upserts++;
-      if (!run_upsert<F>(t, g, ukey, uval, gstart, npend, sidx, out_vals, out_found,
-                         max_attempts, c)) {
+      if (!run_upsert<F>(t, key, val, gs, np, sidx, out_vals, out_found, max_attempts, c, pc)) {
         uint32_t slot = 0;
         if (lane_id() == 0) slot = atomicAdd(&cnt->n_deferred, 1u);
         slot = uniform32(__shfl(slot, 0, WAVE));
-        if (lane_id() == 0) dlist[slot] = g;
+        if (lane_id() == 0) dlist[slot] = base + (uint32_t)l;
         deferred++;
       }
     }
@@ -698,8 +809,11 @@ __global__ __launch_bounds__(64) void k_upsert_serial(TreeView t, const uint64_t
                                                       uint8_t* __restrict__ out_found) {
   const uint32_t n = cnt->n_deferred;
   TxCounters c;
+  ParentCache<F> pc;
   for (uint32_t e = 0; e < n; ++e) {
-    if (!run_upsert<F>(t, dlist[e], ukey, uval, gstart, npend, sidx, out_vals, out_found, 1024u, c)) {
+    const uint32_t g = dlist[e];
+    if (!run_upsert<F>(t, ukey[g], uval[g], gstart[g], npend[g], sidx, out_vals, out_found, 1024u,
+                       c, pc)) {
       if (lane_id() == 0) atomicOr((unsigned long long*)&t.hdr->error, (unsigned long long)ERR_LIVELOCK);
     }
   }
@@ -870,9 +984,9 @@ hipError_t launch_lookup(const TreeView& t, const LookupArgs& a, hipStream_t s) 
 
 template <int F>
 static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
-  k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.ulist,
-                                        a.counters, a.dlist, a.max_attempts, a.per_wave,
-                                        a.out_vals, a.out_found);
+  k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
+                                        a.counters, a.dlist, a.max_attempts, a.out_vals,
+                                        a.out_found);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_upsert_serial<F><<<1, 64, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.dlist,

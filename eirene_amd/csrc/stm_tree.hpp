@@ -22,15 +22,14 @@ struct LookupArgs {
 struct UpsertArgs {
   const uint64_t* ukey;
   const uint64_t* uval;
+  const uint8_t* uop;
   const uint32_t* gstart;
   const uint32_t* npend;
   const uint32_t* sidx;
-  const uint32_t* ulist;
   BatchCounters* counters;
   uint32_t* dlist;
   uint32_t max_attempts;
-  uint32_t per_wave;
-  uint32_t blocks;     // 256-thread workgroups of the work-queue kernel
+  uint32_t blocks;     // 256-thread workgroups; each wavefront owns a range of groups
   uint64_t* out_vals;
   uint8_t* out_found;
 };
