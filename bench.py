@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--fanout", type=int, default=64)
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--group", type=int, default=16)
-    ap.add_argument("--route", choices=["descent", "learned"], default="descent")
+    ap.add_argument("--route", choices=["descent", "learned", "both"], default="both")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -86,54 +86,69 @@ def main():
                                                  0.9, w.DIST_GAMMA, 2.0, 2.0 ** 56)
     log(f"generated {n_batches} batches in {time.time() - t0:.1f}s")
 
-    cfg = ea.StmConfig(fanout=args.fanout, max_keys=n_load + n_batches * n_batch + (1 << 20),
-                       max_batch=n_batch, lanes_per_op=args.lanes, ops_per_group=args.group,
-                       route=ea.ROUTE_LEARNED if args.route == "learned" else ea.ROUTE_DESCENT,
-                       timing=True)
-    tree = ea.StmTree(cfg)
-    t0 = time.time()
-    dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(load_vals)
-    tree.bulk_load_device(dk, dv)
-    dk.free()
-    dv.free()
-    log(f"bulk load {time.time() - t0:.2f}s, stats {tree.stats()['height']} levels")
+    def run_route(route):
+        cfg = ea.StmConfig(fanout=args.fanout, max_keys=n_load + n_batches * n_batch + (1 << 20),
+                           max_batch=n_batch, lanes_per_op=args.lanes, ops_per_group=args.group,
+                           route=ea.ROUTE_LEARNED if route == "learned" else ea.ROUTE_DESCENT,
+                           retrain_every=8, timing=True)
+        tree = ea.StmTree(cfg)
+        t0 = time.time()
+        dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(load_vals)
+        tree.bulk_load_device(dk, dv)
+        dk.free()
+        dv.free()
+        log(f"[{route}] bulk load {time.time() - t0:.2f}s, {tree.stats()['height']} levels")
+        for b in range(args.warmup):
+            tree.run_batch_device(bk, bv, bo, out_v, out_f, n=n_batch, offset=b * n_batch)
+        tree.sync()
+        first = (out_v.download(n_batch), out_f.download(n_batch)) if args.warmup else None
+        tree.reset_stats()
+        lat = []
+        t_start = time.perf_counter()
+        for b in range(args.warmup, n_batches):
+            tb = time.perf_counter()
+            tree.run_batch_device(bk, bv, bo, out_v, out_f, n=n_batch, offset=b * n_batch)
+            lat.append(time.perf_counter() - tb)
+        tree.sync()
+        elapsed = time.perf_counter() - t_start
+        st = tree.stats()
+        tree.close()
+        mops = args.steps * n_batch / elapsed / 1e6
+        log(f"[{route}] {args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s; "
+            f"phases {st['phase_ms']}")
+        return {"mops": mops, "elapsed": elapsed, "lat": lat, "st": st, "first": first}
+
     bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
     out_f = ea.DeviceBuffer(keys.size, np.uint8)
-
-    for b in range(args.warmup):
-        tree.run_batch_device(bk, bv, bo, out_v, out_f, n=n_batch, offset=b * n_batch)
-    tree.sync()
-    tree.reset_stats()
-    lat = []
-    t_start = time.perf_counter()
-    for b in range(args.warmup, n_batches):
-        tb = time.perf_counter()
-        tree.run_batch_device(bk, bv, bo, out_v, out_f, n=n_batch, offset=b * n_batch)
-        lat.append(time.perf_counter() - tb)
-    tree.sync()
-    elapsed = time.perf_counter() - t_start
-    st = tree.stats()
+    routes = ["descent", "learned"] if args.route == "both" else [args.route]
+    res = {r: run_route(r) for r in routes}
+    head = res["learned"] if "learned" in res else res[routes[0]]
+    st, elapsed, lat = head["st"], head["elapsed"], head["lat"]
+    mops = head["mops"]
     total_ops = args.steps * n_batch
-    mops = total_ops / elapsed / 1e6
-    log(f"{args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s; phases {st['phase_ms']}")
+    same = None
+    if len(res) == 2 and res["descent"]["first"] is not None:
+        a_, b_ = res["descent"]["first"], res["learned"]["first"]
+        same = bool(np.array_equal(a_[0], b_[0]) and np.array_equal(a_[1], b_[1]))
 
-    # Roofline of the dominant tree kernel: the read-only lookup kernel.
+    # Roofline of the read-only lookup kernel (the tree kernel every lookup takes).
     lookups = st["unique_ops"] - st["upserts"]
     lk_launch = max(st["phase_launches"]["lookup"], 1)
     lk_ms = st["phase_ms"]["lookup"] / lk_launch
     lk_bytes = lookups / lk_launch * BYTES_LOOKUP
     achieved = lk_bytes / (lk_ms * 1e-3) / 1e9 if lk_ms > 0 else 0.0
     mix_bytes = 0.95 * BYTES_LOOKUP + 0.05 * BYTES_INSERT
+    kname = "k_lookup<%d,%d,%s>" % (args.fanout, args.lanes,
+                                    "true" if "learned" in res else "false")
     roofline = {
-        "kernel": "k_lookup<64,16>" if args.fanout == 64 and args.lanes == 16 else
-                  f"k_lookup<{args.fanout},{args.lanes}>",
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": kname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
         "avg_launch_ms": round(lk_ms, 4), "lookups_per_launch": round(lookups / lk_launch),
         "bytes_per_lookup": BYTES_LOOKUP,
         "whole_batch_gbs": round(total_ops * mix_bytes / elapsed / 1e9, 1),
     }
+    first_gpu = head["first"]
 
     cpu = None
     agree = None
@@ -150,10 +165,8 @@ def main():
         while b < n_batches and (cpu_sec < args.cpu_seconds or b < 2):
             s = slice(b * n_batch, (b + 1) * n_batch)
             cv, cf, sec = ex.run_batch(keys[s], vals[s], ops[s])
-            if b == 0:  # first batch: CPU and GPU start from the same tree
-                gv = out_v.download(n_batch)
-                gf = out_f.download(n_batch)
-                agree = bool(np.array_equal(gv, cv) and np.array_equal(gf, cf))
+            if b == 0 and first_gpu is not None:  # first batch: same starting tree
+                agree = bool(np.array_equal(first_gpu[0], cv) and np.array_equal(first_gpu[1], cf))
             cpu_sec += sec
             cpu_ops += n_batch
             b += 1
@@ -163,6 +176,16 @@ def main():
                "sample": f"first {b} batches of the same workload ({cpu_ops} ops) on a "
                          f"{threads}-thread key-range-partitioned CPU B+tree"}
 
+    def summary(r):
+        x = res[r]
+        s_ = x["st"]
+        return {"mops": round(x["mops"], 2), "ms_per_batch": round(x["elapsed"] / args.steps * 1e3, 3),
+                "p50_batch_ms": round(float(np.percentile(x["lat"], 50)) * 1e3, 3),
+                "p99_batch_ms": round(float(np.percentile(x["lat"], 99)) * 1e3, 3),
+                "phase_ms_per_batch": {k: round(v / args.steps, 4) for k, v in s_["phase_ms"].items()
+                                       if s_["phase_launches"][k]},
+                "route_hits": s_["route_hits"], "route_misses": s_["route_misses"]}
+
     out = {
         "metric": METRIC, "value": round(mops, 2), "unit": "Mops/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -170,19 +193,21 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"learned_gamma: bulk-load 2^{args.load_log2} Gamma(k=2, theta=2^56) "
                                f"keys, {args.steps} batches of 2^{args.batch_log2} ops, 95% lookup "
-                               f"(90% hits) / 5% insert", "global_batch": n_batch,
-                   "route": args.route, "fanout": args.fanout, "lanes_per_op": args.lanes,
+                               f"(90% hits) / 5% insert, router retrained every 8 batches",
+                   "global_batch": n_batch, "value_route": "learned" if "learned" in res else routes[0],
+                   "fanout": args.fanout, "lanes_per_op": args.lanes,
                    "ops_per_group": args.group, "parallelism": "single GPU"},
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
-        "phase_ms_per_batch": {k: round(v / max(st["phase_launches"][k], 1), 4)
-                               for k, v in st["phase_ms"].items() if st["phase_launches"][k]},
+        "routes": {r: summary(r) for r in res},
+        "learned_over_plain": round(res["learned"]["mops"] / res["descent"]["mops"], 4)
+                              if len(res) == 2 else None,
+        "learned_and_plain_results_identical": same,
         "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
                                    "root_splits", "deferred", "n_keys", "height")},
         "roofline": roofline, "cpu_baseline": cpu, "cpu_gpu_first_batch_agree": agree,
     }
     print(json.dumps(out), flush=True)
-    tree.close()
 
 
 if __name__ == "__main__":

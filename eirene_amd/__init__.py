@@ -52,7 +52,7 @@ class _Config(C.Structure):
         ("max_batch", C.c_uint64), ("device", C.c_int32), ("lanes_per_op", C.c_uint32),
         ("ops_per_group", C.c_uint32), ("upsert_ops_per_wave", C.c_uint32),
         ("max_attempts", C.c_uint32), ("route", C.c_uint32), ("timing", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("retrain_every", C.c_uint32), ("train_steps", C.c_uint32), ("reserved", C.c_uint32 * 3),
     ]
 
 
@@ -99,6 +99,9 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
         "stm_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
         "stm_sync": (C.c_int, [_P]),
+        "stm_retrain": (C.c_int, [_P, C.c_uint32]),
+        "stm_route_info": (C.c_int, [_P, _U64P, _U64P, _U64P]),
+        "stm_mlp_gradients": (C.c_int, [_P, _P, C.c_size_t, _P, _P, C.POINTER(C.c_float)]),
         # workload generators (host only)
         "stm_splitmix64": (C.c_uint64, [C.c_uint64]),
         "stm_gen_values": (None, [_P, C.c_size_t, _P]),
@@ -194,12 +197,15 @@ class StmConfig:
     max_attempts: int = 64
     route: int = ROUTE_DESCENT
     timing: bool = False
+    retrain_every: int = 8
+    train_steps: int = 300
 
     def to_c(self) -> _Config:
         c = _Config()
         _check(load_library().stm_config_init(C.byref(c)))
         for f in ("fanout", "leaf_fill", "max_keys", "max_batch", "device", "lanes_per_op",
-                  "ops_per_group", "upsert_ops_per_wave", "max_attempts", "route"):
+                  "ops_per_group", "upsert_ops_per_wave", "max_attempts", "route",
+                  "retrain_every", "train_steps"):
             setattr(c, f, int(getattr(self, f)))
         c.timing = 1 if self.timing else 0
         return c
@@ -301,7 +307,32 @@ class StmTree:
     def reset_stats(self) -> None:
         _check(self._lib.stm_reset_stats(self._h))
 
+    def retrain(self, train_steps: Optional[int] = None) -> None:
+        """Rebuilds the leaf directory and trains the learned router now."""
+        steps = self.config.train_steps if train_steps is None else int(train_steps)
+        _check(self._lib.stm_retrain(self._h, steps))
+
+    def route_info(self) -> dict:
+        """Directory size and window hits / misses of the router on its own training pairs."""
+        n, h, m = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(self._lib.stm_route_info(self._h, C.byref(n), C.byref(h), C.byref(m)))
+        return {"leaves": n.value, "window_hits": h.value, "window_misses": m.value}
+
     def stream(self) -> int:
         p = C.c_void_p()
         _check(self._lib.stm_get_stream(self._h, C.byref(p)))
         return p.value or 0
+
+
+def mlp_gradients(x, t, params):
+    """Loss and gradient of the router MLP computed by the device kernels (float32)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    if p.size != 49 or x.size != t.size:
+        raise ValueError("need 49 parameters and matching x / t")
+    g = np.zeros(49, dtype=np.float32)
+    loss = C.c_float()
+    _check(load_library().stm_mlp_gradients(x.ctypes.data, t.ctypes.data, x.size, p.ctypes.data,
+                                            g.ctypes.data, C.byref(loss)))
+    return float(loss.value), g

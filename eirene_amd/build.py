@@ -27,6 +27,7 @@ HIP_SOURCES = [
     "stm_sort.hip",
     "stm_fold.hip",
     "stm_tree.hip",
+    "stm_learned.hip",
     "stm_api.hip",
 ]
 CPP_SOURCES = [
@@ -87,8 +88,15 @@ def build(force: bool = False, verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # stm_test: the reference-compatible command line, linked against the library.
+    cli = PKG / "stm_test"
+    cmd = ["g++", *CPP_FLAGS, "-o", str(cli), str(CSRC / "stm_test.cpp"), f"-L{PKG}", "-lstmtree",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {cli}")
     return LIB
 
 

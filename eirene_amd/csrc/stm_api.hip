@@ -105,6 +105,11 @@ struct stm_tree {
   hipEvent_t ev_a[STM_NUM_PHASES] = {};
   hipEvent_t ev_b[STM_NUM_PHASES] = {};
   bool phase_used[STM_NUM_PHASES] = {};
+  // learned routing
+  RouteState route;
+  bool route_alloc = false;
+  bool route_valid = false;       // directory + model describe a recent tree
+  uint32_t batches_since_train = 0;
 };
 
 namespace {
@@ -185,6 +190,51 @@ int init_empty(stm_tree* t) {
   return STM_OK;
 }
 
+int alloc_router(stm_tree* t) {
+  if (t->route_alloc) return STM_OK;
+  RouteState& r = t->route;
+  const uint64_t cap = t->capacity;
+  if (dalloc(&r.hdr, 1) || dalloc(&r.fences, cap + 1) || dalloc(&r.lids, cap) ||
+      dalloc(&r.segs, ROUTE_SEGMENTS) || dalloc(&r.dir, cap + 1) || dalloc(&r.rank_of, cap) ||
+      dalloc(&r.lows, cap) || dalloc(&r.lows_tmp, cap) || dalloc(&r.sorted_lows, cap) ||
+      dalloc(&r.order, cap) || dalloc(&r.order_tmp, cap) || dalloc(&r.hist, sort_hist_words(cap)) ||
+      dalloc(&r.partial, 4 * route_partial_blocks()))
+    return fail(STM_ERR_NOMEM, "learned-routing buffers for %llu nodes", (unsigned long long)cap);
+  t->route_alloc = true;
+  t->view.rank_of = r.rank_of;
+  t->view.dir = r.dir;
+  return STM_OK;
+}
+
+void free_router(stm_tree* t) {
+  RouteState& r = t->route;
+  void* ps[] = {r.hdr, r.fences, r.lids, r.segs, r.dir, r.rank_of, r.lows, r.lows_tmp,
+                r.sorted_lows, r.order, r.order_tmp, r.hist, r.partial};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  t->route = RouteState();
+  t->view.rank_of = nullptr;
+  t->view.dir = nullptr;
+  t->route_alloc = false;
+  t->route_valid = false;
+}
+
+// Rebuilds the leaf directory from the tree and trains the router (phase "train").
+int train_router(stm_tree* t, uint32_t steps) {
+  int rc = alloc_router(t);
+  if (rc) return rc;
+  TreeHeader h;
+  rc = read_header(t, &h);
+  if (rc) return rc;
+  const uint32_t nn = (uint32_t)std::min<uint64_t>(h.n_nodes, t->capacity);
+  phase_begin(t, STM_PHASE_TRAIN);
+  HIP_TRY(route_build(t->view, t->route, nn, steps, 0.02f, steps > 0, t->stream));
+  phase_end(t, STM_PHASE_TRAIN);
+  t->route_valid = true;
+  t->batches_since_train = 0;
+  return STM_OK;
+}
+
 // One chunk of at most max_batch ops, all pointers on the device.
 int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
               uint32_t n, uint64_t* out_vals, uint8_t* out_found) {
@@ -213,8 +263,19 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   HIP_TRY(fold_batch(fa, st));
   phase_end(t, STM_PHASE_MERGE);
 
+  const bool learned = t->cfg.route == STM_ROUTE_LEARNED;
+  if (learned && (!t->route_valid || t->batches_since_train >= t->cfg.retrain_every)) {
+    int rc = train_router(t, t->cfg.train_steps);
+    if (rc) return rc;
+  }
   phase_begin(t, STM_PHASE_LOOKUP);
   LookupArgs la{};
+  if (learned) {
+    la.route_hdr = t->route.hdr;
+    la.fences = t->route.fences;
+    la.lids = t->route.lids;
+    la.segs = t->route.segs;
+  }
   la.ukey = s.ukey;
   la.uop = s.uop;
   la.gstart = s.gstart;
@@ -254,6 +315,7 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   t->st.batches += 1;
   t->st.ops += n;
   t->st.unique_ops += bc.n_unique;
+  t->batches_since_train += 1;
   if (bc.n_bad) return fail(STM_ERR_INVALID, "batch holds the reserved key 0xffffffffffffffff");
   return check_device_error(t);
 }
@@ -369,6 +431,7 @@ int build_from_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, s
       h.n_keys = U;
       r2 = hipMemcpyAsync(t->view.hdr, &h, sizeof(h), hipMemcpyHostToDevice, st);
       if (r2 == hipSuccess) r2 = hipStreamSynchronize(st);
+      t->route_valid = false;  // the router describes the previous tree
     }
     (void)hipFree(lows_a);
     (void)hipFree(lows_b);
@@ -399,6 +462,8 @@ int stm_config_init(stm_config* cfg) {
   cfg->max_attempts = 64;
   cfg->route = STM_ROUTE_DESCENT;
   cfg->timing = 0;
+  cfg->retrain_every = 8;
+  cfg->train_steps = 300;
   return STM_OK;
 }
 
@@ -422,6 +487,8 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
   if (cfg.ops_per_group == 0) cfg.ops_per_group = 16;
   if (cfg.upsert_ops_per_wave == 0) cfg.upsert_ops_per_wave = 1;
   if (cfg.max_attempts == 0) cfg.max_attempts = 64;
+  if (cfg.retrain_every == 0) cfg.retrain_every = 8;
+  if (cfg.train_steps > 100000) return fail(STM_ERR_INVALID, "train_steps above 100000");
   if (cfg.route > STM_ROUTE_LEARNED) return fail(STM_ERR_INVALID, "route %u unknown", cfg.route);
   const uint64_t cap = capacity_for(cfg.max_keys, cfg.fanout);
   if (cap >= NODE_NONE) return fail(STM_ERR_INVALID, "max_keys too large for 32-bit node ids");
@@ -445,7 +512,7 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
       (e = dalloc(&t->view.vals, cap * F)) != hipSuccess ||
       (e = dalloc(&t->view.meta, cap)) != hipSuccess ||
       (e = dalloc(&t->view.hdr, 1)) != hipSuccess ||
-      (e = dalloc(&t->view.stats, (size_t)ST_NUM)) != hipSuccess)
+      (e = dalloc(&t->view.stats, (size_t)STATS_WORDS)) != hipSuccess)
     return bail(fail(STM_ERR_NOMEM, "node pool of %llu nodes: %s", (unsigned long long)cap,
                      hipGetErrorString(e)));
   t->view.capacity = (uint32_t)cap;
@@ -454,7 +521,7 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
     return bail(fail(STM_ERR_NOMEM, "batch scratch for %llu ops: %s",
                      (unsigned long long)cfg.max_batch, hipGetErrorString(e)));
   if (hipMemsetAsync(t->view.meta, 0, sizeof(NodeMeta) * cap, t->stream) != hipSuccess ||
-      hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * ST_NUM, t->stream) != hipSuccess)
+      hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * STATS_WORDS, t->stream) != hipSuccess)
     return bail(fail(STM_ERR_DEVICE, "pool init"));
   for (int p = 0; p < STM_NUM_PHASES; ++p) {
     if (hipEventCreate(&t->ev_a[p]) != hipSuccess || hipEventCreate(&t->ev_b[p]) != hipSuccess)
@@ -478,6 +545,7 @@ int stm_destroy(stm_tree* t) {
     if (t->ev_a[p]) (void)hipEventDestroy(t->ev_a[p]);
     if (t->ev_b[p]) (void)hipEventDestroy(t->ev_b[p]);
   }
+  free_router(t);
   if (t->stream) (void)hipStreamDestroy(t->stream);
   delete t;
   return STM_OK;
@@ -673,7 +741,7 @@ int stm_get_stats(stm_tree* t, stm_stats* out) {
   TreeHeader h;
   rc = read_header(t, &h);
   if (rc) return rc;
-  unsigned long long c[ST_NUM];
+  unsigned long long c[STATS_WORDS];
   HIP_TRY(hipMemcpy(c, t->view.stats, sizeof(c), hipMemcpyDeviceToHost));
   *out = t->st;
   out->n_keys = h.n_keys;
@@ -687,9 +755,71 @@ int stm_get_stats(stm_tree* t, stm_stats* out) {
   out->inner_splits = c[ST_INNER_SPLITS];
   out->root_splits = c[ST_ROOT_SPLITS];
   out->deferred = c[ST_DEFERRED];
-  out->route_hits = c[ST_ROUTE_HITS];
-  out->route_misses = c[ST_ROUTE_MISSES];
+  out->route_hits = 0;
+  out->route_misses = 0;
+  for (int i = 0; i < ROUTE_STAT_SLOTS; ++i) {
+    out->route_hits += c[ST_NUM + i];
+    out->route_misses += c[ST_NUM + ROUTE_STAT_SLOTS + i];
+  }
   return STM_OK;
+}
+
+int stm_retrain(stm_tree* t, uint32_t train_steps) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  int rc = set_device(t);
+  if (rc) return rc;
+  rc = train_router(t, train_steps);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  phase_collect(t);
+  return STM_OK;
+}
+
+int stm_route_info(stm_tree* t, uint64_t* n_leaves, uint64_t* window_hits, uint64_t* window_misses) {
+  if (!t || !n_leaves || !window_hits || !window_misses) return fail(STM_ERR_INVALID, "null argument");
+  if (!t->route_valid) return fail(STM_ERR_STATE, "no router has been trained (stm_retrain)");
+  int rc = set_device(t);
+  if (rc) return rc;
+  uint64_t* d = nullptr;
+  HIP_TRY(dalloc(&d, 2));
+  uint64_t hm[2] = {0, 0};
+  hipError_t e = hipMemsetAsync(d, 0, 16, t->stream);
+  if (e == hipSuccess) e = route_evaluate(t->route, d, (uint32_t)t->capacity, t->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(hm, d, 16, hipMemcpyDeviceToHost, t->stream);
+  RouteHeader rh{};
+  if (e == hipSuccess) e = hipMemcpyAsync(&rh, t->route.hdr, sizeof(rh), hipMemcpyDeviceToHost, t->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(STM_ERR_DEVICE, "route info: %s", hipGetErrorString(e));
+  *n_leaves = rh.L;
+  *window_hits = hm[0];
+  *window_misses = hm[1];
+  return STM_OK;
+}
+
+int stm_mlp_gradients(const float* x, const float* tg, size_t n, const float* params49,
+                      float* grad49, float* loss) {
+  if (!params49 || !grad49 || !loss || (n && (!x || !tg))) return fail(STM_ERR_INVALID, "null argument");
+  if (n > (1u << 24)) return fail(STM_ERR_INVALID, "at most 2^24 pairs");
+  float *dx = nullptr, *dt = nullptr, *dp = nullptr, *dg = nullptr, *dl = nullptr;
+  int rc = STM_OK;
+  if (dalloc(&dx, n) || dalloc(&dt, n) || dalloc(&dp, MLP_PARAMS) || dalloc(&dg, MLP_PARAMS) ||
+      dalloc(&dl, 1)) {
+    rc = fail(STM_ERR_NOMEM, "gradient buffers");
+  } else {
+    hipError_t e = hipSuccess;
+    if (n) e = hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) e = hipMemcpy(dt, tg, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dp, params49, MLP_PARAMS * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mlp_grad(dx, dt, (uint32_t)n, dp, dg, dl, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(grad49, dg, MLP_PARAMS * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(loss, dl, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(STM_ERR_DEVICE, "mlp gradients: %s", hipGetErrorString(e));
+  }
+  void* ps[] = {dx, dt, dp, dg, dl};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  return rc;
 }
 
 int stm_reset_stats(stm_tree* t) {
@@ -697,7 +827,7 @@ int stm_reset_stats(stm_tree* t) {
   int rc = set_device(t);
   if (rc) return rc;
   t->st = stm_stats{};
-  HIP_TRY(hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * ST_NUM, t->stream));
+  HIP_TRY(hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * STATS_WORDS, t->stream));
   HIP_TRY(hipStreamSynchronize(t->stream));
   return STM_OK;
 }

@@ -69,6 +69,18 @@ enum StatSlot : int {
   ST_ROUTE_MISSES,
   ST_NUM
 };
+// Route hit / miss counters are spread over this many slots each, after the ST_NUM slots.
+constexpr int ROUTE_STAT_SLOTS = 256;
+constexpr int STATS_WORDS = ST_NUM + 2 * ROUTE_STAT_SLOTS;
+
+// Learned-routing directory entry r: the low fence and node id of the r-th leaf in key order
+// when the directory was built, and whether that leaf has split since (its live high fence
+// must then be read).  16 bytes, so one load brings all three.
+struct alignas(16) DirEntry {
+  uint64_t fence;
+  uint32_t lid;
+  uint32_t dirty;
+};
 
 struct TreeView {
   uint64_t* keys;
@@ -78,6 +90,10 @@ struct TreeView {
   unsigned long long* stats;
   uint32_t capacity;
   uint32_t fanout;
+  // Learned-routing directory hooks (null without a router): a committed leaf split marks
+  // the leaf's directory entry dirty so lookups re-read its live high fence.
+  const uint32_t* rank_of;
+  DirEntry* dir;
 };
 
 // ---------------------------------------------------------------- memory-model helpers

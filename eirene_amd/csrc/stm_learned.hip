@@ -1,0 +1,347 @@
+// stm_learned.hip — leaf directory and GPU training of the learned leaf router.
+//
+// No reference counterpart: the reference routes every op from the root (v38) or through a
+// rebuilt static index (v35, src/gpu-stm-tree-v35/bpt-gpustm-kernel.cu:17-231).  Here the
+// tree emits (key, leaf rank) pairs — the low fence of every leaf in key order — and the
+// router is trained on them on the device:
+//   k_route_directory     leaves sorted by low fence -> fences[r], lids[r], L
+//   k_route_root_*        least-squares line rank ~ a*key + b (fp64, deterministic reduce)
+//   k_route_segments      segment s = ranks whose root prediction falls in bucket s
+//   k_route_train         one wavefront per segment: full-batch Adam on the MSE of a
+//                         1 -> 16 -> 1 ReLU MLP (hand-written forward and backward)
+// k_mlp_grad exposes one forward/backward pass for the gradient check against the CPU.
+#include <cmath>
+
+#include "stm_common.hpp"
+#include "stm_internal.hpp"
+#include "stm_learned.hpp"
+#include "stm_tree.hpp"
+
+namespace stm {
+
+constexpr int RT_BLOCK = 256;
+
+// Sort keys of the directory build: a leaf's low fence, +inf for every other node, so the
+// leaves come first in key order after the sort (no compaction and no shared counter).
+__global__ void k_route_lows(TreeView t, uint32_t n_nodes, uint64_t* __restrict__ lows,
+                             uint32_t* __restrict__ rank_of) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < t.capacity) rank_of[i] = NODE_NONE;
+  if (i < n_nodes) {
+    const NodeMeta m = t.meta[i];
+    lows[i] = meta_level(m.info) == 0 ? m.low : KEY_EMPTY;
+  }
+}
+
+// Directory from the sorted (low fence, node id) pairs.  The last leaf writes L.
+__global__ void k_route_directory(const uint64_t* __restrict__ sorted_lows,
+                                  const uint32_t* __restrict__ order, uint32_t n,
+                                  RouteHeader* __restrict__ rh, uint64_t* __restrict__ fences,
+                                  uint32_t* __restrict__ lids, DirEntry* __restrict__ dir,
+                                  uint32_t* __restrict__ rank_of) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint64_t low = sorted_lows[r];
+  if (low == KEY_EMPTY) return;
+  fences[r] = low;
+  lids[r] = order[r];
+  dir[r] = DirEntry{low, order[r], 0u};
+  rank_of[order[r]] = r;
+  if (r + 1 == n || sorted_lows[r + 1] == KEY_EMPTY) {
+    rh->L = r + 1;
+    rh->S = ROUTE_SEGMENTS;
+    fences[r + 1] = KEY_EMPTY;
+    dir[r + 1] = DirEntry{KEY_EMPTY, NODE_NONE, 1u};
+  }
+}
+
+// Per-block sums of (x, y, xx, xy) with x = fence / 2^64 and y = rank.
+__global__ __launch_bounds__(RT_BLOCK) void k_route_root_partial(const uint64_t* __restrict__ fences,
+                                                                 const RouteHeader* __restrict__ rh,
+                                                                 double* __restrict__ partial) {
+  __shared__ double s[4][RT_BLOCK];
+  const uint32_t L = rh->L;
+  double sx = 0, sy = 0, sxx = 0, sxy = 0;
+  for (uint32_t r = blockIdx.x * RT_BLOCK + threadIdx.x; r < L; r += gridDim.x * RT_BLOCK) {
+    const double x = (double)fences[r] * 5.421010862427522e-20;  // 2^-64
+    const double y = (double)r;
+    sx += x;
+    sy += y;
+    sxx += x * x;
+    sxy += x * y;
+  }
+  s[0][threadIdx.x] = sx;
+  s[1][threadIdx.x] = sy;
+  s[2][threadIdx.x] = sxx;
+  s[3][threadIdx.x] = sxy;
+  __syncthreads();
+  for (int off = RT_BLOCK / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) partial[blockIdx.x * 4 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void k_route_root_final(const double* __restrict__ partial, uint32_t nblocks,
+                                   const uint64_t* __restrict__ fences, RouteHeader* __restrict__ rh) {
+  if (threadIdx.x != 0) return;
+  double sx = 0, sy = 0, sxx = 0, sxy = 0;
+  for (uint32_t b = 0; b < nblocks; ++b) {
+    sx += partial[4 * b];
+    sy += partial[4 * b + 1];
+    sxx += partial[4 * b + 2];
+    sxy += partial[4 * b + 3];
+  }
+  const double n = (double)rh->L;
+  double a = 0, b = 0;
+  const double var = sxx * n - sx * sx;
+  if (n >= 2 && var > 0) a = (sxy * n - sx * sy) / var;  // least-squares slope (diagnostic)
+  rh->ls_slope = a * 5.421010862427522e-20;
+  // The root is the line through the first and last fence: the least-squares line of a
+  // skewed key set leaves a large share of the keys left of its zero crossing, and clamping
+  // them into segment 0 would make that one segment hold most of the leaves.
+  const uint32_t L = rh->L;
+  a = 0;
+  b = 0;
+  if (L >= 2) {
+    const double k0 = (double)fences[0], k1 = (double)fences[L - 1];
+    if (k1 > k0) {
+      a = (double)(L - 1) / (k1 - k0);
+      b = -a * k0;
+    }
+  }
+  rh->a = a;  // rank per key unit
+  rh->b = b;
+}
+
+// First rank whose root bucket is >= s (the root is monotone in the key).
+STM_DEV uint32_t first_rank_of_segment(const RouteView& v, uint32_t s) {
+  uint32_t lo = 0, hi = v.L;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (root_segment(v, v.fences[mid]) >= s) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+STM_DEV void mlp_init(float* p) {
+  // Identity-like start: hinges at j/16, the first unit passes x through.
+  for (int j = 0; j < MLP_HIDDEN; ++j) {
+    p[j] = 1.0f;
+    p[MLP_HIDDEN + j] = -(float)j / (float)MLP_HIDDEN;
+    p[2 * MLP_HIDDEN + j] = j == 0 ? 1.0f : 0.0f;
+  }
+  p[3 * MLP_HIDDEN] = 0.0f;
+}
+
+__global__ void k_route_segments(RouteHeader* __restrict__ rh, const uint64_t* __restrict__ fences,
+                                 const uint32_t* __restrict__ lids, SegModel* __restrict__ segs,
+                                 uint32_t init_params) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const RouteView v = route_view(rh, fences, lids, segs);
+  if (s >= v.S) return;
+  const uint32_t r0 = first_rank_of_segment(v, s);
+  const uint32_t r1 = (s + 1 < v.S) ? first_rank_of_segment(v, s + 1) : v.L;
+  SegModel& m = segs[s];
+  m.r0 = r0;
+  m.cnt = r1 > r0 ? r1 - r0 : 0;
+  m.n = (float)m.cnt;
+  if (m.cnt > 0) {
+    m.klo = fences[r0];
+    const uint64_t khi = fences[r1 - 1];
+    m.inv_w = khi > m.klo ? (float)(1.0 / ((double)khi - (double)m.klo)) : 0.0f;
+  } else {
+    m.klo = r0 < v.L ? fences[r0] : 0;
+    m.inv_w = 0.0f;
+  }
+  if (init_params) mlp_init(m.p);
+}
+
+// Normalized training pair i of a segment: x in [0, 1], target (rank - r0) / cnt.
+STM_DEV float seg_x(const SegModel& m, uint64_t key) {
+  return (float)(((double)key - (double)m.klo) * (double)m.inv_w);
+}
+
+// Adds the gradient of 0.5 * (y - t)^2 * scale at one pair into g (per lane).
+STM_DEV float mlp_accumulate(const float* p, float x, float t, float scale, float* g) {
+  float z[MLP_HIDDEN];
+  float y = p[3 * MLP_HIDDEN];
+#pragma unroll
+  for (int j = 0; j < MLP_HIDDEN; ++j) {
+    z[j] = p[j] * x + p[MLP_HIDDEN + j];
+    y += p[2 * MLP_HIDDEN + j] * fmaxf(z[j], 0.0f);
+  }
+  const float e = y - t;
+  const float dy = e * scale;
+  g[3 * MLP_HIDDEN] += dy;
+#pragma unroll
+  for (int j = 0; j < MLP_HIDDEN; ++j) {
+    const float h = fmaxf(z[j], 0.0f);
+    g[2 * MLP_HIDDEN + j] += dy * h;
+    const float dz = z[j] > 0.0f ? dy * p[2 * MLP_HIDDEN + j] : 0.0f;
+    g[j] += dz * x;
+    g[MLP_HIDDEN + j] += dz;
+  }
+  return 0.5f * e * e * scale;
+}
+
+// Sum over the wavefront of g[0..MLP_PARAMS) into s_out (LDS, MLP_PARAMS floats).  The
+// LDS hand-off between lanes of one wavefront only needs the writes to have completed
+// (wave_order waits for them); no workgroup barrier is involved, so wavefronts of a block
+// may leave at different times.
+STM_DEV void wave_reduce_grad(const float* g, float* s_part /*[64][MLP_PARAMS]*/, float* s_out) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int q = 0; q < MLP_PARAMS; ++q) s_part[lane * MLP_PARAMS + q] = g[q];
+  wave_order();
+  if (lane < MLP_PARAMS) {
+    float acc = 0.0f;
+    for (int l = 0; l < WAVE; ++l) acc += s_part[l * MLP_PARAMS + lane];
+    s_out[lane] = acc;
+  }
+  wave_order();
+}
+
+constexpr int TRAIN_WAVES = 4;
+constexpr uint32_t TRAIN_MAX_PAIRS = 2048;
+
+// One wavefront per segment: full-batch Adam for `steps` steps.
+__global__ __launch_bounds__(TRAIN_WAVES * WAVE) void k_route_train(
+    const RouteHeader* __restrict__ rh, const uint64_t* __restrict__ fences,
+    SegModel* __restrict__ segs, uint32_t steps, float lr) {
+  __shared__ float s_part[TRAIN_WAVES][WAVE * MLP_PARAMS];
+  __shared__ float s_grad[TRAIN_WAVES][MLP_PARAMS];
+  __shared__ float s_p[TRAIN_WAVES][MLP_PARAMS];
+  const int wave = threadIdx.x / WAVE;
+  const int lane = lane_id();
+  const uint32_t s = blockIdx.x * TRAIN_WAVES + wave;
+  if (s >= rh->S) return;  // whole wavefront leaves together
+  SegModel& m = segs[s];
+  const uint32_t cnt = m.cnt;
+  if (cnt < 2 || m.inv_w == 0.0f) return;
+  if (lane < MLP_PARAMS) s_p[wave][lane] = m.p[lane];
+  wave_order();
+  float am = 0.0f, av = 0.0f;  // Adam moments of parameter `lane`
+  const float inv_n = 1.0f / (float)cnt;
+  const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  float b1t = 1.0f, b2t = 1.0f;
+  for (uint32_t it = 0; it < steps; ++it) {
+    float p[MLP_PARAMS];
+#pragma unroll
+    for (int q = 0; q < MLP_PARAMS; ++q) p[q] = s_p[wave][q];
+    float g[MLP_PARAMS];
+#pragma unroll
+    for (int q = 0; q < MLP_PARAMS; ++q) g[q] = 0.0f;
+    // Large segments train on a strided subsample of at most TRAIN_MAX_PAIRS pairs per
+    // step, shifted every step, so one segment cannot stall the launch.
+    const uint32_t used = min(cnt, TRAIN_MAX_PAIRS);
+    const uint32_t span = cnt / used;
+    const uint32_t shift = span > 1 ? (it * 2654435761u) % span : 0;
+    const float scale = 1.0f / (float)used;
+    for (uint32_t j = lane; j < used; j += WAVE) {
+      const uint32_t i = span > 1 ? j * span + shift : j;
+      const float x = seg_x(m, fences[m.r0 + i]);
+      const float tg = (float)i * inv_n;
+      mlp_accumulate(p, x, tg, scale, g);
+    }
+    wave_reduce_grad(g, s_part[wave], s_grad[wave]);
+    if (lane < MLP_PARAMS) {
+      const float gr = s_grad[wave][lane];
+      b1t *= beta1;
+      b2t *= beta2;
+      am = beta1 * am + (1.0f - beta1) * gr;
+      av = beta2 * av + (1.0f - beta2) * gr * gr;
+      const float step_lr = lr * (1.0f - 0.9f * (float)it / (float)steps);
+      s_p[wave][lane] = s_p[wave][lane] - step_lr * (am / (1.0f - b1t)) / (sqrtf(av / (1.0f - b2t)) + eps);
+    }
+    wave_order();
+  }
+  if (lane < MLP_PARAMS) m.p[lane] = s_p[wave][lane];
+}
+
+// Loss and gradient of the mean of 0.5 * (mlp(x_i) - t_i)^2 over n pairs (one wavefront).
+__global__ __launch_bounds__(WAVE) void k_mlp_grad(const float* __restrict__ x, const float* __restrict__ t,
+                                                   uint32_t n, const float* __restrict__ params,
+                                                   float* __restrict__ grad, float* __restrict__ loss) {
+  __shared__ float s_part[WAVE * MLP_PARAMS];
+  __shared__ float s_grad[MLP_PARAMS];
+  float p[MLP_PARAMS];
+  for (int q = 0; q < MLP_PARAMS; ++q) p[q] = params[q];
+  float g[MLP_PARAMS];
+  for (int q = 0; q < MLP_PARAMS; ++q) g[q] = 0.0f;
+  float l = 0.0f;
+  const float inv_n = n ? 1.0f / (float)n : 0.0f;
+  for (uint32_t i = lane_id(); i < n; i += WAVE) l += mlp_accumulate(p, x[i], t[i], inv_n, g);
+  wave_reduce_grad(g, s_part, s_grad);
+  if (lane_id() < MLP_PARAMS) grad[lane_id()] = s_grad[lane_id()];
+  for (int off = 32; off >= 1; off >>= 1) l += __shfl_xor(l, off, WAVE);
+  if (lane_id() == 0) *loss = l;
+}
+
+// Window hit rate of the model on its own training pairs: rank r is a hit when the
+// prediction for fences[r] lies within ROUTE_WINDOW of r (the lookup's window then holds r).
+__global__ void k_route_eval(const RouteHeader* __restrict__ rh, const uint64_t* __restrict__ fences,
+                             const uint32_t* __restrict__ lids, const SegModel* __restrict__ segs,
+                             uint32_t n_max, unsigned long long* __restrict__ hm) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const RouteView v = route_view(rh, fences, lids, segs);
+  bool valid = r < v.L && r < n_max;
+  bool hit = false;
+  if (valid) {
+    const uint64_t key = fences[r];
+    const SegModel& m = segs[root_segment(v, key)];
+    float rank = (float)m.r0 + mlp_forward(m.p, seg_x(m, key)) * m.n;
+    rank = fminf(fmaxf(rank, 0.0f), (float)(v.L - 1));
+    const int64_t p = (int64_t)(rank + 0.5f);
+    hit = p >= (int64_t)r - ROUTE_WINDOW && p <= (int64_t)r + ROUTE_WINDOW;
+  }
+  const uint64_t h = ballot(valid && hit), m = ballot(valid && !hit);
+  if (lane_id() == 0) {
+    if (h) atomicAdd(hm, (unsigned long long)__popcll(h));
+    if (m) atomicAdd(hm + 1, (unsigned long long)__popcll(m));
+  }
+}
+
+hipError_t route_evaluate(const RouteState& rs, uint64_t* d_hm, uint32_t n_max, hipStream_t s) {
+  if (n_max == 0) return hipSuccess;
+  k_route_eval<<<(n_max + 255) / 256, 256, 0, s>>>(rs.hdr, rs.fences, rs.lids, rs.segs, n_max,
+                                                   (unsigned long long*)d_hm);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ host side
+size_t route_partial_blocks() { return 256; }
+
+hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint32_t steps,
+                       float lr, bool train, hipStream_t s) {
+  if (n_nodes == 0 || n_nodes > t.capacity) return hipErrorInvalidValue;
+  k_route_lows<<<(t.capacity + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.lows, rs.rank_of);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = radix_sort_pairs(rs.lows, rs.lows_tmp, rs.sorted_lows, rs.order_tmp, rs.order, n_nodes, 0, 64,
+                       rs.hist, s);
+  if (e != hipSuccess) return e;
+  k_route_directory<<<(n_nodes + 255) / 256, 256, 0, s>>>(rs.sorted_lows, rs.order, n_nodes, rs.hdr,
+                                                          rs.fences, rs.lids, rs.dir, rs.rank_of);
+  k_route_root_partial<<<(uint32_t)route_partial_blocks(), RT_BLOCK, 0, s>>>(rs.fences, rs.hdr, rs.partial);
+  k_route_root_final<<<1, 64, 0, s>>>(rs.partial, (uint32_t)route_partial_blocks(), rs.fences, rs.hdr);
+  k_route_segments<<<(ROUTE_SEGMENTS + 255) / 256, 256, 0, s>>>(rs.hdr, rs.fences, rs.lids, rs.segs, 1u);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (train && steps > 0) {
+    k_route_train<<<(ROUTE_SEGMENTS + TRAIN_WAVES - 1) / TRAIN_WAVES, TRAIN_WAVES * WAVE, 0, s>>>(
+        rs.hdr, rs.fences, rs.segs, steps, lr);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+hipError_t mlp_grad(const float* d_x, const float* d_t, uint32_t n, const float* d_params,
+                    float* d_grad, float* d_loss, hipStream_t s) {
+  k_mlp_grad<<<1, WAVE, 0, s>>>(d_x, d_t, n, d_params, d_grad, d_loss);
+  return hipGetLastError();
+}
+
+}  // namespace stm

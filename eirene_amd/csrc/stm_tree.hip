@@ -15,6 +15,7 @@
 //     (reference: src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:162-181).
 #include "stm_common.hpp"
 #include "stm_internal.hpp"
+#include "stm_learned.hpp"
 #include "stm_tree.hpp"
 
 namespace stm {
@@ -117,7 +118,92 @@ STM_DEV void load_slots(const uint64_t* __restrict__ base, uint32_t node, int g,
   for (int e = 0; e < K; ++e) x[e] = p[e];
 }
 
+template <int G>
+STM_DEV float group_sumf(float v) {
+#pragma unroll
+  for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
+// Per lane-group state of the learned path: the directory entry of the current leaf and a
+// prefetched copy of its successor entry (keys arrive sorted, so the next leaf is usually
+// the next entry and needs no model evaluation).
+struct RouteCursor {
+  uint32_t r = NODE_NONE;   // directory entry of the current leaf
+  uint64_t dir_hi = 0;      // fences[r + 1]
+  uint64_t succ_hi = 0;     // fences[r + 2]; 0 when entry r + 1 does not exist
+  uint32_t succ_lid = NODE_NONE;
+  uint32_t succ_dirty = 1;
+};
+
+// Learned routing of `key` to a directory entry on a tree nobody writes.  On a hit sets the
+// entry's leaf id and dirty flag (the leaf split since the directory was built) and returns
+// true; on a miss returns false and the caller descends from the root.  The successor entry
+// is tried first, then the model's window [p-2, p+2]: one load per lane brings the fence,
+// leaf id and dirty flag of one window entry.
 template <int F, int G>
+STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, RouteCursor& c,
+                           uint32_t& lid, bool& dirty) {
+  const uint32_t L = rv.L;
+  if (L == 0) return false;
+  uint32_t r = NODE_NONE;
+  if (c.r != NODE_NONE && key >= c.dir_hi && key < c.succ_hi) {
+    r = c.r + 1;
+    lid = c.succ_lid;
+    dirty = c.succ_dirty != 0;
+    c.dir_hi = c.succ_hi;
+  } else {
+    const SegModel* m = rv.segs + root_segment(rv, key);
+    const float x = (float)(((double)key - (double)m->klo) * (double)m->inv_w);
+    float acc = 0.0f;
+    for (int j = g; j < MLP_HIDDEN; j += G)
+      acc += m->p[2 * MLP_HIDDEN + j] * fmaxf(m->p[j] * x + m->p[MLP_HIDDEN + j], 0.0f);
+    const float y = group_sumf<G>(acc) + m->p[3 * MLP_HIDDEN];
+    float rank = (float)m->r0 + y * m->n;
+    rank = fminf(fmaxf(rank, 0.0f), (float)(L - 1));  // also maps NaN to 0
+    const int64_t p = (int64_t)(rank + 0.5f);
+    const int64_t idx = p - ROUTE_WINDOW + g;
+    bool le = false;
+    uint64_t f = KEY_EMPTY;
+    uint32_t wl = NODE_NONE, wd = 1;
+    if (g < 2 * ROUTE_WINDOW + 2) {
+      if (idx < 0) {
+        le = true;
+      } else if (idx < (int64_t)L) {
+        const DirEntry e = rv.dir[idx];
+        f = e.fence;
+        wl = e.lid;
+        wd = e.dirty;
+        le = f <= key;
+      }
+    }
+    const int cnt = (int)group_sum<G>(le ? 1u : 0u);
+    // lanes 0..cnt-1 hold fences <= key: entry r = p-2+cnt-1 ends at lane cnt's fence
+    const uint32_t lid_w = (uint32_t)__shfl((int)wl, gbase + max(cnt - 1, 0), WAVE);
+    const uint32_t dirty_w = (uint32_t)__shfl((int)wd, gbase + max(cnt - 1, 0), WAVE);
+    const uint64_t hi_w = shfl64(f, gbase + min(cnt, 2 * ROUTE_WINDOW + 1));
+    if (cnt < 1 || cnt > 2 * ROUTE_WINDOW + 1) {
+      c.r = NODE_NONE;
+      return false;
+    }
+    r = (uint32_t)(p - ROUTE_WINDOW + cnt - 1);
+    lid = lid_w;
+    dirty = dirty_w != 0;
+    c.dir_hi = hi_w;
+  }
+  c.r = r;
+  if (r + 1 < L) {  // prefetch the successor entry; its loads overlap the leaf's
+    const DirEntry e = rv.dir[r + 1];
+    c.succ_hi = rv.dir[r + 2].fence;
+    c.succ_lid = e.lid;
+    c.succ_dirty = e.dirty;
+  } else {
+    c.succ_hi = 0;
+  }
+  return true;
+}
+
+template <int F, int G, bool LEARNED>
 __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __restrict__ ukey,
                                                 const uint8_t* __restrict__ uop,
                                                 const uint32_t* __restrict__ gstart,
@@ -125,7 +211,11 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
                                                 const uint32_t* __restrict__ sidx,
                                                 const BatchCounters* __restrict__ cnt,
                                                 uint32_t chunk, uint64_t* __restrict__ out_vals,
-                                                uint8_t* __restrict__ out_found) {
+                                                uint8_t* __restrict__ out_found,
+                                                const RouteHeader* __restrict__ rh,
+                                                const uint64_t* __restrict__ fences,
+                                                const uint32_t* __restrict__ lids,
+                                                const SegModel* __restrict__ segs) {
   constexpr int K = F / G;
   const int lane = lane_id();
   const int g = lane & (G - 1);
@@ -145,51 +235,119 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   uint32_t par = NODE_NONE;
   uint64_t par_hi = 0;
   uint64_t pk[K], pv[K];
+  RouteView rv{};
+  RouteCursor cur;
+  uint32_t hits = 0, misses = 0;
+  if constexpr (LEARNED) rv = route_view(rh, fences, lids, segs, t.dir);
 
-  for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
-    if (uop[i] != OP_GET) continue;
-    const uint64_t key = ukey[i];
-    if (leaf == NODE_NONE || key >= leaf_hi) {
-      if (height <= 1) {
-        leaf = root;
-        leaf_hi = KEY_EMPTY;
-      } else {
-        if (par == NODE_NONE || key >= par_hi) {
-          uint32_t node = root;
-          uint64_t hi = KEY_EMPTY;
-          for (uint32_t lvl = height - 1; lvl > 1; --lvl) {
-            uint64_t nk[K], nv[K];
-            load_slots<F, G>(t.keys, node, g, nk);
-            load_slots<F, G>(t.vals, node, g, nv);
-            const int s = max(group_count_le<F, G>(nk, key) - 1, 0);
-            const uint64_t child = group_at<F, G>(nv, s, gbase);
-            const uint64_t nx = (s + 1 < F) ? group_at<F, G>(nk, s + 1, gbase) : KEY_EMPTY;
-            hi = min(hi, nx);
-            node = (uint32_t)min<uint64_t>(child, cap - 1);
+  for (uint32_t base = (uint32_t)i0; base < i1; base += G) {
+    // Lane g owns op base + g: its metadata comes in one coalesced round trip, and the
+    // sorted position of its first waiting op in a second one that overlaps the walk.
+    const uint32_t j = base + g;
+    const bool valid = j < i1;
+    const uint8_t op_l = valid ? uop[j] : UOP_SKIP;
+    const uint64_t key_l = valid ? ukey[j] : 0;
+    const uint32_t gs_l = valid ? gstart[j] : 0;
+    const uint32_t np_l = valid ? npend[j] : 0;
+    const bool mine = op_l == OP_GET;
+    const uint32_t pos_l = (mine && np_l) ? sidx[gs_l] : 0;
+    uint64_t vidx_l = 0;  // value slot of op base + g
+    bool found_l = false;
+    const uint32_t n_sub = min(i1 - base, (uint32_t)G);
+    for (uint32_t jj = 0; jj < n_sub; ++jj) {
+      if (__shfl((int)mine, gbase + (int)jj, WAVE) == 0) continue;
+      const uint64_t key = shfl64(key_l, gbase + (int)jj);
+      if (leaf == NODE_NONE || key >= leaf_hi) {
+        bool routed = false;
+        if constexpr (LEARNED) {
+          uint32_t lid = leaf;
+          bool dirty = true;
+          // Still inside the current directory entry: the leaf split, walk right from it.
+          const bool same_entry = cur.r != NODE_NONE && leaf != NODE_NONE && key < cur.dir_hi;
+          routed = same_entry || learned_route<F, G>(rv, key, g, gbase, cur, lid, dirty);
+          if (routed) {
+            leaf = min(lid, cap - 1);
+            uint64_t mh = KEY_EMPTY;
+            if (dirty) mh = t.meta[leaf].high;
+            load_slots<F, G>(t.keys, leaf, g, lk);
+            if (dirty && key >= mh) {  // the key lives in a right sibling made by a split
+              routed = false;
+              for (int step = 0; step < 64; ++step) {
+                leaf = (uint32_t)min<uint64_t>(t.meta[leaf].next, cap - 1);
+                mh = t.meta[leaf].high;
+                if (key < mh) {
+                  routed = true;
+                  break;
+                }
+              }
+              if (routed) load_slots<F, G>(t.keys, leaf, g, lk);
+              else cur.r = NODE_NONE;
+            }
+            leaf_hi = min(cur.dir_hi, mh);
           }
-          load_slots<F, G>(t.keys, node, g, pk);
-          load_slots<F, G>(t.vals, node, g, pv);
-          par = node;
-          par_hi = hi;
+          hits += routed ? 1 : 0;
+          misses += routed ? 0 : 1;
         }
-        const int s = max(group_count_le<F, G>(pk, key) - 1, 0);
-        leaf = (uint32_t)min<uint64_t>(group_at<F, G>(pv, s, gbase), cap - 1);
-        const uint64_t nx = (s + 1 < F) ? group_at<F, G>(pk, s + 1, gbase) : KEY_EMPTY;
-        leaf_hi = min(par_hi, nx);
+        if (!routed) {
+          if (height <= 1) {
+            leaf = root;
+            leaf_hi = KEY_EMPTY;
+          } else {
+            // The learned variant keeps no cached parent: its misses are rare and the
+            // registers buy occupancy.
+            if (LEARNED || par == NODE_NONE || key >= par_hi) {
+              uint32_t node = root;
+              uint64_t hi = KEY_EMPTY;
+              for (uint32_t lvl = height - 1; lvl > 1; --lvl) {
+                uint64_t nk[K], nv[K];
+                load_slots<F, G>(t.keys, node, g, nk);
+                load_slots<F, G>(t.vals, node, g, nv);
+                const int s = max(group_count_le<F, G>(nk, key) - 1, 0);
+                const uint64_t child = group_at<F, G>(nv, s, gbase);
+                const uint64_t nx = (s + 1 < F) ? group_at<F, G>(nk, s + 1, gbase) : KEY_EMPTY;
+                hi = min(hi, nx);
+                node = (uint32_t)min<uint64_t>(child, cap - 1);
+              }
+              load_slots<F, G>(t.keys, node, g, pk);
+              load_slots<F, G>(t.vals, node, g, pv);
+              par = node;
+              par_hi = hi;
+            }
+            const int s = max(group_count_le<F, G>(pk, key) - 1, 0);
+            leaf = (uint32_t)min<uint64_t>(group_at<F, G>(pv, s, gbase), cap - 1);
+            const uint64_t nx = (s + 1 < F) ? group_at<F, G>(pk, s + 1, gbase) : KEY_EMPTY;
+            leaf_hi = min(par_hi, nx);
+          }
+          load_slots<F, G>(t.keys, leaf, g, lk);
+        }
       }
-      load_slots<F, G>(t.keys, leaf, g, lk);
+      const int s = group_count_le<F, G>(lk, key) - 1;
+      const uint64_t at = group_at<F, G>(lk, max(s, 0), gbase);
+      if (g == (int)jj) {
+        found_l = s >= 0 && at == key;
+        vidx_l = (uint64_t)leaf * F + (uint32_t)max(s, 0);
+      }
     }
-    const int s = group_count_le<F, G>(lk, key) - 1;
-    const uint64_t at = group_at<F, G>(lk, max(s, 0), gbase);
-    const bool found = s >= 0 && at == key;
-    uint64_t val = 0;
-    if (found) val = t.vals[(size_t)leaf * F + s];
-    const uint32_t gs = gstart[i];
-    const uint32_t np = npend[i];
-    for (uint32_t p = g; p < np; p += G) {
-      const uint32_t pos = sidx[gs + p];
-      out_vals[pos] = val;
-      out_found[pos] = found ? 1 : 0;
+    // All values of the sub-chunk in one parallel round trip, then the answers.
+    if (mine) {
+      const uint64_t val = found_l ? t.vals[vidx_l] : 0;
+      if (np_l) {
+        out_vals[pos_l] = val;
+        out_found[pos_l] = found_l ? 1 : 0;
+      }
+      for (uint32_t p = 1; p < np_l; ++p) {
+        const uint32_t pos = sidx[gs_l + p];
+        out_vals[pos] = val;
+        out_found[pos] = found_l ? 1 : 0;
+      }
+    }
+  }
+  if constexpr (LEARNED) {
+    // Spread over ROUTE_STAT_SLOTS counters so no single address serializes the kernel.
+    if (g == 0) {
+      const uint32_t slot = (uint32_t)(grp % ROUTE_STAT_SLOTS);
+      if (hits) atomicAdd(t.stats + ST_NUM + slot, (unsigned long long)hits);
+      if (misses) atomicAdd(t.stats + ST_NUM + ROUTE_STAT_SLOTS + slot, (unsigned long long)misses);
     }
   }
 }
@@ -599,6 +757,10 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
     if (lane_id() == 0) st_relaxed(&t.meta[qid].lock, 2ull);
     c.root_splits++;
   }
+  if (t.rank_of && lane_id() == 0) {
+    const uint32_t r = t.rank_of[n.id];
+    if (r != NODE_NONE) t.dir[r].dirty = 1;
+  }
   c.leaf_splits++;
   c.commits++;
   c.new_keys++;
@@ -962,8 +1124,14 @@ static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipSt
   const uint64_t threads = groups * G;
   const uint32_t blocks = (uint32_t)((threads + 255) / 256);
   if (blocks == 0) return hipSuccess;
-  k_lookup<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx, a.counters,
-                                         a.chunk, a.out_vals, a.out_found);
+  if (a.route_hdr)
+    k_lookup<F, G, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
+                                                 a.counters, a.chunk, a.out_vals, a.out_found,
+                                                 a.route_hdr, a.fences, a.lids, a.segs);
+  else
+    k_lookup<F, G, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
+                                                  a.counters, a.chunk, a.out_vals, a.out_found,
+                                                  nullptr, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include "stm_common.hpp"
 #include "stm_internal.hpp"
+#include "stm_learned.hpp"
 
 namespace stm {
 
@@ -17,6 +18,11 @@ struct LookupArgs {
   uint32_t lanes;     // lanes per op: 8, 16, 32, 64
   uint64_t* out_vals;
   uint8_t* out_found;
+  // learned routing (all null for root-to-leaf descent)
+  const RouteHeader* route_hdr;
+  const uint64_t* fences;
+  const uint32_t* lids;
+  const SegModel* segs;
 };
 
 struct UpsertArgs {

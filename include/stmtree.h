@@ -64,7 +64,9 @@ typedef struct stm_config {
     uint32_t max_attempts;/* STM attempts before an op is deferred to the serial pass (default 64) */
     uint32_t route;       /* STM_ROUTE_* */
     uint32_t timing;      /* 1 = record HIP events around every phase (stm_get_stats) */
-    uint32_t reserved[5];
+    uint32_t retrain_every; /* learned routing: rebuild + retrain after this many batches (default 8) */
+    uint32_t train_steps; /* learned routing: Adam steps per segment MLP (default 300) */
+    uint32_t reserved[3];
 } stm_config;
 
 /* Per-phase timing slots of stm_stats.phase_ms / phase_launches. */
@@ -148,6 +150,20 @@ int stm_reset_stats(stm_tree* tree);
 
 /* The hipStream_t every kernel of this tree runs on. */
 int stm_get_stream(stm_tree* tree, void** stream);
+
+/* Learned routing (no reference counterpart; the reference always descends from the root).
+ * stm_retrain rebuilds the leaf directory from the current tree and trains the router on
+ * the device now (train_steps = 0 keeps the untrained, identity-like segment models).
+ * stm_route_info reports the directory size and, for every leaf low fence, whether the
+ * current model's +/-2 window contains it (hits) or not (misses). */
+int stm_retrain(stm_tree* tree, uint32_t train_steps);
+int stm_route_info(stm_tree* tree, uint64_t* n_leaves, uint64_t* window_hits, uint64_t* window_misses);
+
+/* One forward/backward pass of the router's segment MLP (1 -> 16 -> 1, ReLU, fp32) on the
+ * device, for checking the training kernels: loss = mean of 0.5 * (mlp(x_i) - t_i)^2 and its
+ * gradient with respect to the 49 parameters (w1[16], b1[16], w2[16], b2), host arrays. */
+int stm_mlp_gradients(const float* x, const float* t, size_t n, const float* params49,
+                      float* grad49, float* loss);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* stm_last_error(void);

@@ -1,0 +1,77 @@
+"""GPU tests of the learned leaf router: device gradients against the CPU restatement,
+bit-exact batch results with learned routing, and what the trained network gains over
+the untrained router (the linear root with identity segment models)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_gradients_match_cpu():
+    import eirene_amd as ea
+    from oracle import mlp
+    rng = np.random.default_rng(3)
+    for n in (1, 63, 64, 65, 1000, 4096):
+        p = rng.normal(0, 1, size=49).astype(np.float32)
+        x = rng.uniform(0, 1, size=n).astype(np.float32)
+        t = np.sort(rng.uniform(0, 1, size=n)).astype(np.float32)
+        loss, g = ea.mlp_gradients(x, t, p)
+        want_loss, want_g = mlp.loss_and_grad(p.astype(np.float64), x.astype(np.float64),
+                                              t.astype(np.float64))
+        scale = max(np.abs(want_g).max(), 1e-3)
+        # float32 accumulation in a different order: relative tolerance on the gradient scale
+        np.testing.assert_allclose(g, want_g, rtol=0, atol=2e-5 * scale + 1e-7)
+        assert abs(loss - want_loss) <= 2e-5 * max(want_loss, 1e-6) + 1e-7
+
+
+def gamma_tree(ea, w, n, seed, **kw):
+    load = w.gamma_keys(n, seed, 2.0, 2.0 ** 56)
+    vals = w.values_for(load)
+    cfg = dict(max_keys=4 * n, max_batch=1 << 18, route=ea.ROUTE_LEARNED)
+    cfg.update(kw)
+    t = ea.StmTree(ea.StmConfig(**cfg))
+    t.bulk_load(load, vals)
+    return t, load, vals
+
+
+def test_learned_routing_results_match_oracle(oracle_mod):
+    import eirene_amd as ea
+    from eirene_amd import workloads as w
+    t, load, vals = gamma_tree(ea, w, 1 << 18, 5, retrain_every=2)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, vals)
+    srt = w.sorted_copy(load)
+    for b in range(5):  # retrains after batches 2 and 4, stale directory in between
+        lookup = 0.95 if b % 2 == 0 else 0.5  # insert-heavy batches split many leaves
+        keys, bv, ops = w.mixed_batch(srt, 1 << 18, 40 + b, lookup, 0.9, w.DIST_GAMMA, 2.0, 2.0 ** 56)
+        gv, gf = t.run_batch(keys, bv, ops)
+        ev, ef = o.run_batch(keys, bv, ops)
+        np.testing.assert_array_equal(gf, ef)
+        np.testing.assert_array_equal(gv, ev)
+        assert t.check() == 0
+    st = t.stats()
+    assert st["route_hits"] > 0
+    assert st["route_hits"] > 4 * st["route_misses"], st
+    k, v = t.dump()
+    ek, evv = o.dump()
+    np.testing.assert_array_equal(k, ek)
+    np.testing.assert_array_equal(v, evv)
+    t.close()
+
+
+def test_trained_router_beats_untrained():
+    import eirene_amd as ea
+    from eirene_amd import workloads as w
+    # 2^24 keys: ~85 leaves per segment, enough that the identity-like segment models miss
+    t, load, vals = gamma_tree(ea, w, 1 << 24, 6)
+    t.retrain(0)
+    untrained = t.route_info()
+    t.retrain(300)
+    trained = t.route_info()
+    assert trained["leaves"] == untrained["leaves"] > 100000
+    rate_u = untrained["window_hits"] / trained["leaves"]
+    rate_t = trained["window_hits"] / trained["leaves"]
+    print(f"window hit rate untrained {rate_u:.4f} trained {rate_t:.4f}")
+    assert trained["window_misses"] * 2 < untrained["window_misses"]
+    assert rate_t > 0.9
+    t.close()
