@@ -11,7 +11,13 @@ Prints one JSON line.  `value` is Mops/s over the K timed batches (wall clock fr
 first launch to the last batch's completion).  Per-kernel device times come from HIP
 events the library records on its own stream around every phase (stm_config.timing).
 
+A second section ("uneven") runs the uneven-batching workload: 2^26 Normal(2^62, 2^58)
+keys and a stream of batches with sizes log-uniform in [2^10, 2^22], replayed from captured
+hipGraphs with launch parameters chosen by eirene_amd.autotune, beside the same stream
+launched batch by batch from the host with the library defaults.
+
     python bench.py [--steps K] [--warmup W] [--route descent|learned] [--no-cpu]
+                    [--uneven-batches N] [--tuning FILE]
 """
 from __future__ import annotations
 
@@ -51,11 +57,84 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU-baseline budget (batches run until it is spent)")
+    ap.add_argument("--uneven-batches", type=int, default=256,
+                    help="uneven stream: batches log-uniform in [2^10, 2^22] (0 = skip)")
+    ap.add_argument("--tuning", default=None,
+                    help="tuning record to use for the uneven stream instead of autotuning")
+    ap.add_argument("--tune-batches", type=int, default=40,
+                    help="batches of the stream the autotuner measures")
     return ap.parse_args()
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_uneven(args, ea, w):
+    """Uneven batching: 2^load_log2 Normal(2^62, 2^58) keys, a stream of batches with sizes
+    log-uniform in [2^10, 2^22] at 95/5, replayed from captured graphs with the autotuned
+    launch parameters, against the same stream launched batch by batch from the host with the
+    library defaults."""
+    from eirene_amd import autotune as at
+    n_load = 1 << args.load_log2
+    t0 = time.time()
+    load = w.normal_keys(n_load, args.seed + 3)
+    lv = w.values_for(load)
+    srt = w.sorted_copy(load)
+    keys, vals, ops, offsets = at.uneven_sample(srt, args.uneven_batches, args.seed + 3, 10, 22,
+                                                w.DIST_NORMAL, 2.0 ** 62, 2.0 ** 58)
+    n_ops = int(offsets[-1])
+    log(f"[uneven] generated {n_load} keys and {args.uneven_batches} batches ({n_ops} ops) "
+        f"in {time.time() - t0:.1f}s")
+    sample = at.Sample(load, lv, keys, vals, ops, offsets)
+    base = ea.StmConfig(max_keys=n_load + n_ops + (1 << 20), max_batch=1 << 22)
+    t0 = time.time()
+    if args.tuning:
+        rec = at.load_tuning(args.tuning)
+        tuned_from = args.tuning
+    else:
+        tb = max(1, min(args.tune_batches, args.uneven_batches))
+        tune_sample = at.Sample(load, lv, keys[: int(offsets[tb])], vals[: int(offsets[tb])],
+                                ops[: int(offsets[tb])], offsets[: tb + 1])
+        rec = at.tune(tune_sample, base, log=lambda m: log(f"[tune] {m}"))
+        tune_sample.free()
+        tuned_from = f"autotuned in this run on the first {tb} batches"
+    log(f"[uneven] tuning {rec['best']} ({time.time() - t0:.1f}s)")
+
+    def timed(cfg, graphs):
+        cfg = ea.StmConfig(**{**cfg.__dict__, "use_graphs": graphs})
+        with ea.StmTree(cfg) as t:
+            t.bulk_load_device(sample.load_k, sample.load_v, sample.n_load)
+            if graphs:
+                t.capture_graphs()
+            t.sync()
+            t0 = time.perf_counter()
+            t.run_batches_device(sample.k, sample.v, sample.o, sample.out_v, sample.out_f, offsets)
+            dt = time.perf_counter() - t0
+            st = t.stats()
+            out = (sample.out_v.download(), sample.out_f.download(), t.dump())
+        return n_ops / dt / 1e6, dt, st, out
+
+    tuned = at.apply_tuning(base, rec["best"])
+    m_tg, dt_tg, st_tg, out_tg = timed(tuned, True)
+    m_dg, dt_dg, _, _ = timed(base, True)
+    m_dh, dt_dh, _, out_dh = timed(base, False)
+    same = bool(all(np.array_equal(a, b) for a, b in zip(out_tg[:2], out_dh[:2]))
+                and all(np.array_equal(a, b) for a, b in zip(out_tg[2], out_dh[2])))
+    log(f"[uneven] graphs+tuned {m_tg:.1f}, graphs+default {m_dg:.1f}, host+default {m_dh:.1f} "
+        f"Mops/s; identical={same}")
+    sample.free()
+    return {
+        "workload": f"bulk-load 2^{args.load_log2} Normal(2^62, 2^58) keys; {args.uneven_batches} "
+                    f"batches log-uniform in [2^10, 2^22] ({n_ops} ops), 95% lookup / 5% insert",
+        "mops_graphs_tuned": round(m_tg, 2), "mops_graphs_default": round(m_dg, 2),
+        "mops_host_default": round(m_dh, 2),
+        "ms_graphs_tuned": round(dt_tg * 1e3, 2), "ms_host_default": round(dt_dh * 1e3, 2),
+        "tuned_config": rec["best"], "tuned_from": tuned_from,
+        "autotune_trials": len(rec.get("trials", [])),
+        "results_identical_tuned_vs_default": same,
+        "stm": {k: st_tg[k] for k in ("upserts", "aborts", "leaf_splits", "deferred", "height")},
+    }
 
 
 def main():
@@ -186,6 +265,11 @@ def main():
                                        if s_["phase_launches"][k]},
                 "route_hits": s_["route_hits"], "route_misses": s_["route_misses"]}
 
+    uneven = None
+    if args.uneven_batches > 0:
+        bk.free(); bv.free(); bo.free(); out_v.free(); out_f.free()
+        uneven = run_uneven(args, ea, w)
+
     out = {
         "metric": METRIC, "value": round(mops, 2), "unit": "Mops/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -206,6 +290,7 @@ def main():
         "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
                                    "root_splits", "deferred", "n_keys", "height")},
         "roofline": roofline, "cpu_baseline": cpu, "cpu_gpu_first_batch_agree": agree,
+        "uneven": uneven,
     }
     print(json.dumps(out), flush=True)
 

@@ -52,7 +52,8 @@ class _Config(C.Structure):
         ("max_batch", C.c_uint64), ("device", C.c_int32), ("lanes_per_op", C.c_uint32),
         ("ops_per_group", C.c_uint32), ("upsert_ops_per_wave", C.c_uint32),
         ("max_attempts", C.c_uint32), ("route", C.c_uint32), ("timing", C.c_uint32),
-        ("retrain_every", C.c_uint32), ("train_steps", C.c_uint32), ("reserved", C.c_uint32 * 3),
+        ("retrain_every", C.c_uint32), ("train_steps", C.c_uint32),
+        ("sort_tile_items", C.c_uint32), ("use_graphs", C.c_uint32), ("reserved", C.c_uint32 * 1),
     ]
 
 
@@ -87,6 +88,8 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_bulk_load_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
         "stm_run_batch": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
         "stm_run_batch_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
+        "stm_run_batches_device": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, _P, _P]),
+        "stm_capture_graphs": (C.c_int, [_P]),
         "stm_size": (C.c_int, [_P, _U64P]),
         "stm_dump": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "stm_check": (C.c_int, [_P, _U64P]),
@@ -193,19 +196,21 @@ class StmConfig:
     device: int = 0
     lanes_per_op: int = 16
     ops_per_group: int = 16
-    upsert_ops_per_wave: int = 1
+    upsert_ops_per_wave: int = 512
     max_attempts: int = 64
     route: int = ROUTE_DESCENT
     timing: bool = False
     retrain_every: int = 8
     train_steps: int = 300
+    sort_tile_items: int = 16
+    use_graphs: bool = True
 
     def to_c(self) -> _Config:
         c = _Config()
         _check(load_library().stm_config_init(C.byref(c)))
         for f in ("fanout", "leaf_fill", "max_keys", "max_batch", "device", "lanes_per_op",
                   "ops_per_group", "upsert_ops_per_wave", "max_attempts", "route",
-                  "retrain_every", "train_steps"):
+                  "retrain_every", "train_steps", "sort_tile_items", "use_graphs"):
             setattr(c, f, int(getattr(self, f)))
         c.timing = 1 if self.timing else 0
         return c
@@ -272,6 +277,22 @@ class StmTree:
         _check(self._lib.stm_run_batch_device(
             self._h, keys.ptr + 8 * offset, vals.ptr + 8 * offset, ops.ptr + offset, n,
             out_vals.ptr + 8 * offset, out_found.ptr + offset))
+
+    def run_batches_device(self, keys: "DeviceBuffer", vals: "DeviceBuffer", ops: "DeviceBuffer",
+                           out_vals: "DeviceBuffer", out_found: "DeviceBuffer", offsets) -> None:
+        """Stream of device-resident batches: batch b is elements [offsets[b], offsets[b+1])."""
+        off = _u64(offsets)
+        if off.size < 1:
+            raise ValueError("offsets needs n_batches + 1 entries")
+        if off.size > 1 and int(off[-1]) > keys.n:
+            raise ValueError("offsets run past the buffers")
+        _check(self._lib.stm_run_batches_device(
+            self._h, keys.ptr, vals.ptr, ops.ptr, _ptr(off), off.size - 1, out_vals.ptr,
+            out_found.ptr))
+
+    def capture_graphs(self) -> None:
+        """Captures every batch size class's graph now (otherwise done on first use)."""
+        _check(self._lib.stm_capture_graphs(self._h))
 
     def sync(self) -> None:
         _check(self._lib.stm_sync(self._h))

@@ -53,6 +53,45 @@ hipError_t dalloc(T** p, size_t count) {
   return hipMalloc((void**)p, count * sizeof(T));
 }
 
+// Totals of a batch stream, accumulated on the device so replays need no host round trip.
+struct StreamAcc {
+  unsigned long long unique;   // groups after the per-key fold
+  unsigned long long bad;      // ops rejected for the reserved key
+  unsigned long long batches;  // batches that ran
+  unsigned long long skipped;  // batches skipped after an error
+  unsigned long long pad[4];
+};
+
+constexpr uint32_t DESC_TABLE = 4096;  // descriptors uploaded per window of a stream
+constexpr int GRAPH_CLASSES = 12;      // size classes 2^10, 2^12, ..., up to max_batch
+
+// First node of every replayed graph: takes the next descriptor of the window.  Once a batch
+// has failed (reserved key, full node pool) the later batches of the stream run empty, so the
+// tree holds exactly the batches up to the failing one.
+__global__ void k_advance(const BatchDesc* __restrict__ table, uint32_t* __restrict__ cursor,
+                          BatchDesc* __restrict__ live, const StreamAcc* __restrict__ acc,
+                          const TreeHeader* __restrict__ hdr) {
+  if (threadIdx.x != 0) return;
+  const uint32_t c = cursor[0];
+  BatchDesc d = table[c];
+  if (acc->bad || hdr->error) d.n = 0;
+  *live = d;
+  cursor[0] = c + 1;
+}
+
+// Last node of every replayed graph.
+__global__ void k_epilogue(const BatchCounters* __restrict__ cnt, const BatchDesc* __restrict__ live,
+                           StreamAcc* __restrict__ acc) {
+  if (threadIdx.x != 0) return;
+  if (live->n == 0) {
+    acc->skipped += 1;
+    return;
+  }
+  acc->unique += cnt->n_unique;
+  acc->bad += cnt->n_bad;
+  acc->batches += 1;
+}
+
 struct Scratch {
   uint64_t n = 0;
   uint64_t *in_keys = nullptr, *in_vals = nullptr, *out_vals = nullptr;
@@ -65,6 +104,11 @@ struct Scratch {
   uint8_t* uop = nullptr;
   uint32_t *gstart = nullptr, *npend = nullptr, *dlist = nullptr;
   BatchCounters* counters = nullptr;
+  // batch streams replayed from captured graphs
+  BatchDesc* desc_table = nullptr;  // DESC_TABLE entries
+  BatchDesc* desc_live = nullptr;   // the batch the current replay works on
+  uint32_t* cursor = nullptr;       // next desc_table entry
+  StreamAcc* acc = nullptr;
 
   hipError_t alloc(uint64_t cap, bool staging) {
     n = cap;
@@ -81,12 +125,16 @@ struct Scratch {
     A_(ukey, cap); A_(uval, cap); A_(uop, cap); A_(gstart, cap); A_(npend, cap);
     A_(dlist, cap);
     A_(counters, 1);
+    if (staging) {
+      A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
+    }
 #undef A_
     return hipSuccess;
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
-                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, counters};
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, counters,
+                  desc_table, desc_live, cursor, acc};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = Scratch();
@@ -110,6 +158,8 @@ struct stm_tree {
   bool route_alloc = false;
   bool route_valid = false;       // directory + model describe a recent tree
   uint32_t batches_since_train = 0;
+  // one instantiated graph per batch size class (null until first needed)
+  hipGraphExec_t graphs[GRAPH_CLASSES] = {};
 };
 
 namespace {
@@ -190,6 +240,13 @@ int init_empty(stm_tree* t) {
   return STM_OK;
 }
 
+void drop_graphs(stm_tree* t) {
+  for (hipGraphExec_t& g : t->graphs) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
 int alloc_router(stm_tree* t) {
   if (t->route_alloc) return STM_OK;
   RouteState& r = t->route;
@@ -203,6 +260,7 @@ int alloc_router(stm_tree* t) {
   t->route_alloc = true;
   t->view.rank_of = r.rank_of;
   t->view.dir = r.dir;
+  drop_graphs(t);  // captured with the previous TreeView
   return STM_OK;
 }
 
@@ -230,29 +288,48 @@ int train_router(stm_tree* t, uint32_t steps) {
   phase_begin(t, STM_PHASE_TRAIN);
   HIP_TRY(route_build(t->view, t->route, nn, steps, 0.02f, steps > 0, t->stream));
   phase_end(t, STM_PHASE_TRAIN);
+  if (t->cfg.timing) {  // every retrain is counted, also between graph replays
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    phase_collect(t);
+  }
   t->route_valid = true;
   t->batches_since_train = 0;
   return STM_OK;
 }
 
-// One chunk of at most max_batch ops, all pointers on the device.
-int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
-              uint32_t n, uint64_t* out_vals, uint8_t* out_found) {
+// The launch sequence of one batch: sort -> fold -> lookups -> upserts.  With in.desc the
+// batch (its arrays and its length, at most in.n) is read on the device, which is what a
+// captured graph replays; otherwise in.n and the pointers describe it.  No host round trip.
+struct PipeIn {
+  const uint64_t* keys;
+  const uint64_t* vals;
+  const uint8_t* ops;
+  uint32_t n;
+  uint64_t* out_vals;
+  uint8_t* out_found;
+  const BatchDesc* desc;
+};
+
+hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   Scratch& s = t->scr;
   hipStream_t st = t->stream;
-  phase_begin(t, STM_PHASE_SORT);
-  HIP_TRY(radix_sort_pairs(keys, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, n, 0, 64, s.hist, st));
-  phase_end(t, STM_PHASE_SORT);
+  hipError_t e;
+  if (timed) phase_begin(t, STM_PHASE_SORT);
+  e = radix_sort_pairs_desc(in.keys, in.desc, in.n, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, 0,
+                            64, s.hist, (int)t->cfg.sort_tile_items, st);
+  if (e != hipSuccess) return e;
+  if (timed) phase_end(t, STM_PHASE_SORT);
 
-  phase_begin(t, STM_PHASE_MERGE);
+  if (timed) phase_begin(t, STM_PHASE_MERGE);
   FoldArgs fa{};
   fa.skeys = s.skeys;
   fa.sidx = s.sidx;
-  fa.vals = vals;
-  fa.ops = ops;
-  fa.n = n;
-  fa.out_vals = out_vals;
-  fa.out_found = out_found;
+  fa.vals = in.vals;
+  fa.ops = in.ops;
+  fa.n = in.n;
+  fa.desc = in.desc;
+  fa.out_vals = in.out_vals;
+  fa.out_found = in.out_found;
   fa.ukey = s.ukey;
   fa.uval = s.uval;
   fa.uop = s.uop;
@@ -260,17 +337,12 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   fa.npend = s.npend;
   fa.counters = s.counters;
   fa.tile_scratch = s.fold_tiles;
-  HIP_TRY(fold_batch(fa, st));
-  phase_end(t, STM_PHASE_MERGE);
+  if ((e = fold_batch(fa, st)) != hipSuccess) return e;
+  if (timed) phase_end(t, STM_PHASE_MERGE);
 
-  const bool learned = t->cfg.route == STM_ROUTE_LEARNED;
-  if (learned && (!t->route_valid || t->batches_since_train >= t->cfg.retrain_every)) {
-    int rc = train_router(t, t->cfg.train_steps);
-    if (rc) return rc;
-  }
-  phase_begin(t, STM_PHASE_LOOKUP);
+  if (timed) phase_begin(t, STM_PHASE_LOOKUP);
   LookupArgs la{};
-  if (learned) {
+  if (t->cfg.route == STM_ROUTE_LEARNED) {
     la.route_hdr = t->route.hdr;
     la.fences = t->route.fences;
     la.lids = t->route.lids;
@@ -282,15 +354,16 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   la.npend = s.npend;
   la.sidx = s.sidx;
   la.counters = s.counters;
-  la.n_max = n;
+  la.n_max = in.n;
   la.chunk = t->cfg.ops_per_group;
   la.lanes = t->cfg.lanes_per_op;
-  la.out_vals = out_vals;
-  la.out_found = out_found;
-  HIP_TRY(launch_lookup(t->view, la, st));
-  phase_end(t, STM_PHASE_LOOKUP);
+  la.out_vals = in.out_vals;
+  la.out_found = in.out_found;
+  la.desc = in.desc;
+  if ((e = launch_lookup(t->view, la, st)) != hipSuccess) return e;
+  if (timed) phase_end(t, STM_PHASE_LOOKUP);
 
-  phase_begin(t, STM_PHASE_UPSERT);
+  if (timed) phase_begin(t, STM_PHASE_UPSERT);
   UpsertArgs ua{};
   ua.ukey = s.ukey;
   ua.uval = s.uval;
@@ -301,13 +374,34 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   ua.counters = s.counters;
   ua.dlist = s.dlist;
   ua.max_attempts = t->cfg.max_attempts;
-  // One wavefront per 512 groups, at most 8192 wavefronts (every SIMD fully occupied).
-  ua.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 2047) / 2048, 2048));
-  ua.out_vals = out_vals;
-  ua.out_found = out_found;
-  HIP_TRY(launch_upsert(t->view, ua, st));
-  phase_end(t, STM_PHASE_UPSERT);
+  // Each wavefront owns the groups of upsert_ops_per_wave batch slots; at most 8192 wavefronts
+  // (every SIMD of the chip fully occupied).
+  const uint64_t waves = ((uint64_t)in.n + t->cfg.upsert_ops_per_wave - 1) / t->cfg.upsert_ops_per_wave;
+  ua.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 2048));
+  ua.out_vals = in.out_vals;
+  ua.out_found = in.out_found;
+  ua.desc = in.desc;
+  if ((e = launch_upsert(t->view, ua, st)) != hipSuccess) return e;
+  if (timed) phase_end(t, STM_PHASE_UPSERT);
+  return hipSuccess;
+}
 
+// Learned routing: (re)build the directory and retrain when it is missing or stale.
+int maybe_retrain(stm_tree* t) {
+  if (t->cfg.route != STM_ROUTE_LEARNED) return STM_OK;
+  if (t->route_valid && t->batches_since_train < t->cfg.retrain_every) return STM_OK;
+  return train_router(t, t->cfg.train_steps);
+}
+
+// One chunk of at most max_batch ops, all pointers on the device; waits for its completion.
+int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+              uint32_t n, uint64_t* out_vals, uint8_t* out_found) {
+  Scratch& s = t->scr;
+  hipStream_t st = t->stream;
+  int rc = maybe_retrain(t);
+  if (rc) return rc;
+  PipeIn in{keys, vals, ops, n, out_vals, out_found, nullptr};
+  HIP_TRY(enqueue_pipeline(t, in, true));
   BatchCounters bc;
   HIP_TRY(hipMemcpyAsync(&bc, s.counters, sizeof(bc), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -318,6 +412,98 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   t->batches_since_train += 1;
   if (bc.n_bad) return fail(STM_ERR_INVALID, "batch holds the reserved key 0xffffffffffffffff");
   return check_device_error(t);
+}
+
+// ------------------------------------------------------------------ graph replay of streams
+uint32_t class_nmax(const stm_tree* t, int c) {
+  const uint64_t v = 1024ull << (2 * c);
+  return (uint32_t)std::min<uint64_t>(v, t->cfg.max_batch);
+}
+
+int class_of(const stm_tree* t, uint32_t n) {
+  for (int c = 0; c < GRAPH_CLASSES; ++c)
+    if (class_nmax(t, c) >= n) return c;
+  return GRAPH_CLASSES - 1;  // max_batch <= 2^30 fits class 10
+}
+
+// Captures advance -> pipeline(desc_live, class bound) -> epilogue into an executable graph.
+int capture_class(stm_tree* t, int c) {
+  if (t->graphs[c]) return STM_OK;
+  Scratch& s = t->scr;
+  hipStream_t st = t->stream;
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  k_advance<<<1, 64, 0, st>>>(s.desc_table, s.cursor, s.desc_live, s.acc, t->view.hdr);
+  hipError_t e = hipGetLastError();
+  PipeIn in{nullptr, nullptr, nullptr, class_nmax(t, c), nullptr, nullptr, s.desc_live};
+  if (e == hipSuccess) e = enqueue_pipeline(t, in, false);
+  if (e == hipSuccess) {
+    k_epilogue<<<1, 64, 0, st>>>(s.counters, s.desc_live, s.acc);
+    e = hipGetLastError();
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipGraphInstantiate(&t->graphs[c], g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    t->graphs[c] = nullptr;
+    return fail(STM_ERR_DEVICE, "graph capture of size class %d: %s", c, hipGetErrorString(e));
+  }
+  return STM_OK;
+}
+
+int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+               const uint64_t* offsets, size_t nb, uint64_t* out_vals, uint8_t* out_found) {
+  Scratch& s = t->scr;
+  hipStream_t st = t->stream;
+  const uint64_t mb = t->cfg.max_batch;
+  // Batches cut into chunks of at most max_batch ops, in order.
+  std::vector<BatchDesc> descs;
+  std::vector<uint8_t> cls;
+  for (size_t b = 0; b < nb; ++b) {
+    for (uint64_t off = offsets[b]; off < offsets[b + 1]; off += mb) {
+      const uint32_t m = (uint32_t)std::min<uint64_t>(mb, offsets[b + 1] - off);
+      descs.push_back(BatchDesc{keys + off, vals + off, ops + off, out_vals + off, out_found + off, m, 0});
+      cls.push_back((uint8_t)class_of(t, m));
+    }
+  }
+  if (descs.empty()) return STM_OK;
+  int rc = maybe_retrain(t);  // learned routing: the router's buffers exist before capture
+  if (rc) return rc;
+  for (uint8_t c : cls) {
+    rc = capture_class(t, c);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipMemsetAsync(s.acc, 0, sizeof(StreamAcc), st));
+  for (size_t w0 = 0; w0 < descs.size(); w0 += DESC_TABLE) {
+    const size_t w1 = std::min<size_t>(descs.size(), w0 + DESC_TABLE);
+    // Ordered after the replays of the previous window on the same stream.
+    HIP_TRY(hipMemcpyAsync(s.desc_table, descs.data() + w0, (w1 - w0) * sizeof(BatchDesc),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(s.cursor, 0, sizeof(uint32_t), st));
+    for (size_t i = w0; i < w1; ++i) {
+      rc = maybe_retrain(t);
+      if (rc) return rc;
+      HIP_TRY(hipGraphLaunch(t->graphs[cls[i]], st));
+      t->st.batches += 1;
+      t->st.ops += descs[i].n;
+      t->batches_since_train += 1;
+    }
+  }
+  StreamAcc acc;
+  HIP_TRY(hipMemcpyAsync(&acc, s.acc, sizeof(acc), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  t->st.unique_ops += acc.unique;
+  if (acc.bad)
+    return fail(STM_ERR_INVALID, "a batch holds the reserved key 0xffffffffffffffff (%llu later batches skipped)",
+                acc.skipped);
+  rc = check_device_error(t);
+  if (rc && acc.skipped) {
+    char buf[64];
+    snprintf(buf, sizeof(buf), " (%llu later batches skipped)", acc.skipped);
+    g_last_error += buf;
+  }
+  return rc;
 }
 
 int validate_host_batch(const uint64_t* keys, const uint8_t* ops, size_t n, bool need_vals,
@@ -458,12 +644,14 @@ int stm_config_init(stm_config* cfg) {
   cfg->device = 0;
   cfg->lanes_per_op = 16;
   cfg->ops_per_group = 16;
-  cfg->upsert_ops_per_wave = 1;
+  cfg->upsert_ops_per_wave = 512;
   cfg->max_attempts = 64;
   cfg->route = STM_ROUTE_DESCENT;
   cfg->timing = 0;
   cfg->retrain_every = 8;
   cfg->train_steps = 300;
+  cfg->sort_tile_items = 16;
+  cfg->use_graphs = 1;
   return STM_OK;
 }
 
@@ -485,7 +673,10 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
   if (!(lp == 8 || lp == 16 || lp == 32 || lp == 64) || lp > cfg.fanout)
     return fail(STM_ERR_INVALID, "lanes_per_op %u not in {8,16,32,64} or above fanout", lp);
   if (cfg.ops_per_group == 0) cfg.ops_per_group = 16;
-  if (cfg.upsert_ops_per_wave == 0) cfg.upsert_ops_per_wave = 1;
+  if (cfg.upsert_ops_per_wave == 0) cfg.upsert_ops_per_wave = 512;
+  if (cfg.sort_tile_items == 0) cfg.sort_tile_items = 16;
+  if (cfg.sort_tile_items != 8 && cfg.sort_tile_items != 16)
+    return fail(STM_ERR_INVALID, "sort_tile_items %u not in {8, 16}", cfg.sort_tile_items);
   if (cfg.max_attempts == 0) cfg.max_attempts = 64;
   if (cfg.retrain_every == 0) cfg.retrain_every = 8;
   if (cfg.train_steps > 100000) return fail(STM_ERR_INVALID, "train_steps above 100000");
@@ -537,6 +728,7 @@ int stm_destroy(stm_tree* t) {
   if (!t) return STM_OK;
   (void)hipSetDevice(t->cfg.device);
   if (t->stream) (void)hipStreamSynchronize(t->stream);
+  drop_graphs(t);
   t->scr.release();
   void* ps[] = {t->view.keys, t->view.vals, t->view.meta, t->view.hdr, t->view.stats};
   for (void* p : ps)
@@ -616,6 +808,46 @@ int stm_run_batch_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_
   int rc = set_device(t);
   if (rc) return rc;
   return run_device(t, d_keys, d_vals, d_ops, n, d_out_vals, d_out_found);
+}
+
+int stm_run_batches_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_vals,
+                           const uint8_t* d_ops, const uint64_t* offsets, size_t n_batches,
+                           uint64_t* d_out_vals, uint8_t* d_out_found) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n_batches == 0) return STM_OK;
+  if (!offsets) return fail(STM_ERR_INVALID, "null offsets");
+  for (size_t b = 0; b < n_batches; ++b)
+    if (offsets[b + 1] < offsets[b])
+      return fail(STM_ERR_INVALID, "offsets decrease at batch %zu", b);
+  if (offsets[n_batches] == offsets[0]) return STM_OK;
+  if (!d_keys || !d_vals || !d_ops || !d_out_vals || !d_out_found)
+    return fail(STM_ERR_INVALID, "null device array (device batches always carry values)");
+  int rc = set_device(t);
+  if (rc) return rc;
+  if (!t->cfg.use_graphs) {
+    for (size_t b = 0; b < n_batches; ++b) {
+      const uint64_t o = offsets[b];
+      rc = run_device(t, d_keys + o, d_vals + o, d_ops + o, offsets[b + 1] - o, d_out_vals + o,
+                      d_out_found + o);
+      if (rc) return rc;
+    }
+    return STM_OK;
+  }
+  return run_stream(t, d_keys, d_vals, d_ops, offsets, n_batches, d_out_vals, d_out_found);
+}
+
+int stm_capture_graphs(stm_tree* t) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  int rc = set_device(t);
+  if (rc) return rc;
+  rc = maybe_retrain(t);  // learned routing: the router's buffers exist before capture
+  if (rc) return rc;
+  const int last = class_of(t, (uint32_t)t->cfg.max_batch);
+  for (int c = 0; c <= last; ++c) {
+    rc = capture_class(t, c);
+    if (rc) return rc;
+  }
+  return STM_OK;
 }
 
 int stm_run_batch(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,

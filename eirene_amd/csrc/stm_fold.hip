@@ -96,26 +96,44 @@ STM_DEV Seg seg_element(const FoldArgs& a, uint32_t i, uint64_t key, uint64_t pr
 // A tile is processed in FOLD_ITEMS rounds of FOLD_BLOCK consecutive elements: thread t
 // owns element round * FOLD_BLOCK + t, so every global access of a round is coalesced and
 // the group outputs of a round land in consecutive slots.
-__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_reduce(FoldArgs a, Seg* tile_agg) {
-  __shared__ Seg s_w[FOLD_WAVES];
-  const uint32_t tile0 = blockIdx.x * FOLD_TILE;
-  Seg acc = seg_identity();
-  for (int r = 0; r < FOLD_ITEMS; ++r) {
-    const uint32_t r0 = tile0 + r * FOLD_BLOCK;
-    if (r0 >= a.n) break;  // uniform over the block
-    const uint32_t i = r0 + threadIdx.x;
-    Seg e = seg_identity();
-    if (i < a.n) e = seg_element(a, i, a.skeys[i], i ? a.skeys[i - 1] : 0);
-    Seg total;
-    (void)block_exclusive_seg(e, s_w, total);
-    acc = seg_combine(acc, total);
-  }
-  if (threadIdx.x == 0) tile_agg[blockIdx.x] = acc;
+// With a batch descriptor the element count and the caller's arrays come from device memory
+// (one launch sequence serves every batch size).
+STM_DEV void fold_resolve(FoldArgs& a) {
+  const BatchDesc* d = a.desc;
+  if (!d) return;
+  a.n = d->n;
+  a.vals = d->vals;
+  a.ops = d->ops;
+  a.out_vals = d->out_vals;
+  a.out_found = d->out_found;
 }
 
-__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, uint32_t ntiles,
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_reduce(FoldArgs a, Seg* tile_agg) {
+  __shared__ Seg s_w[FOLD_WAVES];
+  fold_resolve(a);
+  const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
+  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const uint32_t tile0 = b * FOLD_TILE;
+    Seg acc = seg_identity();
+    for (int r = 0; r < FOLD_ITEMS; ++r) {
+      const uint32_t r0 = tile0 + r * FOLD_BLOCK;
+      if (r0 >= a.n) break;  // uniform over the block
+      const uint32_t i = r0 + threadIdx.x;
+      Seg e = seg_identity();
+      if (i < a.n) e = seg_element(a, i, a.skeys[i], i ? a.skeys[i - 1] : 0);
+      Seg total;
+      (void)block_exclusive_seg(e, s_w, total);
+      acc = seg_combine(acc, total);
+    }
+    if (threadIdx.x == 0) tile_agg[b] = acc;
+  }
+}
+
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, FoldArgs a,
                                                                 BatchCounters* counters) {
   __shared__ Seg s_w[FOLD_WAVES];
+  fold_resolve(a);
+  const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
   const uint32_t per = (ntiles + FOLD_BLOCK - 1) / FOLD_BLOCK;
   const uint32_t b0 = threadIdx.x * per;
   const uint32_t b1 = min(b0 + per, ntiles);
@@ -142,8 +160,11 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
   __shared__ uint32_t s_up;
   if (threadIdx.x == 0) s_up = 0;
   uint32_t n_up = 0;  // merged INSERTs found by this wavefront (lane 0 keeps the count)
-  const uint32_t tile0 = blockIdx.x * FOLD_TILE;
-  Seg carry = tile_prefix[blockIdx.x];  // state of every element before this round
+  fold_resolve(a);
+  const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
+  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+  const uint32_t tile0 = b * FOLD_TILE;
+  Seg carry = tile_prefix[b];  // state of every element before this round
   for (int r = 0; r < FOLD_ITEMS; ++r) {
     const uint32_t r0 = tile0 + r * FOLD_BLOCK;
     if (r0 >= a.n) break;  // uniform over the block
@@ -194,6 +215,7 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
     const uint64_t want = ballot(append);
     if (lane_id() == 0) n_up += (uint32_t)__popcll(want);
   }
+  }  // tiles
   // One counter update per workgroup: a single hot atomic per wavefront round would
   // serialize the whole kernel on one L2 channel.
   if (lane_id() == 0 && n_up) atomicAdd(&s_up, n_up);
@@ -207,13 +229,15 @@ size_t fold_scratch_bytes(uint64_t n) {
   return (size_t)ntiles * sizeof(Seg);
 }
 
+// a.n is the element count, or with a.desc the largest count the scratch was sized for.
 hipError_t fold_batch(const FoldArgs& a, hipStream_t stream) {
   const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
   if (a.n == 0) return hipSuccess;
+  const uint32_t grid = std::min<uint32_t>(ntiles, 2048);
   Seg* tiles = (Seg*)a.tile_scratch;
-  k_fold_reduce<<<ntiles, FOLD_BLOCK, 0, stream>>>(a, tiles);
-  k_fold_scan_tiles<<<1, FOLD_BLOCK, 0, stream>>>(tiles, ntiles, a.counters);
-  k_fold_apply<<<ntiles, FOLD_BLOCK, 0, stream>>>(a, tiles);
+  k_fold_reduce<<<grid, FOLD_BLOCK, 0, stream>>>(a, tiles);
+  k_fold_scan_tiles<<<1, FOLD_BLOCK, 0, stream>>>(tiles, a, a.counters);
+  k_fold_apply<<<grid, FOLD_BLOCK, 0, stream>>>(a, tiles);
   return hipGetLastError();
 }
 

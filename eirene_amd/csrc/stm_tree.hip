@@ -20,6 +20,8 @@
 
 namespace stm {
 
+constexpr uint32_t LOOKUP_GRID_CAP = 8192;
+
 // ================================================================== bulk build
 template <int F>
 __global__ void k_build_leaves(TreeView t, const uint64_t* __restrict__ ukey,
@@ -215,16 +217,20 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
                                                 const RouteHeader* __restrict__ rh,
                                                 const uint64_t* __restrict__ fences,
                                                 const uint32_t* __restrict__ lids,
-                                                const SegModel* __restrict__ segs) {
+                                                const SegModel* __restrict__ segs,
+                                                const BatchDesc* __restrict__ desc) {
   constexpr int K = F / G;
   const int lane = lane_id();
   const int g = lane & (G - 1);
   const int gbase = lane - g;
-  const uint64_t grp = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G;
+  const uint64_t grp0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G;
+  const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / G;
   const uint32_t U = cnt->n_unique;
-  const uint64_t i0 = grp * chunk;
-  if (i0 >= U) return;
-  const uint32_t i1 = (uint32_t)min<uint64_t>(i0 + chunk, U);
+  if (grp0 * chunk >= U) return;
+  if (desc) {
+    out_vals = desc->out_vals;
+    out_found = desc->out_found;
+  }
   const uint32_t cap = t.capacity;
   const uint32_t root = (uint32_t)min<uint64_t>(t.hdr->root, cap - 1);
   const uint32_t height = (uint32_t)min<uint64_t>(t.hdr->height, 64);
@@ -240,7 +246,12 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   uint32_t hits = 0, misses = 0;
   if constexpr (LEARNED) rv = route_view(rh, fences, lids, segs, t.dir);
 
-  for (uint32_t base = (uint32_t)i0; base < i1; base += G) {
+  // Grid-stride over chunks: a lane group's later chunks hold larger keys, so its cached
+  // leaf, parent and directory cursor stay usable.
+  for (uint64_t grp = grp0; grp * chunk < U; grp += ngroups) {
+  const uint32_t i0 = (uint32_t)(grp * chunk);
+  const uint32_t i1 = (uint32_t)min<uint64_t>((uint64_t)i0 + chunk, U);
+  for (uint32_t base = i0; base < i1; base += G) {
     // Lane g owns op base + g: its metadata comes in one coalesced round trip, and the
     // sorted position of its first waiting op in a second one that overlaps the walk.
     const uint32_t j = base + g;
@@ -342,10 +353,11 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
       }
     }
   }
+  }  // chunks
   if constexpr (LEARNED) {
     // Spread over ROUTE_STAT_SLOTS counters so no single address serializes the kernel.
     if (g == 0) {
-      const uint32_t slot = (uint32_t)(grp % ROUTE_STAT_SLOTS);
+      const uint32_t slot = (uint32_t)(grp0 % ROUTE_STAT_SLOTS);
       if (hits) atomicAdd(t.stats + ST_NUM + slot, (unsigned long long)hits);
       if (misses) atomicAdd(t.stats + ST_NUM + ROUTE_STAT_SLOTS + slot, (unsigned long long)misses);
     }
@@ -914,8 +926,13 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
                                                 BatchCounters* cnt, uint32_t* __restrict__ dlist,
                                                 uint32_t max_attempts,
                                                 uint64_t* __restrict__ out_vals,
-                                                uint8_t* __restrict__ out_found) {
+                                                uint8_t* __restrict__ out_found,
+                                                const BatchDesc* __restrict__ desc) {
   const uint32_t U = cnt->n_unique;
+  if (desc) {
+    out_vals = desc->out_vals;
+    out_found = desc->out_found;
+  }
   const uint32_t nw = gridDim.x * (blockDim.x / WAVE);
   const uint32_t w = uniform32((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   const uint32_t per = (U + nw - 1) / nw;
@@ -968,8 +985,13 @@ __global__ __launch_bounds__(64) void k_upsert_serial(TreeView t, const uint64_t
                                                       const uint32_t* __restrict__ dlist,
                                                       const BatchCounters* cnt,
                                                       uint64_t* __restrict__ out_vals,
-                                                      uint8_t* __restrict__ out_found) {
+                                                      uint8_t* __restrict__ out_found,
+                                                      const BatchDesc* __restrict__ desc) {
   const uint32_t n = cnt->n_deferred;
+  if (desc) {
+    out_vals = desc->out_vals;
+    out_found = desc->out_found;
+  }
   TxCounters c;
   ParentCache<F> pc;
   for (uint32_t e = 0; e < n; ++e) {
@@ -1122,16 +1144,17 @@ template <int F, int G>
 static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipStream_t s) {
   const uint64_t groups = (a.n_max + a.chunk - 1) / a.chunk;
   const uint64_t threads = groups * G;
-  const uint32_t blocks = (uint32_t)((threads + 255) / 256);
+  // Persistent grid: at most LOOKUP_GRID_CAP workgroups, each lane group loops over chunks.
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((threads + 255) / 256, LOOKUP_GRID_CAP);
   if (blocks == 0) return hipSuccess;
   if (a.route_hdr)
     k_lookup<F, G, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                  a.counters, a.chunk, a.out_vals, a.out_found,
-                                                 a.route_hdr, a.fences, a.lids, a.segs);
+                                                 a.route_hdr, a.fences, a.lids, a.segs, a.desc);
   else
     k_lookup<F, G, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                   a.counters, a.chunk, a.out_vals, a.out_found,
-                                                  nullptr, nullptr, nullptr, nullptr);
+                                                  nullptr, nullptr, nullptr, nullptr, a.desc);
   return hipGetLastError();
 }
 
@@ -1154,11 +1177,11 @@ template <int F>
 static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
   k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
                                         a.counters, a.dlist, a.max_attempts, a.out_vals,
-                                        a.out_found);
+                                        a.out_found, a.desc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_upsert_serial<F><<<1, 64, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.dlist,
-                                      a.counters, a.out_vals, a.out_found);
+                                      a.counters, a.out_vals, a.out_found, a.desc);
   return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ struct LookupArgs {
   const uint64_t* fences;
   const uint32_t* lids;
   const SegModel* segs;
+  const BatchDesc* desc;  // optional: out_vals / out_found are taken from here
 };
 
 struct UpsertArgs {
@@ -38,6 +39,7 @@ struct UpsertArgs {
   uint32_t blocks;     // 256-thread workgroups; each wavefront owns a range of groups
   uint64_t* out_vals;
   uint8_t* out_found;
+  const BatchDesc* desc;
 };
 
 hipError_t launch_build_leaves(const TreeView& t, const uint64_t* ukey, const uint64_t* uval,

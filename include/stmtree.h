@@ -60,13 +60,18 @@ typedef struct stm_config {
     int32_t device;       /* HIP device ordinal (default 0) */
     uint32_t lanes_per_op;/* lookup kernel lanes per op: 8, 16, 32 or 64 (default 16) */
     uint32_t ops_per_group;/* sorted ops walked by one lane group (default 16) */
-    uint32_t upsert_ops_per_wave; /* upserts handled by one wavefront (default 1) */
+    uint32_t upsert_ops_per_wave; /* batch ops per upsert wavefront: the upsert grid is
+                                     max_batch / this many wavefronts, at most 8192 (default 512) */
     uint32_t max_attempts;/* STM attempts before an op is deferred to the serial pass (default 64) */
     uint32_t route;       /* STM_ROUTE_* */
     uint32_t timing;      /* 1 = record HIP events around every phase (stm_get_stats) */
     uint32_t retrain_every; /* learned routing: rebuild + retrain after this many batches (default 8) */
     uint32_t train_steps; /* learned routing: Adam steps per segment MLP (default 300) */
-    uint32_t reserved[3];
+    uint32_t sort_tile_items; /* radix sort elements per thread: 8 or 16 (tiles of 2048 or 4096
+                                 keys; default 16) */
+    uint32_t use_graphs;  /* stm_run_batches_device: 1 = replay one captured hipGraph per batch
+                             size class (default 1), 0 = launch every kernel from the host */
+    uint32_t reserved[1];
 } stm_config;
 
 /* Per-phase timing slots of stm_stats.phase_ms / phase_launches. */
@@ -132,6 +137,27 @@ int stm_run_batch(stm_tree* tree, const uint64_t* keys, const uint64_t* vals, co
  * the tree's stream (stm_get_stream); the call returns after the batch has completed. */
 int stm_run_batch_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
                          const uint8_t* d_ops, size_t n, uint64_t* d_out_vals, uint8_t* d_out_found);
+
+/* Run a stream of batches stored back to back in device memory: batch b is ops
+ * [offsets[b], offsets[b+1]) of d_keys / d_vals / d_ops and is answered into the same range of
+ * d_out_vals / d_out_found.  offsets is a host array of n_batches + 1 non-decreasing entries;
+ * a batch longer than max_batch is chunked as in stm_run_batch_device.  The results equal
+ * n_batches calls of stm_run_batch_device, in order.  With use_graphs the launch sequence of a
+ * batch is captured once per size class (powers of 4 up to max_batch) as a hipGraph that reads
+ * its batch from a device-side descriptor table, and replayed: the host reads nothing back
+ * until the stream has run (a reserved key or a full node pool is reported at the end, the
+ * tree then holds every batch up to and including the failing one).
+ * Replaces the batch loop of GPU_STM_Tree::gpu_work_main (src/gpu-stm-tree-v38/bpt-gpustm.cpp:77-135:
+ * transferData, launchKernelBefore, launchKernel, launchKernelAfter, transferAns per batch). */
+int stm_run_batches_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
+                           const uint8_t* d_ops, const uint64_t* offsets, size_t n_batches,
+                           uint64_t* d_out_vals, uint8_t* d_out_found);
+
+/* Captures the graphs of every batch size class now (stm_run_batches_device captures a class
+ * the first time a batch of that size arrives), so a latency-sensitive caller pays no capture
+ * inside its stream.  No reference counterpart (the reference launches every kernel from the
+ * host, src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:264-332). */
+int stm_capture_graphs(stm_tree* tree);
 
 /* Number of keys in the tree. */
 int stm_size(stm_tree* tree, uint64_t* n_keys);
