@@ -57,13 +57,32 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU-baseline budget (batches run until it is spent)")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="skip the learned-vs-plain section (profiling the uneven stream alone)")
     ap.add_argument("--uneven-batches", type=int, default=256,
                     help="uneven stream: batches log-uniform in [2^10, 2^22] (0 = skip)")
     ap.add_argument("--tuning", default=None,
                     help="tuning record to use for the uneven stream instead of autotuning")
     ap.add_argument("--tune-batches", type=int, default=40,
                     help="batches of the stream the autotuner measures")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the library's copy kernel 4 times over MIB MiB (the known-size "
+                         "kernel HBM traffic counters are scaled by) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import eirene_amd as ea
+    n = mib * (1 << 20) // 8
+    src = ea.DeviceBuffer(n, np.uint64)
+    dst = ea.DeviceBuffer(n, np.uint64)
+    dst.copy_from(src)  # first touch
+    t0 = time.perf_counter()
+    for _ in range(4):
+        dst.copy_from(src)
+    dt = (time.perf_counter() - t0) / 4
+    print(json.dumps({"kernel": "k_copy16", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "dispatches": 5, "gbs_read_plus_write": round(2 * n * 8 / dt / 1e9, 1)}))
 
 
 def log(msg):
@@ -139,6 +158,9 @@ def run_uneven(args, ea, w):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the batch path has no "
                           "multi-GPU mode (key-range sharding is a design note)",
@@ -146,6 +168,11 @@ def main():
         sys.exit(2)
     import eirene_amd as ea
     from eirene_amd import workloads as w
+
+    if args.no_headline:
+        print(json.dumps({"uneven": run_uneven(args, ea, w) if args.uneven_batches > 0 else None}),
+              flush=True)
+        return
 
     n_load = 1 << args.load_log2
     n_batch = 1 << args.batch_log2

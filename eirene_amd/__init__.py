@@ -35,7 +35,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libstmtree.so"
 OP_INSERT, OP_GET, OP_DELETE, OP_RANGE = 0, 1, 2, 3
 KEY_RESERVED = 0xFFFFFFFFFFFFFFFF
 ROUTE_DESCENT, ROUTE_LEARNED = 0, 1
-PHASES = ("sort", "merge", "lookup", "upsert", "deferred", "route", "train", "other")
+PHASES = ("sort", "merge", "lookup", "upsert", "deferred", "route", "train", "scan")
 
 STATUS = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "STATE"}
 
@@ -89,6 +89,9 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_run_batch": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
         "stm_run_batch_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
         "stm_run_batches_device": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, _P, _P]),
+        "stm_run_batch_scan": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P, _P, _P, C.c_size_t]),
+        "stm_run_batch_scan_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P, _P, _P,
+                                                C.c_size_t]),
         "stm_capture_graphs": (C.c_int, [_P]),
         "stm_size": (C.c_int, [_P, _U64P]),
         "stm_dump": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -101,6 +104,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_device_free": (C.c_int, [_P]),
         "stm_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
         "stm_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+        "stm_device_copy": (C.c_int, [_P, _P, C.c_size_t]),
         "stm_sync": (C.c_int, [_P]),
         "stm_retrain": (C.c_int, [_P, C.c_uint32]),
         "stm_route_info": (C.c_int, [_P, _U64P, _U64P, _U64P]),
@@ -174,6 +178,13 @@ class DeviceBuffer:
         out = np.empty(n, dtype=self.dtype)
         _check(load_library().stm_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
         return out
+
+    def copy_from(self, other: "DeviceBuffer") -> None:
+        """Device-to-device copy of other's bytes (same size) by the library's copy kernel."""
+        nbytes = self.n * np.dtype(self.dtype).itemsize
+        if other.n * np.dtype(other.dtype).itemsize != nbytes:
+            raise ValueError("buffers differ in size")
+        _check(load_library().stm_device_copy(self.ptr, other.ptr, nbytes))
 
     def free(self) -> None:
         if getattr(self, "ptr", None):
@@ -268,6 +279,37 @@ class StmTree:
         _check(self._lib.stm_run_batch(self._h, _ptr(k), _ptr(v) if v is not None else None,
                                        _ptr(o), n, _ptr(out_v), _ptr(out_f)))
         return out_v, out_f
+
+    def run_batch_scan(self, keys, vals, ops):
+        """One batch in which op 3 is SCAN(key, L = vals[i]).  Returns (values, found,
+        scan_keys, scan_vals): scan op i's pairs start at the sum of the L of the scans before
+        it, and values[i] is the number of pairs it found."""
+        k, v, o = _u64(keys), _u64(vals), _u8(ops)
+        n = k.size
+        if o.size != n or v.size != n:
+            raise ValueError("keys, vals and ops differ in length")
+        cap = int(v[o == 3].sum()) if n else 0
+        out_v = np.zeros(n, dtype=np.uint64)
+        out_f = np.zeros(n, dtype=np.uint8)
+        sk = np.zeros(max(cap, 1), dtype=np.uint64)
+        sv = np.zeros(max(cap, 1), dtype=np.uint64)
+        _check(self._lib.stm_run_batch_scan(self._h, _ptr(k), _ptr(v), _ptr(o), n, _ptr(out_v),
+                                            _ptr(out_f), _ptr(sk), _ptr(sv), cap))
+        return out_v, out_f, sk[:cap], sv[:cap]
+
+    def run_batch_scan_device(self, keys: "DeviceBuffer", vals: "DeviceBuffer",
+                              ops: "DeviceBuffer", out_vals: "DeviceBuffer",
+                              out_found: "DeviceBuffer", scan_keys: "DeviceBuffer",
+                              scan_vals: "DeviceBuffer", n: Optional[int] = None,
+                              offset: int = 0, scan_offset: int = 0) -> None:
+        """Device-resident scan batch of n ops from element `offset`; scan pairs go to
+        scan_keys / scan_vals from element `scan_offset`."""
+        n = keys.n - offset if n is None else int(n)
+        cap = scan_keys.n - scan_offset
+        _check(self._lib.stm_run_batch_scan_device(
+            self._h, keys.ptr + 8 * offset, vals.ptr + 8 * offset, ops.ptr + offset, n,
+            out_vals.ptr + 8 * offset, out_found.ptr + offset, scan_keys.ptr + 8 * scan_offset,
+            scan_vals.ptr + 8 * scan_offset, cap))
 
     def run_batch_device(self, keys: "DeviceBuffer", vals: "DeviceBuffer", ops: "DeviceBuffer",
                          out_vals: "DeviceBuffer", out_found: "DeviceBuffer",

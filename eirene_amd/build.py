@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "stm_fold.hip",
     "stm_tree.hip",
     "stm_learned.hip",
+    "stm_scan.hip",
     "stm_api.hip",
 ]
 CPP_SOURCES = [

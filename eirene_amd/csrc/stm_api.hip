@@ -92,6 +92,13 @@ __global__ void k_epilogue(const BatchCounters* __restrict__ cnt, const BatchDes
   acc->batches += 1;
 }
 
+// Plain streaming copy: the known-size reference kernel for HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                uint64_t n16) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256ull)
+    dst[i] = src[i];
+}
+
 struct Scratch {
   uint64_t n = 0;
   uint64_t *in_keys = nullptr, *in_vals = nullptr, *out_vals = nullptr;
@@ -104,6 +111,9 @@ struct Scratch {
   uint8_t* uop = nullptr;
   uint32_t *gstart = nullptr, *npend = nullptr, *dlist = nullptr;
   BatchCounters* counters = nullptr;
+  // range scans
+  uint64_t *soff = nullptr, *scan_tiles = nullptr;
+  int32_t* lastins = nullptr;
   // batch streams replayed from captured graphs
   BatchDesc* desc_table = nullptr;  // DESC_TABLE entries
   BatchDesc* desc_live = nullptr;   // the batch the current replay works on
@@ -127,6 +137,7 @@ struct Scratch {
     A_(counters, 1);
     if (staging) {
       A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
+      A_(soff, cap); A_(lastins, cap); A_(scan_tiles, scan_tile_words(cap));
     }
 #undef A_
     return hipSuccess;
@@ -134,7 +145,7 @@ struct Scratch {
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
                   hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, counters,
-                  desc_table, desc_live, cursor, acc};
+                  desc_table, desc_live, cursor, acc, soff, lastins, scan_tiles};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = Scratch();
@@ -308,6 +319,11 @@ struct PipeIn {
   uint64_t* out_vals;
   uint8_t* out_found;
   const BatchDesc* desc;
+  // scan mode: op 3 is SCAN(key, L = vals[i]) with output into scan_keys / scan_vals
+  bool scans;
+  uint64_t* scan_keys;
+  uint64_t* scan_vals;
+  uint64_t scan_cap;
 };
 
 hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
@@ -337,8 +353,32 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   fa.npend = s.npend;
   fa.counters = s.counters;
   fa.tile_scratch = s.fold_tiles;
+  fa.lastins = in.scans ? s.lastins : nullptr;
   if ((e = fold_batch(fa, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_MERGE);
+
+  ScanArgs sa{};
+  if (in.scans) {
+    sa.keys = in.keys;
+    sa.vals = in.vals;
+    sa.ops = in.ops;
+    sa.n = in.n;
+    sa.desc = in.desc;
+    sa.out_vals = in.out_vals;
+    sa.out_found = in.out_found;
+    sa.scan_keys = in.scan_keys;
+    sa.scan_vals = in.scan_vals;
+    sa.scan_cap = in.scan_cap;
+    sa.ukey = s.ukey;
+    sa.uop = s.uop;
+    sa.gstart = s.gstart;
+    sa.npend = s.npend;
+    sa.sidx = s.sidx;
+    sa.lastins = s.lastins;
+    sa.counters = s.counters;
+    sa.soff = s.soff;
+    sa.tile_sums = s.scan_tiles;
+  }
 
   if (timed) phase_begin(t, STM_PHASE_LOOKUP);
   LookupArgs la{};
@@ -363,6 +403,13 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   if ((e = launch_lookup(t->view, la, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_LOOKUP);
 
+  if (in.scans) {  // on the tree as it was before the batch: before the upsert kernel
+    if (timed) phase_begin(t, STM_PHASE_SCAN);
+    if ((e = launch_scan_offsets(sa, st)) != hipSuccess) return e;
+    if ((e = launch_range_scan(t->view, sa, st)) != hipSuccess) return e;
+    if (timed) phase_end(t, STM_PHASE_SCAN);
+  }
+
   if (timed) phase_begin(t, STM_PHASE_UPSERT);
   UpsertArgs ua{};
   ua.ukey = s.ukey;
@@ -383,6 +430,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   ua.desc = in.desc;
   if ((e = launch_upsert(t->view, ua, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_UPSERT);
+  if (in.scans && (e = launch_scan_answers(sa, st)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -393,14 +441,28 @@ int maybe_retrain(stm_tree* t) {
   return train_router(t, t->cfg.train_steps);
 }
 
+// Scan output area of a chunk (device pointers); null = op 3 runs as GET.
+struct ScanOut {
+  uint64_t* keys;
+  uint64_t* vals;
+  uint64_t cap;
+  uint64_t used;  // set by run_chunk: the sum of the chunk's scan lengths
+};
+
 // One chunk of at most max_batch ops, all pointers on the device; waits for its completion.
 int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
-              uint32_t n, uint64_t* out_vals, uint8_t* out_found) {
+              uint32_t n, uint64_t* out_vals, uint8_t* out_found, ScanOut* so = nullptr) {
   Scratch& s = t->scr;
   hipStream_t st = t->stream;
   int rc = maybe_retrain(t);
   if (rc) return rc;
-  PipeIn in{keys, vals, ops, n, out_vals, out_found, nullptr};
+  PipeIn in{keys, vals, ops, n, out_vals, out_found, nullptr, false, nullptr, nullptr, 0};
+  if (so) {
+    in.scans = true;
+    in.scan_keys = so->keys;
+    in.scan_vals = so->vals;
+    in.scan_cap = so->cap;
+  }
   HIP_TRY(enqueue_pipeline(t, in, true));
   BatchCounters bc;
   HIP_TRY(hipMemcpyAsync(&bc, s.counters, sizeof(bc), hipMemcpyDeviceToHost, st));
@@ -411,6 +473,13 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   t->st.unique_ops += bc.n_unique;
   t->batches_since_train += 1;
   if (bc.n_bad) return fail(STM_ERR_INVALID, "batch holds the reserved key 0xffffffffffffffff");
+  if (so) {
+    so->used = bc.scan_total;
+    if (bc.scan_bad)
+      return fail(STM_ERR_INVALID, "%u scan ops have a length outside [1, %u] or overflow the "
+                  "scan output area (%llu pairs)", bc.scan_bad, SCAN_MAX_LEN,
+                  (unsigned long long)so->cap);
+  }
   return check_device_error(t);
 }
 
@@ -434,7 +503,8 @@ int capture_class(stm_tree* t, int c) {
   HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   k_advance<<<1, 64, 0, st>>>(s.desc_table, s.cursor, s.desc_live, s.acc, t->view.hdr);
   hipError_t e = hipGetLastError();
-  PipeIn in{nullptr, nullptr, nullptr, class_nmax(t, c), nullptr, nullptr, s.desc_live};
+  PipeIn in{nullptr, nullptr, nullptr, class_nmax(t, c), nullptr, nullptr, s.desc_live,
+            false, nullptr, nullptr, 0};
   if (e == hipSuccess) e = enqueue_pipeline(t, in, false);
   if (e == hipSuccess) {
     k_epilogue<<<1, 64, 0, st>>>(s.counters, s.desc_live, s.acc);
@@ -463,7 +533,8 @@ int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const ui
   for (size_t b = 0; b < nb; ++b) {
     for (uint64_t off = offsets[b]; off < offsets[b + 1]; off += mb) {
       const uint32_t m = (uint32_t)std::min<uint64_t>(mb, offsets[b + 1] - off);
-      descs.push_back(BatchDesc{keys + off, vals + off, ops + off, out_vals + off, out_found + off, m, 0});
+      descs.push_back(BatchDesc{keys + off, vals + off, ops + off, out_vals + off, out_found + off, m,
+                                0, nullptr, nullptr, 0});
       cls.push_back((uint8_t)class_of(t, m));
     }
   }
@@ -836,6 +907,98 @@ int stm_run_batches_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* 
   return run_stream(t, d_keys, d_vals, d_ops, offsets, n_batches, d_out_vals, d_out_found);
 }
 
+int stm_run_batch_scan_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_vals,
+                              const uint8_t* d_ops, size_t n, uint64_t* d_out_vals,
+                              uint8_t* d_out_found, uint64_t* d_scan_keys, uint64_t* d_scan_vals,
+                              size_t scan_cap) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n == 0) return STM_OK;
+  if (!d_keys || !d_vals || !d_ops || !d_out_vals || !d_out_found)
+    return fail(STM_ERR_INVALID, "null device array (device batches always carry values)");
+  if (scan_cap && (!d_scan_keys || !d_scan_vals)) return fail(STM_ERR_INVALID, "null scan area");
+  int rc = set_device(t);
+  if (rc) return rc;
+  const uint64_t mb = t->cfg.max_batch;
+  uint64_t base = 0;
+  for (size_t off = 0; off < n; off += mb) {
+    const uint32_t m = (uint32_t)std::min<uint64_t>(mb, n - off);
+    ScanOut so{d_scan_keys ? d_scan_keys + base : nullptr, d_scan_vals ? d_scan_vals + base : nullptr,
+               scan_cap - base, 0};
+    rc = run_chunk(t, d_keys + off, d_vals + off, d_ops + off, m, d_out_vals + off,
+                   d_out_found + off, &so);
+    if (rc) return rc;
+    base += std::min<uint64_t>(so.used, scan_cap - base);
+  }
+  return STM_OK;
+}
+
+int stm_run_batch_scan(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+                       size_t n, uint64_t* out_vals, uint8_t* out_found, uint64_t* scan_keys,
+                       uint64_t* scan_vals, size_t scan_cap) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n == 0) return STM_OK;
+  if (!keys || !vals || !ops || !out_vals || !out_found) return fail(STM_ERR_INVALID, "null array");
+  int rc = set_device(t);
+  if (rc) return rc;
+  rc = validate_host_batch(keys, ops, n, true, vals);
+  if (rc) return rc;
+  // Scan lengths: checked here, and each chunk's output area sized from them.
+  const uint64_t mb = t->cfg.max_batch;
+  std::vector<uint64_t> chunk_pairs;
+  uint64_t total = 0;
+  for (size_t off = 0; off < n; off += mb) {
+    const size_t m = (size_t)std::min<uint64_t>(mb, n - off);
+    uint64_t c = 0;
+    for (size_t i = off; i < off + m; ++i) {
+      if (ops[i] != STM_OP_RANGE) continue;
+      if (vals[i] < 1 || vals[i] > STM_SCAN_MAX_LEN)
+        return fail(STM_ERR_INVALID, "scan op %zu has length %llu, not in [1, %u]", i,
+                    (unsigned long long)vals[i], STM_SCAN_MAX_LEN);
+      c += vals[i];
+    }
+    chunk_pairs.push_back(c);
+    total += c;
+  }
+  if (total > scan_cap)
+    return fail(STM_ERR_INVALID, "scans need %llu pairs, the area holds %zu",
+                (unsigned long long)total, scan_cap);
+  if (total && (!scan_keys || !scan_vals)) return fail(STM_ERR_INVALID, "null scan area");
+  const uint64_t most = *std::max_element(chunk_pairs.begin(), chunk_pairs.end());
+  uint64_t *dsk = nullptr, *dsv = nullptr;
+  if (dalloc(&dsk, most) != hipSuccess || dalloc(&dsv, most) != hipSuccess) {
+    if (dsk) (void)hipFree(dsk);
+    return fail(STM_ERR_NOMEM, "scan output area of %llu pairs", (unsigned long long)most);
+  }
+  Scratch& s = t->scr;
+  uint64_t base = 0;
+  size_t ci = 0;
+  for (size_t off = 0; off < n && rc == STM_OK; off += mb, ++ci) {
+    const size_t m = (size_t)std::min<uint64_t>(mb, n - off);
+    hipError_t e = hipMemcpyAsync(s.in_keys, keys + off, m * 8, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.in_vals, vals + off, m * 8, hipMemcpyHostToDevice, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.in_ops, ops + off, m, hipMemcpyHostToDevice, t->stream);
+    if (e != hipSuccess) {
+      rc = fail(STM_ERR_DEVICE, "batch copy: %s", hipGetErrorString(e));
+      break;
+    }
+    ScanOut so{dsk, dsv, std::max<uint64_t>(most, 1), 0};
+    rc = run_chunk(t, s.in_keys, s.in_vals, s.in_ops, (uint32_t)m, s.out_vals, s.out_found, &so);
+    if (rc) break;
+    const uint64_t c = chunk_pairs[ci];
+    e = hipMemcpyAsync(out_vals + off, s.out_vals, m * 8, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out_found + off, s.out_found, m, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && c) e = hipMemcpyAsync(scan_keys + base, dsk, c * 8, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess && c) e = hipMemcpyAsync(scan_vals + base, dsv, c * 8, hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    if (e != hipSuccess) rc = fail(STM_ERR_DEVICE, "result copy: %s", hipGetErrorString(e));
+    base += c;
+  }
+  (void)hipStreamSynchronize(t->stream);
+  (void)hipFree(dsk);
+  (void)hipFree(dsv);
+  return rc;
+}
+
 int stm_capture_graphs(stm_tree* t) {
   if (!t) return fail(STM_ERR_INVALID, "null tree");
   int rc = set_device(t);
@@ -1094,6 +1257,19 @@ int stm_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   if (!bytes) return STM_OK;
   if (!dst || !src) return fail(STM_ERR_INVALID, "null argument");
   HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return STM_OK;
+}
+
+int stm_device_copy(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return STM_OK;
+  if (!dst || !src) return fail(STM_ERR_INVALID, "null argument");
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+    return fail(STM_ERR_INVALID, "device copy needs 16-byte aligned pointers and size");
+  const uint64_t n16 = bytes / 16;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 8192);
+  k_copy16<<<blocks, 256, 0, nullptr>>>((uint4*)dst, (const uint4*)src, n16);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
   return STM_OK;
 }
 

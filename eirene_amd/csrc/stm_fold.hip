@@ -184,6 +184,7 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
     uint32_t g = 0;
     if (valid) {
       const int32_t last_before = e.h ? -1 : before.last;  // latest earlier insert of this key
+      if (a.lastins) a.lastins[i] = e.first >= 0 ? (int32_t)i : last_before;
       if (a.out_vals && last_before >= 0) {
         const uint32_t pos = a.sidx[i];
         a.out_vals[pos] = a.vals[a.sidx[last_before]];

@@ -17,6 +17,10 @@ struct BatchDesc {
   uint8_t* out_found;
   uint32_t n;
   uint32_t pad;
+  // range-scan output area (scan mode only)
+  uint64_t* scan_keys;
+  uint64_t* scan_vals;
+  uint64_t scan_cap;
 };
 
 // ------------------------------------------------------------------ radix sort (stm_sort.hip)
@@ -44,8 +48,11 @@ struct BatchCounters {
   uint32_t n_deferred;  // upserts moved to the serial pass
   uint32_t queue_head;  // work-queue cursor of the upsert kernel
   uint32_t n_bad;       // ops rejected (reserved key)
-  uint32_t pad[11];
+  uint32_t scan_bad;    // scan ops rejected (length or output area)
+  uint64_t scan_total;  // sum of the scan lengths (pairs the output area must hold)
+  uint32_t pad[8];
 };
+static_assert(sizeof(BatchCounters) == 64, "one 64-byte block");
 
 struct FoldArgs {
   const uint64_t* skeys;  // sorted keys
@@ -64,9 +71,43 @@ struct FoldArgs {
   uint32_t* npend;        // ops answered by the tree (prefix of the group)
   BatchCounters* counters;
   void* tile_scratch;     // fold_scratch_bytes(n)
+  int32_t* lastins;       // scan mode: per sorted entry, the latest insert of its key at or
+                          // before it (sorted index), -1 if none; null otherwise
 };
 constexpr uint8_t UOP_SKIP = 0xff;
+constexpr uint8_t OP_RANGE = 3;
+constexpr uint32_t SCAN_MAX_LEN = 65536;  // longest scan (pairs)
 size_t fold_scratch_bytes(uint64_t n);
 hipError_t fold_batch(const FoldArgs& a, hipStream_t stream);
+
+// ------------------------------------------------------------------ range scans (stm_scan.hip)
+struct ScanArgs {
+  // the batch (or desc)
+  const uint64_t* keys;
+  const uint64_t* vals;
+  const uint8_t* ops;
+  uint32_t n;
+  const BatchDesc* desc;
+  uint64_t* out_vals;
+  uint8_t* out_found;
+  uint64_t* scan_keys;
+  uint64_t* scan_vals;
+  uint64_t scan_cap;
+  // fold output
+  const uint64_t* ukey;
+  const uint8_t* uop;
+  const uint32_t* gstart;
+  const uint32_t* npend;
+  const uint32_t* sidx;
+  const int32_t* lastins;
+  BatchCounters* counters;
+  // scratch
+  uint64_t* soff;       // per input position: output offset, then pair count
+  uint64_t* tile_sums;  // scan_tile_words(n)
+};
+size_t scan_tile_words(uint64_t n);
+hipError_t launch_scan_offsets(const ScanArgs& a, hipStream_t s);  // before the scan kernel
+hipError_t launch_range_scan(const TreeView& t, const ScanArgs& a, hipStream_t s);  // pre-batch tree
+hipError_t launch_scan_answers(const ScanArgs& a, hipStream_t s);  // after the upsert kernel
 
 }  // namespace stm

@@ -82,7 +82,7 @@ typedef struct stm_config {
 #define STM_PHASE_DEFERRED 4
 #define STM_PHASE_ROUTE 5
 #define STM_PHASE_TRAIN 6
-#define STM_PHASE_OTHER 7
+#define STM_PHASE_SCAN 7
 #define STM_NUM_PHASES 8
 
 typedef struct stm_stats {
@@ -137,6 +137,28 @@ int stm_run_batch(stm_tree* tree, const uint64_t* keys, const uint64_t* vals, co
  * the tree's stream (stm_get_stream); the call returns after the batch has completed. */
 int stm_run_batch_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
                          const uint8_t* d_ops, size_t n, uint64_t* d_out_vals, uint8_t* d_out_found);
+
+/* Longest range scan (pairs). */
+#define STM_SCAN_MAX_LEN 65536
+
+/* Run one batch in which op 3 (STM_OP_RANGE) is a range scan: SCAN(keys[i], L = vals[i]),
+ * 1 <= L <= STM_SCAN_MAX_LEN, returns the first L (key, value) pairs with key >= keys[i] in
+ * the state before op i (the tree plus the batch's earlier inserts), in ascending key order.
+ * Scan op i writes its pairs to scan_keys / scan_vals starting at off_i = the sum of the L of
+ * the scan ops before it in the batch (so the area needs the sum of all L, scan_cap >= that);
+ * out_vals[i] = pairs written (< L only at the end of the key space), out_found[i] = (pairs >
+ * 0); the rest of its L slots are left unwritten.  Other ops as stm_run_batch (DELETE as GET).
+ * A scan with a bad length or an overflowing area fails the call with STM_ERR_INVALID after
+ * the batch's other ops have run.  No reference counterpart: the reference declares RANGE but
+ * never implements it (src/tree.h:37-39, src/main.cpp:30-34). */
+int stm_run_batch_scan(stm_tree* tree, const uint64_t* keys, const uint64_t* vals,
+                       const uint8_t* ops, size_t n, uint64_t* out_vals, uint8_t* out_found,
+                       uint64_t* scan_keys, uint64_t* scan_vals, size_t scan_cap);
+/* Same with device-resident inputs, outputs and scan area. */
+int stm_run_batch_scan_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
+                              const uint8_t* d_ops, size_t n, uint64_t* d_out_vals,
+                              uint8_t* d_out_found, uint64_t* d_scan_keys, uint64_t* d_scan_vals,
+                              size_t scan_cap);
 
 /* Run a stream of batches stored back to back in device memory: batch b is ops
  * [offsets[b], offsets[b+1]) of d_keys / d_vals / d_ops and is answered into the same range of
@@ -201,6 +223,9 @@ int stm_device_alloc(int device, size_t bytes, void** ptr);
 int stm_device_free(void* ptr);
 int stm_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int stm_memcpy_d2h(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned pointers and size); it
+ * is also the known-size kernel that HBM traffic counters are scaled by. */
+int stm_device_copy(void* dst, const void* src, size_t bytes);
 /* Blocks until every kernel queued on the tree's stream has finished. */
 int stm_sync(stm_tree* tree);
 

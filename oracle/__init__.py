@@ -51,6 +51,8 @@ def _load():
             ("oracle_bulk_load", None, [P, P, P, C.c_size_t]),
             ("oracle_run_batch", None, [P, P, P, P, C.c_size_t, P, P]),
             ("oracle_pipeline_batch", None, [P, P, P, P, C.c_size_t, P, P]),
+            ("oracle_run_batch_scan", C.c_uint64, [P, P, P, P, C.c_size_t, P, P, P, P,
+                                                   C.c_uint64]),
             ("oracle_dump", None, [P, P, P]),
             ("oracle_stable_sort_perm", None, [P, C.c_size_t, P]),
             ("cpu_exec_create", P, [P, P, C.c_size_t, C.c_int]),
@@ -100,6 +102,23 @@ class Oracle:
     def run_batch(self, keys, vals, ops):
         """Sequential replay in input order."""
         return self._batch(self._lib.oracle_run_batch, keys, vals, ops)
+
+    def run_batch_scan(self, keys, vals, ops):
+        """Sequential replay with op 3 = SCAN(key, L = vals[i]).  Returns (values, found,
+        scan_keys, scan_vals); scan op i's pairs start at the sum of the L of earlier scans."""
+        k, o = _a(keys, np.uint64), _a(ops, np.uint8)
+        v = _a(vals, np.uint64)
+        cap = int(v[o == 3].sum()) if k.size else 0
+        out_v = np.zeros(k.size, dtype=np.uint64)
+        out_f = np.zeros(k.size, dtype=np.uint8)
+        sk = np.zeros(max(cap, 1), dtype=np.uint64)
+        sv = np.zeros(max(cap, 1), dtype=np.uint64)
+        used = self._lib.oracle_run_batch_scan(self._h, k.ctypes.data, v.ctypes.data,
+                                               o.ctypes.data, k.size, out_v.ctypes.data,
+                                               out_f.ctypes.data, sk.ctypes.data, sv.ctypes.data,
+                                               cap)
+        assert used == cap
+        return out_v, out_f, sk[:cap], sv[:cap]
 
     def pipeline_batch(self, keys, vals, ops):
         """The reference's sort / RLE / scan / mergeMission / writeback pipeline."""

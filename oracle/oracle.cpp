@@ -68,6 +68,44 @@ void oracle_run_batch(void* h, const uint64_t* keys, const uint64_t* vals, const
   }
 }
 
+// Sequential replay with range scans (no reference counterpart: the reference declares RANGE
+// but never implements it, src/tree.h:37-39, src/main.cpp:30-34).  Op 3 = SCAN(key, L = vals[i]):
+// the first L pairs with key' >= key in the state before op i, written at
+// scan_keys/scan_vals[off_i ...] with off_i = sum of the L of the scan ops before i;
+// out_vals[i] = pairs written, out_found[i] = (pairs > 0).  Other ops as oracle_run_batch.
+// Returns the sum of all L (the scan area the batch needs), or ~0 if it exceeds scan_cap.
+uint64_t oracle_run_batch_scan(void* h, const uint64_t* keys, const uint64_t* vals,
+                               const uint8_t* ops, size_t n, uint64_t* out_vals,
+                               uint8_t* out_found, uint64_t* scan_keys, uint64_t* scan_vals,
+                               uint64_t scan_cap) {
+  auto& m = static_cast<Oracle*>(h)->m;
+  uint64_t off = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (ops[i] == 3) {
+      const uint64_t L = vals[i];
+      if (off + L > scan_cap) return ~0ull;
+      uint64_t c = 0;
+      for (auto it = m.lower_bound(keys[i]); it != m.end() && c < L; ++it, ++c) {
+        scan_keys[off + c] = it->first;
+        scan_vals[off + c] = it->second;
+      }
+      out_vals[i] = c;
+      out_found[i] = c > 0 ? 1 : 0;
+      off += L;
+      continue;
+    }
+    auto it = m.find(keys[i]);
+    const bool found = it != m.end();
+    out_found[i] = found ? 1 : 0;
+    out_vals[i] = found ? it->second : 0;
+    if (ops[i] == OP_INSERT) {
+      if (found) it->second = vals[i];
+      else m.emplace(keys[i], vals[i]);
+    }
+  }
+  return off;
+}
+
 // Restatement of the reference's v38 pipeline on the oracle's map.
 void oracle_pipeline_batch(void* h, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
                            size_t n, uint64_t* out_vals, uint8_t* out_found) {

@@ -131,3 +131,64 @@ def test_cpu_executor_matches_oracle(oracle_mod, threads):
         np.testing.assert_array_equal(f, ef)
         np.testing.assert_array_equal(v, ev)
         assert sec > 0
+
+
+def py_replay_scan(state, keys, vals, ops):
+    """Brute force: the sorted key list is rebuilt for every scan."""
+    out_v = np.zeros(len(keys), dtype=np.uint64)
+    out_f = np.zeros(len(keys), dtype=np.uint8)
+    sk, sv = [], []
+    for i, (k, v, o) in enumerate(zip(keys.tolist(), vals.tolist(), ops.tolist())):
+        if o == 3:
+            got = [x for x in sorted(state) if x >= k][:v]
+            sk += got + [0] * (v - len(got))
+            sv += [state[x] for x in got] + [0] * (v - len(got))
+            out_v[i], out_f[i] = len(got), 1 if got else 0
+            continue
+        if k in state:
+            out_f[i], out_v[i] = 1, state[k]
+        if o == 0:
+            state[k] = v
+    return out_v, out_f, np.array(sk, dtype=np.uint64), np.array(sv, dtype=np.uint64)
+
+
+def test_scan_known_answers(oracle_mod):
+    o = oracle_mod.Oracle()
+    o.bulk_load(np.array([10, 20, 30], dtype=np.uint64), np.array([1, 2, 3], dtype=np.uint64))
+    keys = np.array([15, 25, 15, 40, 0, 31], dtype=np.uint64)
+    vals = np.array([2, 9, 3, 1, 5, 2], dtype=np.uint64)
+    ops = np.array([3, 0, 3, 3, 3, 3], dtype=np.uint8)
+    v, f, sk, sv = o.run_batch_scan(keys, vals, ops)
+    # scan(15,2) -> 20,30; insert 25=9; scan(15,3) -> 20,25,30; scan(40,1) -> none;
+    # scan(0,5) -> all four; scan(31,2) -> none
+    assert v.tolist() == [2, 0, 3, 0, 4, 0] and f.tolist() == [1, 0, 1, 0, 1, 0]
+    assert sk.size == 2 + 3 + 1 + 5 + 2
+    assert sk[:2].tolist() == [20, 30] and sv[:2].tolist() == [2, 3]
+    assert sk[2:5].tolist() == [20, 25, 30] and sv[2:5].tolist() == [2, 9, 3]
+    assert sk[6:10].tolist() == [10, 20, 25, 30] and sv[6:10].tolist() == [1, 2, 9, 3]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_scan_replay_matches_python(oracle_mod, seed):
+    rng = np.random.default_rng(100 + seed)
+    domain = rng.integers(0, 1 << 20, size=400, dtype=np.uint64)
+    load = rng.choice(domain, 150)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, load + 1)
+    state = {int(k): int(k) + 1 for k in load}
+    for _ in range(3):
+        n = 300
+        keys = rng.choice(domain, n)
+        ops = rng.choice(np.array([0, 1, 3], dtype=np.uint8), n, p=[0.3, 0.5, 0.2])
+        vals = rng.integers(0, 1 << 40, size=n, dtype=np.uint64)
+        vals[ops == 3] = rng.integers(1, 40, size=int((ops == 3).sum()))
+        ev, ef, esk, esv = py_replay_scan(state, keys, vals, ops)
+        v, f, sk, sv = o.run_batch_scan(keys, vals, ops)
+        np.testing.assert_array_equal(v, ev)
+        np.testing.assert_array_equal(f, ef)
+        # compare only the written part of every scan's slot range
+        off = np.concatenate([[0], np.cumsum(vals[ops == 3])])[:-1].astype(np.int64)
+        for j, i in enumerate(np.nonzero(ops == 3)[0]):
+            c, a = int(v[i]), int(off[j])
+            np.testing.assert_array_equal(sk[a:a + c], esk[a:a + c])
+            np.testing.assert_array_equal(sv[a:a + c], esv[a:a + c])
