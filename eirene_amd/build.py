@@ -30,6 +30,7 @@ HIP_SOURCES = [
     "stm_learned.hip",
     "stm_scan.hip",
     "stm_api.hip",
+    "stm_server.hip",
 ]
 CPP_SOURCES = [
     "stm_gen.cpp",

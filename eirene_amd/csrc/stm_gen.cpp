@@ -267,6 +267,36 @@ int stm_gen_zipf_batch(const uint64_t* loaded_sorted, size_t n_loaded, size_t n_
   return 0;
 }
 
+// One serving-loop request: n_ops ops, lookup_frac lookups (hits on loaded keys), insert_frac
+// inserts of fresh uniform keys, the rest range scans from a uniform start key with length
+// uniform in [scan_lo, scan_hi] (held in vals).  Request r of seed S draws from its own
+// streams, so requests can be generated in any order.
+int stm_gen_serving_request(const uint64_t* loaded_sorted, size_t n_loaded, uint64_t seed,
+                            uint64_t req, size_t n_ops, double lookup_frac, double insert_frac,
+                            uint32_t scan_lo, uint32_t scan_hi, uint64_t* keys, uint64_t* vals,
+                            uint8_t* ops) {
+  if (n_loaded == 0 || scan_lo == 0 || scan_hi < scan_lo) return 1;
+  const uint64_t rs = splitmix64(seed ^ (req * GOLDEN));
+  for (size_t i = 0; i < n_ops; ++i) {
+    const double u = unit(draw(rs, 40, i));
+    const uint64_t r = draw(rs, 41, i);
+    if (u <= lookup_frac) {
+      keys[i] = loaded_sorted[r % n_loaded];
+      vals[i] = 0;
+      ops[i] = 1;
+    } else if (u <= lookup_frac + insert_frac) {
+      keys[i] = r & (KEY_LIMIT - 1);
+      vals[i] = splitmix64(keys[i]);
+      ops[i] = 0;
+    } else {
+      keys[i] = r & (KEY_LIMIT - 1);
+      vals[i] = scan_lo + draw(rs, 42, i) % (uint64_t)(scan_hi - scan_lo + 1);
+      ops[i] = 3;
+    }
+  }
+  return 0;
+}
+
 // Batch sizes floor(2^(lo + (hi - lo) u)) for an uneven stream of batches.
 void stm_gen_batch_sizes(size_t count, uint64_t seed, double lo_exp, double hi_exp, uint64_t* out) {
   for (size_t i = 0; i < count; ++i) {

@@ -1,0 +1,599 @@
+// stm_server.hip — continuous-batching request loop over one device-resident tree.
+//
+// The reference runs one op file as a fixed sequence of batches, each copied in, run and
+// copied out before the next (GPU_STM_Tree::gpu_work_main,
+// src/gpu-stm-tree-v38/bpt-gpustm.cpp:77-135).  Here requests arrive at any time and are
+// admitted into the batch being formed as long as its share of the HBM budget holds them:
+//
+//   slots      the budget is split into `slots` equal batch slots (pinned host staging plus
+//              device buffers for keys, values, op codes, answers and scan pairs).  At any
+//              time one slot is forming, one is running on the tree, the others drain.
+//   admission  FIFO.  A request joins the forming slot if its ops and scan pairs fit;
+//              otherwise it and every later request wait until a slot frees up.  Admission
+//              copies the request into the slot's pinned buffers (the caller's thread).
+//   runner     when the tree is idle it closes the forming slot (the next free slot starts
+//              forming), copies it to HBM on its own stream, runs it as one batch (scan mode
+//              when it holds scans) and queues the answer copies on a third stream.
+//   completer  waits for a slot's answers, scatters them into the requests' buffers, marks
+//              the requests done and returns the slot.
+// Requests of one batch are concatenated in admission order, and batches run in order, so
+// every request sees exactly the state it would see if the requests ran one by one in
+// admission order: a result is the same alone and in the mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/stmtree.h"
+
+namespace {
+
+thread_local std::string g_server_error;
+
+int sfail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_server_error = buf;
+  return code;
+}
+
+double now_s() {
+  using clk = std::chrono::steady_clock;
+  return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+constexpr uint64_t BYTES_PER_OP = 8 + 8 + 1 + 8 + 1;  // key, value, op in; value, found out
+constexpr uint64_t BYTES_PER_PAIR = 16;
+
+struct Req {
+  uint64_t ticket = 0;
+  const uint64_t* keys = nullptr;
+  const uint64_t* vals = nullptr;
+  const uint8_t* ops = nullptr;
+  size_t n = 0;
+  uint64_t* out_vals = nullptr;
+  uint8_t* out_found = nullptr;
+  uint64_t* scan_keys = nullptr;
+  uint64_t* scan_vals = nullptr;
+  uint64_t pairs = 0;  // sum of its scan lengths
+  uint64_t scans = 0;
+  double t_arrival = 0, t_admit = 0, t_done = 0;
+  int status = STM_OK;
+  std::string error;
+  bool done = false;
+};
+
+struct Slot {
+  uint64_t cap_ops = 0, cap_pairs = 0;
+  uint64_t *h_keys = nullptr, *h_vals = nullptr, *h_ovals = nullptr, *h_skeys = nullptr,
+           *h_svals = nullptr;
+  uint8_t *h_ops = nullptr, *h_ofound = nullptr;
+  uint64_t *d_keys = nullptr, *d_vals = nullptr, *d_ovals = nullptr, *d_skeys = nullptr,
+           *d_svals = nullptr;
+  uint8_t *d_ops = nullptr, *d_ofound = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  std::vector<Req*> reqs;
+  uint64_t n = 0, pairs = 0, scans = 0;
+  int status = STM_OK;
+  std::string error;
+
+  void reset() {
+    reqs.clear();
+    n = pairs = scans = 0;
+    status = STM_OK;
+    error.clear();
+  }
+};
+
+}  // namespace
+
+struct stm_server {
+  stm_tree* tree = nullptr;
+  int device = 0;
+  stm_server_config cfg{};
+  uint64_t budget = 0;
+  hipStream_t s_in = nullptr, s_out = nullptr, s_tree = nullptr;
+  std::vector<std::unique_ptr<Slot>> slots;
+
+  std::mutex mu;
+  std::condition_variable cv_run;    // runner: a forming slot has requests, or stop
+  std::condition_variable cv_done;   // a request completed
+  std::condition_variable cv_drain;  // completer: a slot has answers queued, or stop
+  std::deque<Req*> pending;          // waiting for admission
+  std::deque<Slot*> free_slots;
+  std::deque<Slot*> draining;
+  Slot* forming = nullptr;
+  std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+  uint64_t next_ticket = 1;
+  uint64_t outstanding = 0;  // submitted, not yet done
+  bool stop = false;
+  std::thread runner, completer;
+
+  // statistics
+  uint64_t st_requests = 0, st_ops = 0, st_batches = 0, st_scans = 0, st_pairs = 0;
+  uint64_t st_max_pending = 0;
+  double st_busy = 0;
+
+  // Admission of queued requests into the forming slot (mu held).
+  void admit_locked() {
+    if (!forming && !free_slots.empty()) {
+      forming = free_slots.front();
+      free_slots.pop_front();
+    }
+    while (forming && !pending.empty()) {
+      Req* r = pending.front();
+      Slot* s = forming;
+      if (s->n + r->n > s->cap_ops || s->pairs + r->pairs > s->cap_pairs) break;
+      pending.pop_front();
+      std::memcpy(s->h_keys + s->n, r->keys, r->n * 8);
+      std::memcpy(s->h_vals + s->n, r->vals, r->n * 8);
+      std::memcpy(s->h_ops + s->n, r->ops, r->n);
+      s->n += r->n;
+      s->pairs += r->pairs;
+      s->scans += r->scans;
+      r->t_admit = now_s();
+      s->reqs.push_back(r);
+    }
+    if (forming && forming->n) cv_run.notify_one();
+  }
+
+  void run_loop() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      Slot* s = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_run.wait(lk, [&] { return stop || (forming && forming->n); });
+        if (!forming || !forming->n) {
+          if (stop) break;
+          continue;
+        }
+        s = forming;
+        forming = nullptr;
+        admit_locked();  // the next free slot starts forming
+      }
+      const double t0 = now_s();
+      hipError_t e = hipMemcpyAsync(s->d_keys, s->h_keys, s->n * 8, hipMemcpyHostToDevice, s_in);
+      if (e == hipSuccess) e = hipMemcpyAsync(s->d_vals, s->h_vals, s->n * 8, hipMemcpyHostToDevice, s_in);
+      if (e == hipSuccess) e = hipMemcpyAsync(s->d_ops, s->h_ops, s->n, hipMemcpyHostToDevice, s_in);
+      if (e == hipSuccess) e = hipEventRecord(s->ev_in, s_in);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s_tree, s->ev_in, 0);
+      int rc = e == hipSuccess ? STM_OK : sfail(STM_ERR_DEVICE, "request copy: %s", hipGetErrorString(e));
+      if (rc == STM_OK) {
+        rc = s->scans ? stm_run_batch_scan_device(tree, s->d_keys, s->d_vals, s->d_ops, s->n,
+                                                  s->d_ovals, s->d_ofound, s->d_skeys, s->d_svals,
+                                                  s->pairs)
+                      : stm_run_batch_device(tree, s->d_keys, s->d_vals, s->d_ops, s->n, s->d_ovals,
+                                             s->d_ofound);
+        if (rc != STM_OK) s->error = stm_last_error();
+      } else {
+        s->error = g_server_error;
+      }
+      s->status = rc;
+      const double t1 = now_s();
+      e = hipMemcpyAsync(s->h_ovals, s->d_ovals, s->n * 8, hipMemcpyDeviceToHost, s_out);
+      if (e == hipSuccess) e = hipMemcpyAsync(s->h_ofound, s->d_ofound, s->n, hipMemcpyDeviceToHost, s_out);
+      if (e == hipSuccess && s->pairs)
+        e = hipMemcpyAsync(s->h_skeys, s->d_skeys, s->pairs * 8, hipMemcpyDeviceToHost, s_out);
+      if (e == hipSuccess && s->pairs)
+        e = hipMemcpyAsync(s->h_svals, s->d_svals, s->pairs * 8, hipMemcpyDeviceToHost, s_out);
+      if (e == hipSuccess) e = hipEventRecord(s->ev_out, s_out);
+      if (e != hipSuccess && s->status == STM_OK) {
+        s->status = STM_ERR_DEVICE;
+        s->error = std::string("answer copy: ") + hipGetErrorString(e);
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        st_batches += 1;
+        st_busy += t1 - t0;
+        draining.push_back(s);
+      }
+      cv_drain.notify_one();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      draining.push_back(nullptr);  // end marker for the completer
+    }
+    cv_drain.notify_one();
+  }
+
+  void complete_loop() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      Slot* s = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_drain.wait(lk, [&] { return !draining.empty(); });
+        s = draining.front();
+        draining.pop_front();
+      }
+      if (!s) break;
+      const hipError_t e = hipEventSynchronize(s->ev_out);
+      if (e != hipSuccess && s->status == STM_OK) {
+        s->status = STM_ERR_DEVICE;
+        s->error = std::string("answer copy: ") + hipGetErrorString(e);
+      }
+      uint64_t off = 0, poff = 0;
+      for (Req* r : s->reqs) {
+        if (s->status == STM_OK) {
+          std::memcpy(r->out_vals, s->h_ovals + off, r->n * 8);
+          std::memcpy(r->out_found, s->h_ofound + off, r->n);
+          if (r->pairs) {
+            std::memcpy(r->scan_keys, s->h_skeys + poff, r->pairs * 8);
+            std::memcpy(r->scan_vals, s->h_svals + poff, r->pairs * 8);
+          }
+        }
+        off += r->n;
+        poff += r->pairs;
+      }
+      const double t = now_s();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Req* r : s->reqs) {
+          r->status = s->status;
+          r->error = s->error;
+          r->t_done = t;
+          r->done = true;
+          st_requests += 1;
+          st_ops += r->n;
+          st_scans += r->scans;
+          st_pairs += r->pairs;
+          outstanding -= 1;
+        }
+        s->reset();
+        free_slots.push_back(s);
+        admit_locked();
+      }
+      cv_done.notify_all();
+    }
+  }
+
+  void free_all() {
+    for (auto& sp : slots) {
+      Slot* s = sp.get();
+      void* hp[] = {s->h_keys, s->h_vals, s->h_ovals, s->h_skeys, s->h_svals, s->h_ops, s->h_ofound};
+      for (void* p : hp)
+        if (p) (void)hipHostFree(p);
+      void* dp[] = {s->d_keys, s->d_vals, s->d_ovals, s->d_skeys, s->d_svals, s->d_ops, s->d_ofound};
+      for (void* p : dp)
+        if (p) (void)hipFree(p);
+      if (s->ev_in) (void)hipEventDestroy(s->ev_in);
+      if (s->ev_out) (void)hipEventDestroy(s->ev_out);
+    }
+    slots.clear();
+    if (s_in) (void)hipStreamDestroy(s_in);
+    if (s_out) (void)hipStreamDestroy(s_out);
+    s_in = s_out = nullptr;
+  }
+};
+
+extern "C" {
+
+const char* stm_server_last_error(void) { return g_server_error.c_str(); }
+
+double stm_server_now(void) { return now_s(); }
+
+int stm_server_config_init(stm_server_config* cfg) {
+  if (!cfg) return sfail(STM_ERR_INVALID, "null config");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->hbm_budget = 0;
+  cfg->slots = 3;
+  cfg->max_batch_ops = 0;
+  cfg->scan_fraction = 0.5;
+  return STM_OK;
+}
+
+int stm_server_create(stm_tree* tree, const stm_server_config* cfg_in, stm_server** out) {
+  if (!tree || !out) return sfail(STM_ERR_INVALID, "null argument");
+  *out = nullptr;
+  stm_server_config cfg;
+  if (cfg_in) cfg = *cfg_in;
+  else stm_server_config_init(&cfg);
+  if (cfg.slots < 2 || cfg.slots > 8) return sfail(STM_ERR_INVALID, "slots %u not in [2, 8]", cfg.slots);
+  if (!(cfg.scan_fraction > 0 && cfg.scan_fraction < 1))
+    return sfail(STM_ERR_INVALID, "scan_fraction must lie in (0, 1)");
+  void* stream = nullptr;
+  if (stm_get_stream(tree, &stream) != STM_OK) return sfail(STM_ERR_INVALID, "tree stream");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return sfail(STM_ERR_DEVICE, "hipGetDevice");
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return sfail(STM_ERR_DEVICE, "hipMemGetInfo");
+  // Admission budget: the caller's, never above 90 % of the HBM free now.
+  uint64_t budget = cfg.hbm_budget ? cfg.hbm_budget : std::min<uint64_t>(free_b / 4, 8ull << 30);
+  budget = std::min<uint64_t>(budget, (uint64_t)(free_b * 0.9));
+  const uint64_t slot_bytes = budget / cfg.slots;
+  const uint64_t op_bytes = (uint64_t)(slot_bytes * (1.0 - cfg.scan_fraction));
+  uint64_t cap_ops = op_bytes / BYTES_PER_OP;
+  if (cfg.max_batch_ops) cap_ops = std::min<uint64_t>(cap_ops, cfg.max_batch_ops);
+  const uint64_t cap_pairs = (uint64_t)(slot_bytes * cfg.scan_fraction) / BYTES_PER_PAIR;
+  if (cap_ops < 1024 || cap_pairs < STM_SCAN_MAX_LEN)
+    return sfail(STM_ERR_NOMEM, "HBM budget of %llu bytes leaves slots too small (%llu ops, %llu pairs)",
+                 (unsigned long long)budget, (unsigned long long)cap_ops, (unsigned long long)cap_pairs);
+  auto sv = std::make_unique<stm_server>();
+  stm_server* s = sv.get();
+  s->tree = tree;
+  s->device = dev;
+  s->cfg = cfg;
+  s->budget = budget;
+  s->s_tree = (hipStream_t)stream;
+  if (hipStreamCreate(&s->s_in) != hipSuccess || hipStreamCreate(&s->s_out) != hipSuccess) {
+    s->free_all();
+    return sfail(STM_ERR_DEVICE, "hipStreamCreate");
+  }
+  for (uint32_t i = 0; i < cfg.slots; ++i) {
+    auto sl = std::make_unique<Slot>();
+    Slot* p = sl.get();
+    s->slots.push_back(std::move(sl));
+    p->cap_ops = cap_ops;
+    p->cap_pairs = cap_pairs;
+    bool ok = hipHostMalloc((void**)&p->h_keys, cap_ops * 8, 0) == hipSuccess &&
+              hipHostMalloc((void**)&p->h_vals, cap_ops * 8, 0) == hipSuccess &&
+              hipHostMalloc((void**)&p->h_ops, cap_ops, 0) == hipSuccess &&
+              hipHostMalloc((void**)&p->h_ovals, cap_ops * 8, 0) == hipSuccess &&
+              hipHostMalloc((void**)&p->h_ofound, cap_ops, 0) == hipSuccess &&
+              hipHostMalloc((void**)&p->h_skeys, cap_pairs * 8, 0) == hipSuccess &&
+              hipHostMalloc((void**)&p->h_svals, cap_pairs * 8, 0) == hipSuccess &&
+              hipMalloc((void**)&p->d_keys, cap_ops * 8) == hipSuccess &&
+              hipMalloc((void**)&p->d_vals, cap_ops * 8) == hipSuccess &&
+              hipMalloc((void**)&p->d_ops, cap_ops) == hipSuccess &&
+              hipMalloc((void**)&p->d_ovals, cap_ops * 8) == hipSuccess &&
+              hipMalloc((void**)&p->d_ofound, cap_ops) == hipSuccess &&
+              hipMalloc((void**)&p->d_skeys, cap_pairs * 8) == hipSuccess &&
+              hipMalloc((void**)&p->d_svals, cap_pairs * 8) == hipSuccess &&
+              hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      s->free_all();
+      return sfail(STM_ERR_NOMEM, "server slot %u (%llu ops, %llu scan pairs)", i,
+                   (unsigned long long)cap_ops, (unsigned long long)cap_pairs);
+    }
+    s->free_slots.push_back(p);
+  }
+  s->runner = std::thread([s] { s->run_loop(); });
+  s->completer = std::thread([s] { s->complete_loop(); });
+  *out = sv.release();
+  return STM_OK;
+}
+
+int stm_server_submit(stm_server* s, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+                      size_t n, uint64_t* out_vals, uint8_t* out_found, uint64_t* scan_keys,
+                      uint64_t* scan_vals, size_t scan_cap, double arrival_s, uint64_t* ticket) {
+  if (!s || !ticket) return sfail(STM_ERR_INVALID, "null argument");
+  *ticket = 0;
+  if (n == 0) return sfail(STM_ERR_INVALID, "empty request");
+  if (!keys || !vals || !ops || !out_vals || !out_found) return sfail(STM_ERR_INVALID, "null array");
+  uint64_t pairs = 0, scans = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (keys[i] == STM_KEY_RESERVED)
+      return sfail(STM_ERR_INVALID, "op %zu uses the reserved key 0xffffffffffffffff", i);
+    if (ops[i] > STM_OP_RANGE) return sfail(STM_ERR_INVALID, "op %zu has op code %u", i, ops[i]);
+    if (ops[i] == STM_OP_RANGE) {
+      if (vals[i] < 1 || vals[i] > STM_SCAN_MAX_LEN)
+        return sfail(STM_ERR_INVALID, "scan op %zu has length %llu, not in [1, %u]", i,
+                     (unsigned long long)vals[i], STM_SCAN_MAX_LEN);
+      pairs += vals[i];
+      scans += 1;
+    }
+  }
+  if (pairs > scan_cap || (pairs && (!scan_keys || !scan_vals)))
+    return sfail(STM_ERR_INVALID, "request scans need %llu pairs, its area holds %zu",
+                 (unsigned long long)pairs, scan_cap);
+  if (n > s->slots[0]->cap_ops || pairs > s->slots[0]->cap_pairs)
+    return sfail(STM_ERR_NOMEM, "request of %zu ops and %llu scan pairs exceeds a batch slot "
+                 "(%llu ops, %llu pairs); raise hbm_budget", n, (unsigned long long)pairs,
+                 (unsigned long long)s->slots[0]->cap_ops, (unsigned long long)s->slots[0]->cap_pairs);
+  auto r = std::make_unique<Req>();
+  r->keys = keys;
+  r->vals = vals;
+  r->ops = ops;
+  r->n = n;
+  r->out_vals = out_vals;
+  r->out_found = out_found;
+  r->scan_keys = scan_keys;
+  r->scan_vals = scan_vals;
+  r->pairs = pairs;
+  r->scans = scans;
+  r->t_arrival = arrival_s >= 0 ? arrival_s : now_s();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return sfail(STM_ERR_STATE, "server is shutting down");
+  r->ticket = s->next_ticket++;
+  *ticket = r->ticket;
+  Req* rp = r.get();
+  s->reqs.emplace(rp->ticket, std::move(r));
+  s->pending.push_back(rp);
+  s->outstanding += 1;
+  s->st_max_pending = std::max<uint64_t>(s->st_max_pending, s->pending.size());
+  s->admit_locked();
+  return STM_OK;
+}
+
+int stm_server_wait(stm_server* s, uint64_t ticket, double* latency_s) {
+  if (!s) return sfail(STM_ERR_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return sfail(STM_ERR_INVALID, "unknown ticket %llu", (unsigned long long)ticket);
+  Req* r = it->second.get();
+  s->cv_done.wait(lk, [&] { return r->done; });
+  if (latency_s) *latency_s = r->t_done - r->t_arrival;
+  const int rc = r->status;
+  if (rc != STM_OK) g_server_error = r->error;
+  s->reqs.erase(it);
+  return rc;
+}
+
+int stm_server_drain(stm_server* s) {
+  if (!s) return sfail(STM_ERR_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+  return STM_OK;
+}
+
+int stm_server_get_stats(stm_server* s, stm_server_stats* out) {
+  if (!s || !out) return sfail(STM_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(out, 0, sizeof(*out));
+  out->requests = s->st_requests;
+  out->ops = s->st_ops;
+  out->batches = s->st_batches;
+  out->scans = s->st_scans;
+  out->scan_pairs = s->st_pairs;
+  out->max_pending = s->st_max_pending;
+  out->busy_s = s->st_busy;
+  out->hbm_budget = s->budget;
+  out->slot_ops = s->slots.empty() ? 0 : s->slots[0]->cap_ops;
+  out->slot_pairs = s->slots.empty() ? 0 : s->slots[0]->cap_pairs;
+  return STM_OK;
+}
+
+int stm_server_destroy(stm_server* s) {
+  if (!s) return STM_OK;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+    s->stop = true;
+  }
+  s->cv_run.notify_all();
+  if (s->runner.joinable()) s->runner.join();
+  if (s->completer.joinable()) s->completer.join();
+  (void)hipSetDevice(s->device);
+  s->free_all();
+  delete s;
+  return STM_OK;
+}
+
+// Workload generators of stm_gen.cpp (exported by the same library).
+int stm_gen_serving_request(const uint64_t* loaded_sorted, size_t n_loaded, uint64_t seed,
+                            uint64_t req, size_t n_ops, double lookup_frac, double insert_frac,
+                            uint32_t scan_lo, uint32_t scan_hi, uint64_t* keys, uint64_t* vals,
+                            uint8_t* ops);
+void stm_gen_batch_sizes(size_t count, uint64_t seed, double lo_exp, double hi_exp, uint64_t* out);
+uint64_t stm_splitmix64(uint64_t x);
+
+int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_serve_workload* w,
+                        const uint64_t* loaded_sorted, size_t n_loaded, stm_serve_report* out,
+                        double* latencies_ms) {
+  if (!tree || !w || !out || !loaded_sorted || !n_loaded) return sfail(STM_ERR_INVALID, "null argument");
+  const size_t R = (size_t)w->requests;
+  if (R == 0 || !(w->offered_mops > 0)) return sfail(STM_ERR_INVALID, "no load to offer");
+  std::memset(out, 0, sizeof(*out));
+  // Request sizes, arrival times (Poisson at offered_mops), then the request data.
+  std::vector<uint64_t> sizes(R);
+  stm_gen_batch_sizes(R, w->seed, w->ops_lo_log2, w->ops_hi_log2, sizes.data());
+  std::vector<uint64_t> off(R + 1, 0);
+  for (size_t r = 0; r < R; ++r) off[r + 1] = off[r] + std::max<uint64_t>(sizes[r], 1);
+  const double mean_ops = (double)off[R] / (double)R;
+  const double lambda = w->offered_mops * 1e6 / mean_ops;  // requests per second
+  std::vector<double> arrival(R);
+  double t = 0;
+  for (size_t r = 0; r < R; ++r) {
+    const uint64_t h = stm_splitmix64(w->seed ^ stm_splitmix64(0xa441 + r));
+    const double u = ((h >> 11) + 1) * (1.0 / 9007199254740992.0);
+    t += -std::log(u) / lambda;
+    arrival[r] = t;
+  }
+  const uint64_t N = off[R];
+  std::vector<uint64_t> keys(N), vals(N), ovals(N);
+  std::vector<uint8_t> ops(N), ofound(N);
+  {
+    const unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    std::atomic<int> bad{0};
+    for (unsigned k = 0; k < T; ++k)
+      th.emplace_back([&, k] {
+        for (size_t r = k; r < R; r += T)
+          if (stm_gen_serving_request(loaded_sorted, n_loaded, w->seed, r, off[r + 1] - off[r],
+                                      w->lookup_frac, w->insert_frac, w->scan_lo, w->scan_hi,
+                                      keys.data() + off[r], vals.data() + off[r], ops.data() + off[r]))
+            bad = 1;
+      });
+    for (auto& x : th) x.join();
+    if (bad) return sfail(STM_ERR_INVALID, "serving workload parameters");
+  }
+  std::vector<uint64_t> poff(R + 1, 0);
+  for (size_t r = 0; r < R; ++r) {
+    uint64_t p = 0;
+    for (uint64_t i = off[r]; i < off[r + 1]; ++i)
+      if (ops[i] == STM_OP_RANGE) p += vals[i];
+    poff[r + 1] = poff[r] + p;
+  }
+  std::vector<uint64_t> skeys(std::max<uint64_t>(poff[R], 1)), svals(std::max<uint64_t>(poff[R], 1));
+
+  stm_server* srv = nullptr;
+  int rc = stm_server_create(tree, cfg, &srv);
+  if (rc) return rc;
+  std::vector<uint64_t> tickets(R, 0);
+  const double t0 = now_s() + 0.005;
+  for (size_t r = 0; r < R && rc == STM_OK; ++r) {
+    const double due = t0 + arrival[r];
+    for (double n = now_s(); n < due; n = now_s())
+      if (due - n > 300e-6) std::this_thread::sleep_for(std::chrono::microseconds(100));
+    rc = stm_server_submit(srv, keys.data() + off[r], vals.data() + off[r], ops.data() + off[r],
+                           off[r + 1] - off[r], ovals.data() + off[r], ofound.data() + off[r],
+                           skeys.data() + poff[r], svals.data() + poff[r], poff[r + 1] - poff[r], due,
+                           &tickets[r]);
+  }
+  const int submit_rc = rc;
+  std::vector<double> lat;
+  lat.reserve(R);
+  uint64_t failed = 0;
+  for (size_t r = 0; r < R; ++r) {
+    if (!tickets[r]) continue;
+    double l = 0;
+    if (stm_server_wait(srv, tickets[r], &l) != STM_OK) failed += 1;
+    lat.push_back(l * 1e3);
+    if (latencies_ms) latencies_ms[r] = l * 1e3;
+  }
+  const double t_end = now_s();
+  stm_server_stats st{};
+  stm_server_get_stats(srv, &st);
+  stm_server_destroy(srv);
+  if (submit_rc) return submit_rc;
+  std::vector<double> sorted_lat = lat;
+  std::sort(sorted_lat.begin(), sorted_lat.end());
+  auto pct = [&](double q) {
+    if (sorted_lat.empty()) return 0.0;
+    const size_t i = std::min(sorted_lat.size() - 1, (size_t)std::floor(q * (sorted_lat.size() - 1) + 0.5));
+    return sorted_lat[i];
+  };
+  double sum = 0;
+  for (double x : lat) sum += x;
+  out->requests = R;
+  out->ops = N;
+  out->batches = st.batches;
+  out->scans = st.scans;
+  out->scan_pairs = st.scan_pairs;
+  out->max_pending = st.max_pending;
+  out->failed = failed;
+  out->wall_s = t_end - t0;
+  out->mops = N / out->wall_s / 1e6;
+  out->offered_mops = R > 1 ? N / (arrival[R - 1] - arrival[0]) / 1e6 : 0;
+  out->p50_ms = pct(0.50);
+  out->p90_ms = pct(0.90);
+  out->p99_ms = pct(0.99);
+  out->p999_ms = pct(0.999);
+  out->max_ms = sorted_lat.empty() ? 0 : sorted_lat.back();
+  out->mean_ms = lat.empty() ? 0 : sum / lat.size();
+  out->busy_frac = st.busy_s / out->wall_s;
+  out->mean_batch_ops = st.batches ? (double)st.ops / st.batches : 0;
+  out->hbm_budget = st.hbm_budget;
+  out->slot_ops = st.slot_ops;
+  return failed ? sfail(STM_ERR_DEVICE, "%llu requests failed", (unsigned long long)failed) : STM_OK;
+}
+
+}  // extern "C"

@@ -213,6 +213,81 @@ int stm_route_info(stm_tree* tree, uint64_t* n_leaves, uint64_t* window_hits, ui
 int stm_mlp_gradients(const float* x, const float* t, size_t n, const float* params49,
                       float* grad49, float* loss);
 
+/* ---------------------------------------------------------------- serving loop
+ * Continuous batching over one device-resident tree (no reference counterpart; it generalises
+ * the fixed batch loop of GPU_STM_Tree::gpu_work_main, src/gpu-stm-tree-v38/bpt-gpustm.cpp:77-135).
+ * Requests (host arrays, mixed GET / INSERT / SCAN ops as in stm_run_batch_scan) are admitted
+ * FIFO into the batch being formed while it has room in its slot of the HBM budget; batches run
+ * one after another on the tree while the next forms and the previous drains, on separate
+ * streams.  A request's results equal running the requests one by one in admission
+ * (= submission) order.  The server owns the tree while it exists: call no other stm_* function
+ * on that tree until stm_server_destroy returns. */
+typedef struct stm_server stm_server;
+
+typedef struct stm_server_config {
+    uint64_t hbm_budget;   /* device bytes for batches in flight; 0 = a quarter of the free HBM,
+                              at most 8 GiB; never above 90 % of the free HBM */
+    uint32_t slots;        /* batch slots the budget is split into: forming, running, draining
+                              (2..8, default 3) */
+    uint32_t max_batch_ops;/* ops per batch at most (0 = what a slot holds) */
+    double scan_fraction;  /* share of a slot kept for scan pairs (default 0.5) */
+    uint32_t reserved[4];
+} stm_server_config;
+
+typedef struct stm_server_stats {
+    uint64_t requests, ops, batches, scans, scan_pairs;
+    uint64_t max_pending;  /* most requests waiting for admission at once */
+    double busy_s;         /* wall time the tree spent running batches */
+    uint64_t hbm_budget, slot_ops, slot_pairs;
+} stm_server_stats;
+
+int stm_server_config_init(stm_server_config* cfg);
+int stm_server_create(stm_tree* tree, const stm_server_config* cfg, stm_server** out);
+/* Queues one request; returns at once with a ticket.  The arrays must stay valid (inputs) and
+ * untouched (outputs) until stm_server_wait returns for the ticket.  scan_keys / scan_vals hold
+ * the request's scan pairs laid out as in stm_run_batch_scan.  arrival_s is the arrival time on
+ * the stm_server_now() clock used for the latency (< 0 = now). */
+int stm_server_submit(stm_server* srv, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
+                      size_t n, uint64_t* out_vals, uint8_t* out_found, uint64_t* scan_keys,
+                      uint64_t* scan_vals, size_t scan_cap, double arrival_s, uint64_t* ticket);
+/* Blocks until the request is done; *latency_s = completion - arrival.  Returns its status. */
+int stm_server_wait(stm_server* srv, uint64_t ticket, double* latency_s);
+int stm_server_drain(stm_server* srv);
+int stm_server_get_stats(stm_server* srv, stm_server_stats* out);
+/* Waits for every submitted request, then frees the server (not the tree). */
+int stm_server_destroy(stm_server* srv);
+double stm_server_now(void);
+const char* stm_server_last_error(void);
+
+/* Open-loop load of the serving loop: requests with Poisson arrivals at offered_mops, sizes
+ * floor(2^(lo + (hi - lo) u)) ops, lookup_frac lookups of loaded keys, insert_frac inserts of
+ * fresh keys, the rest scans of length uniform in [scan_lo, scan_hi].  The requests are generated
+ * before the clock starts; a client thread submits each at its arrival time. */
+typedef struct stm_serve_workload {
+    uint64_t requests;
+    double offered_mops;
+    uint32_t ops_lo_log2, ops_hi_log2;
+    double lookup_frac, insert_frac;
+    uint32_t scan_lo, scan_hi;
+    uint64_t seed;
+    uint32_t reserved[4];
+} stm_serve_workload;
+
+typedef struct stm_serve_report {
+    uint64_t requests, ops, batches, scans, scan_pairs, max_pending, failed;
+    double wall_s;          /* first arrival to last completion */
+    double mops;            /* ops / wall_s */
+    double offered_mops;    /* ops / (last arrival - first arrival) */
+    double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms;  /* request latency */
+    double busy_frac;       /* share of wall_s the tree was running a batch */
+    double mean_batch_ops;
+    uint64_t hbm_budget, slot_ops;
+} stm_serve_report;
+
+int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_serve_workload* w,
+                        const uint64_t* loaded_sorted, size_t n_loaded, stm_serve_report* out,
+                        double* latencies_ms);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* stm_last_error(void);
 
