@@ -110,6 +110,11 @@ struct Scratch {
   uint64_t *ukey = nullptr, *uval = nullptr;
   uint8_t* uop = nullptr;
   uint32_t *gstart = nullptr, *npend = nullptr, *dlist = nullptr;
+  uint32_t* uleaf = nullptr;  // leaf hint of every merged INSERT
+  uint32_t* slist = nullptr;  // merged INSERTs the fast path hands back
+  uint32_t* ilist = nullptr;  // merged INSERTs of the batch
+  uint32_t* upar = nullptr;   // level-1 node hint of every merged INSERT
+  uint64_t* uver = nullptr;   // version of the hinted leaf
   BatchCounters* counters = nullptr;
   // range scans
   uint64_t *soff = nullptr, *scan_tiles = nullptr;
@@ -134,6 +139,11 @@ struct Scratch {
     fold_tiles = ft;
     A_(ukey, cap); A_(uval, cap); A_(uop, cap); A_(gstart, cap); A_(npend, cap);
     A_(dlist, cap);
+    A_(uleaf, cap);
+    A_(slist, cap);
+    A_(ilist, cap);
+    A_(upar, cap);
+    A_(uver, cap);
     A_(counters, 1);
     if (staging) {
       A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
@@ -144,7 +154,7 @@ struct Scratch {
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
-                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, counters,
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, ilist, upar, uver, counters,
                   desc_table, desc_live, cursor, acc, soff, lastins, scan_tiles};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -208,6 +218,20 @@ void phase_collect(stm_tree* t) {
     }
     t->phase_used[p] = false;
   }
+}
+
+// Sum of the stripes of every device counter (the stream is synchronised first).
+int read_counters(stm_tree* t, uint64_t out[ST_NUM]) {
+  std::vector<unsigned long long> c(STATS_WORDS);
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  HIP_TRY(hipMemcpy(c.data(), t->view.stats, sizeof(unsigned long long) * STATS_WORDS,
+                    hipMemcpyDeviceToHost));
+  for (int k = 0; k < ST_NUM; ++k) {
+    uint64_t v = 0;
+    for (int j = 0; j < STAT_STRIPES; ++j) v += c[(size_t)k * STAT_STRIPES + j];
+    out[k] = v;
+  }
+  return STM_OK;
 }
 
 int read_header(stm_tree* t, TreeHeader* h) {
@@ -331,7 +355,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   hipStream_t st = t->stream;
   hipError_t e;
   if (timed) phase_begin(t, STM_PHASE_SORT);
-  e = radix_sort_pairs_desc(in.keys, in.desc, in.n, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, 0,
+  e = radix_sort_pairs_desc(in.keys, in.ops, in.desc, in.n, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, 0,
                             64, s.hist, (int)t->cfg.sort_tile_items, st);
   if (e != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_SORT);
@@ -354,6 +378,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   fa.counters = s.counters;
   fa.tile_scratch = s.fold_tiles;
   fa.lastins = in.scans ? s.lastins : nullptr;
+  fa.ilist = s.ilist;
   if ((e = fold_batch(fa, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_MERGE);
 
@@ -400,6 +425,9 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   la.out_vals = in.out_vals;
   la.out_found = in.out_found;
   la.desc = in.desc;
+  la.uleaf = s.uleaf;
+  la.uver = s.uver;
+  la.upar = s.upar;
   if ((e = launch_lookup(t->view, la, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_LOOKUP);
 
@@ -428,6 +456,12 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   ua.out_vals = in.out_vals;
   ua.out_found = in.out_found;
   ua.desc = in.desc;
+  ua.uleaf = s.uleaf;
+  ua.uver = s.uver;
+  ua.upar = s.upar;
+  ua.ilist = s.ilist;
+  ua.slist = s.slist;
+  ua.lanes = t->cfg.upsert_lanes;
   if ((e = launch_upsert(t->view, ua, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_UPSERT);
   if (in.scans && (e = launch_scan_answers(sa, st)) != hipSuccess) return e;
@@ -687,6 +721,9 @@ int build_from_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, s
       h.error = 0;
       h.n_keys = U;
       r2 = hipMemcpyAsync(t->view.hdr, &h, sizeof(h), hipMemcpyHostToDevice, st);
+      if (r2 == hipSuccess)  // the key count restarts from the built tree
+        r2 = hipMemsetAsync(t->view.stats + (size_t)ST_KEYS * STAT_STRIPES, 0,
+                            sizeof(unsigned long long) * STAT_STRIPES, st);
       if (r2 == hipSuccess) r2 = hipStreamSynchronize(st);
       t->route_valid = false;  // the router describes the previous tree
     }
@@ -723,6 +760,7 @@ int stm_config_init(stm_config* cfg) {
   cfg->train_steps = 300;
   cfg->sort_tile_items = 16;
   cfg->use_graphs = 1;
+  cfg->upsert_lanes = 16;
   return STM_OK;
 }
 
@@ -748,6 +786,10 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
   if (cfg.sort_tile_items == 0) cfg.sort_tile_items = 16;
   if (cfg.sort_tile_items != 8 && cfg.sort_tile_items != 16)
     return fail(STM_ERR_INVALID, "sort_tile_items %u not in {8, 16}", cfg.sort_tile_items);
+  if (cfg.upsert_lanes == 0) cfg.upsert_lanes = std::min<uint32_t>(16, cfg.fanout);
+  if (!(cfg.upsert_lanes == 8 || cfg.upsert_lanes == 16 || cfg.upsert_lanes == 32) ||
+      cfg.upsert_lanes > cfg.fanout)
+    return fail(STM_ERR_INVALID, "upsert_lanes %u not in {8, 16, 32} or above fanout", cfg.upsert_lanes);
   if (cfg.max_attempts == 0) cfg.max_attempts = 64;
   if (cfg.retrain_every == 0) cfg.retrain_every = 8;
   if (cfg.train_steps > 100000) return fail(STM_ERR_INVALID, "train_steps above 100000");
@@ -821,7 +863,10 @@ int stm_size(stm_tree* t, uint64_t* n_keys) {
   TreeHeader h;
   rc = read_header(t, &h);
   if (rc) return rc;
-  *n_keys = h.n_keys;
+  uint64_t c[ST_NUM];
+  rc = read_counters(t, c);
+  if (rc) return rc;
+  *n_keys = h.n_keys + c[ST_KEYS];
   return STM_OK;
 }
 
@@ -1136,10 +1181,11 @@ int stm_get_stats(stm_tree* t, stm_stats* out) {
   TreeHeader h;
   rc = read_header(t, &h);
   if (rc) return rc;
-  unsigned long long c[STATS_WORDS];
-  HIP_TRY(hipMemcpy(c, t->view.stats, sizeof(c), hipMemcpyDeviceToHost));
+  uint64_t c[ST_NUM];
+  rc = read_counters(t, c);
+  if (rc) return rc;
   *out = t->st;
-  out->n_keys = h.n_keys;
+  out->n_keys = h.n_keys + c[ST_KEYS];
   out->n_nodes = std::min<uint64_t>(h.n_nodes, t->capacity);
   out->capacity_nodes = t->capacity;
   out->height = h.height;
@@ -1150,12 +1196,8 @@ int stm_get_stats(stm_tree* t, stm_stats* out) {
   out->inner_splits = c[ST_INNER_SPLITS];
   out->root_splits = c[ST_ROOT_SPLITS];
   out->deferred = c[ST_DEFERRED];
-  out->route_hits = 0;
-  out->route_misses = 0;
-  for (int i = 0; i < ROUTE_STAT_SLOTS; ++i) {
-    out->route_hits += c[ST_NUM + i];
-    out->route_misses += c[ST_NUM + ROUTE_STAT_SLOTS + i];
-  }
+  out->route_hits = c[ST_ROUTE_HITS];
+  out->route_misses = c[ST_ROUTE_MISSES];
   return STM_OK;
 }
 
@@ -1222,7 +1264,7 @@ int stm_reset_stats(stm_tree* t) {
   int rc = set_device(t);
   if (rc) return rc;
   t->st = stm_stats{};
-  HIP_TRY(hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * STATS_WORDS, t->stream));
+  HIP_TRY(hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * STATS_RESET_WORDS, t->stream));
   HIP_TRY(hipStreamSynchronize(t->stream));
   return STM_OK;
 }

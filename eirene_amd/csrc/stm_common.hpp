@@ -45,7 +45,7 @@ struct alignas(64) TreeHeader {
   uint64_t n_nodes;   // bump allocator
   uint64_t capacity;  // node-pool capacity
   uint64_t error;     // sticky error bits (ERR_*) raised by kernels
-  uint64_t n_keys;    // keys in the tree (maintained by the batch pipeline)
+  uint64_t n_keys;    // keys of the last bulk build; inserts since then are counted in ST_KEYS
   uint64_t pad;
 };
 
@@ -55,7 +55,9 @@ constexpr uint64_t ERR_LIVELOCK = 2;      // the serial pass could not commit an
 constexpr uint64_t ERR_BAD_KEY = 4;       // a batch carried the reserved key
 constexpr uint64_t ERR_BAD_OP = 8;        // a batch carried an op code > 3
 
-// Device counters (u64 each) accumulated by kernels.
+// Device counters accumulated by kernels.  Each counter is striped over STAT_STRIPES words
+// (a wavefront adds to stripe wave_id mod STAT_STRIPES) and the host sums the stripes: one hot
+// address per counter would serialize a kernel of thousands of wavefronts on one L2 channel.
 enum StatSlot : int {
   ST_COMMITS = 0,
   ST_ABORTS,
@@ -63,15 +65,19 @@ enum StatSlot : int {
   ST_INNER_SPLITS,
   ST_ROOT_SPLITS,
   ST_DEFERRED,
-  ST_NEW_KEYS,
   ST_UPSERTS,
   ST_ROUTE_HITS,
   ST_ROUTE_MISSES,
+  ST_KEYS,  // keys added since the last bulk build (kept by stm_reset_stats); must stay last
   ST_NUM
 };
-// Route hit / miss counters are spread over this many slots each, after the ST_NUM slots.
-constexpr int ROUTE_STAT_SLOTS = 256;
-constexpr int STATS_WORDS = ST_NUM + 2 * ROUTE_STAT_SLOTS;
+constexpr int STAT_STRIPES = 256;
+constexpr int STATS_WORDS = ST_NUM * STAT_STRIPES;
+constexpr int STATS_RESET_WORDS = ST_KEYS * STAT_STRIPES;
+
+STM_DEV void stat_add(unsigned long long* stats, int slot, uint32_t stripe, uint64_t v) {
+  if (v) atomicAdd(stats + slot * STAT_STRIPES + (stripe & (STAT_STRIPES - 1)), (unsigned long long)v);
+}
 
 // Learned-routing directory entry r: the low fence and node id of the r-th leaf in key order
 // when the directory was built, and whether that leaf has split since (its live high fence
@@ -153,12 +159,6 @@ __host__ __device__ inline uint32_t meta_count(uint64_t info) { return (uint32_t
 __host__ __device__ inline uint32_t meta_level(uint64_t info) { return (uint32_t)(info >> 32); }
 __host__ __device__ inline uint64_t make_info(uint32_t count, uint32_t level) {
   return (uint64_t)count | ((uint64_t)level << 32);
-}
-
-// Wave-aggregated counter bump (one atomic per wave).
-STM_DEV void stat_add(unsigned long long* stats, int slot, uint64_t v) {
-  const uint64_t exec = __builtin_amdgcn_read_exec();
-  if (lane_id() == __builtin_ctzll(exec) && v) atomicAdd(stats + slot, (unsigned long long)v);
 }
 
 }  // namespace stm

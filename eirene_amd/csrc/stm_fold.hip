@@ -83,7 +83,7 @@ STM_DEV Seg block_exclusive_seg(const Seg& v, Seg* s_w /*[FOLD_WAVES]*/, Seg& to
 
 STM_DEV Seg seg_element(const FoldArgs& a, uint32_t i, uint64_t key, uint64_t prev_key) {
   const bool head = (i == 0) || key != prev_key;
-  const bool ins = a.ops ? (a.ops[a.sidx[i]] == OP_INSERT) : true;
+  const bool ins = op_of(a.sidx[i]) == OP_INSERT;  // op code packed by the sort
   Seg e;
   e.h = head ? 1 : 0;
   e.head = head ? (int32_t)i : -1;
@@ -152,18 +152,20 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, F
     counters->n_deferred = 0;
     counters->queue_head = 0;
     counters->n_bad = 0;
+    counters->n_slow = 0;
   }
 }
 
 __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg* tile_prefix) {
   __shared__ Seg s_w[FOLD_WAVES];
-  __shared__ uint32_t s_up;
-  if (threadIdx.x == 0) s_up = 0;
-  uint32_t n_up = 0;  // merged INSERTs found by this wavefront (lane 0 keeps the count)
+  __shared__ uint32_t s_ins[FOLD_TILE];  // the tile's merged INSERT groups
+  __shared__ uint32_t s_nins, s_base;
   fold_resolve(a);
   const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
   for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
   const uint32_t tile0 = b * FOLD_TILE;
+  if (threadIdx.x == 0) s_nins = 0;
+  __syncthreads();
   Seg carry = tile_prefix[b];  // state of every element before this round
   for (int r = 0; r < FOLD_ITEMS; ++r) {
     const uint32_t r0 = tile0 + r * FOLD_BLOCK;
@@ -186,8 +188,8 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
       const int32_t last_before = e.h ? -1 : before.last;  // latest earlier insert of this key
       if (a.lastins) a.lastins[i] = e.first >= 0 ? (int32_t)i : last_before;
       if (a.out_vals && last_before >= 0) {
-        const uint32_t pos = a.sidx[i];
-        a.out_vals[pos] = a.vals[a.sidx[last_before]];
+        const uint32_t pos = pos_of(a.sidx[i]);
+        a.out_vals[pos] = a.vals[pos_of(a.sidx[last_before])];
         a.out_found[pos] = 1;
       }
       const Seg x = seg_combine(before, e);
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
           atomicAdd(&a.counters->n_bad, (uint32_t)(i - x.head + 1));
         } else if (x.last >= 0) {
           op = OP_INSERT;
-          a.uval[g] = a.vals[a.sidx[x.last]];
+          a.uval[g] = a.vals[pos_of(a.sidx[x.last])];
           append = true;
         } else {
           op = OP_GET;
@@ -214,14 +216,20 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg
       }
     }
     const uint64_t want = ballot(append);
-    if (lane_id() == 0) n_up += (uint32_t)__popcll(want);
+    uint32_t wb = 0;
+    if (lane_id() == 0 && want) wb = atomicAdd(&s_nins, (uint32_t)__popcll(want));  // LDS
+    wb = (uint32_t)__shfl((int)wb, 0, WAVE);
+    if (append) s_ins[wb + (uint32_t)__popcll(want & lanemask_lt())] = g;
   }
-  }  // tiles
-  // One counter update per workgroup: a single hot atomic per wavefront round would
-  // serialize the whole kernel on one L2 channel.
-  if (lane_id() == 0 && n_up) atomicAdd(&s_up, n_up);
+  // One counter update per tile (a hot atomic per wavefront round would serialize the
+  // kernel on one L2 channel), then the tile's INSERT groups go to the batch's list.
   __syncthreads();
-  if (threadIdx.x == 0 && s_up) atomicAdd(&a.counters->n_upserts, s_up);
+  if (threadIdx.x == 0) s_base = s_nins ? atomicAdd(&a.counters->n_upserts, s_nins) : 0u;
+  __syncthreads();
+  if (a.ilist)
+    for (uint32_t k = threadIdx.x; k < s_nins; k += FOLD_BLOCK) a.ilist[s_base + k] = s_ins[k];
+  __syncthreads();
+  }  // tiles
 }
 
 size_t fold_scratch_bytes(uint64_t n) {

@@ -23,6 +23,14 @@ struct BatchDesc {
   uint64_t scan_cap;
 };
 
+// Sort payload of a batch element: its input position in bits 0..29 and its op code
+// (0..3, codes above 3 stored as GET) in bits 30..31, so the fold reads op codes from the
+// sorted payload instead of gathering them.  max_batch <= 2^30 keeps positions in range.
+constexpr uint32_t POS_BITS = 30;
+constexpr uint32_t POS_MASK = (1u << POS_BITS) - 1;
+__host__ __device__ inline uint32_t pos_of(uint32_t payload) { return payload & POS_MASK; }
+__host__ __device__ inline uint32_t op_of(uint32_t payload) { return payload >> POS_BITS; }
+
 // ------------------------------------------------------------------ radix sort (stm_sort.hip)
 // Words of scratch (u32) the sort needs for n elements.
 size_t sort_hist_words(uint64_t n);
@@ -32,10 +40,12 @@ hipError_t radix_sort_pairs(const uint64_t* keys_in, uint64_t* keys_tmp, uint64_
                             uint32_t* idx_tmp, uint32_t* idx_out, uint32_t n, int begin_bit,
                             int end_bit, uint32_t* hist_words, hipStream_t stream);
 // Same with a choice of tile (tile_items = 8 or 16 elements per thread: tiles of 2048 or 4096
-// keys).  With desc the keys and their count are read from the device-side descriptor
-// (keys_in unused) and n_max only bounds that count (grid and scratch sizing); without it
-// keys_in holds n_max keys.
-hipError_t radix_sort_pairs_desc(const uint64_t* keys_in, const BatchDesc* desc, uint32_t n_max,
+// keys) and op codes packed into the payload (pos | op << POS_BITS; ops_in null = all 0).
+// With desc the keys, op codes and their count are read from the device-side descriptor
+// (keys_in and ops_in unused) and n_max only bounds that count (grid and scratch sizing);
+// without it keys_in holds n_max keys.
+hipError_t radix_sort_pairs_desc(const uint64_t* keys_in, const uint8_t* ops_in,
+                                 const BatchDesc* desc, uint32_t n_max,
                                  uint64_t* keys_tmp, uint64_t* keys_out, uint32_t* idx_tmp,
                                  uint32_t* idx_out, int begin_bit, int end_bit, uint32_t* hist_words,
                                  int tile_items, hipStream_t stream);
@@ -50,7 +60,8 @@ struct BatchCounters {
   uint32_t n_bad;       // ops rejected (reserved key)
   uint32_t scan_bad;    // scan ops rejected (length or output area)
   uint64_t scan_total;  // sum of the scan lengths (pairs the output area must hold)
-  uint32_t pad[8];
+  uint32_t n_slow;      // merged INSERTs the fast path handed to the transactional kernel
+  uint32_t pad[7];
 };
 static_assert(sizeof(BatchCounters) == 64, "one 64-byte block");
 
@@ -73,6 +84,8 @@ struct FoldArgs {
   void* tile_scratch;     // fold_scratch_bytes(n)
   int32_t* lastins;       // scan mode: per sorted entry, the latest insert of its key at or
                           // before it (sorted index), -1 if none; null otherwise
+  uint32_t* ilist;        // optional: the groups whose merged op is INSERT (n_upserts of them,
+                          // key order within each fold tile)
 };
 constexpr uint8_t UOP_SKIP = 0xff;
 constexpr uint8_t OP_RANGE = 3;

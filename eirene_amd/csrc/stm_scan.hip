@@ -255,17 +255,17 @@ STM_DEV uint32_t run_scan(const TreeView& t, const ScanArgs& a, const ScanIn& in
         const uint32_t gs = a.gstart[g];
         const uint32_t fi = gs + max(a.npend[g], 1u) - 1;  // sorted index of the first insert
         const uint32_t ge = (g + 1 < U) ? a.gstart[g + 1] : in.n;
-        if (a.sidx[fi] < i) {
+        if (pos_of(a.sidx[fi]) < i) {
           uint32_t lo = fi, hi = ge;  // sidx[lo] < i; the last such entry lies in [lo, hi)
           while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (a.sidx[mid] < i) lo = mid;
+            if (pos_of(a.sidx[mid]) < i) lo = mid;
             else hi = mid;
           }
           const int32_t li = a.lastins[lo];
           if (li >= 0) {
             vis = true;
-            bval = in.vals[a.sidx[li]];
+            bval = in.vals[pos_of(a.sidx[li])];
           }
         }
       }

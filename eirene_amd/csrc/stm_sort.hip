@@ -122,6 +122,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scan_hist(uint32_t* __restrict__
 template <int ITEMS>
 __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
+    const uint8_t* __restrict__ ops_in,
     const BatchDesc* __restrict__ desc, uint32_t n_host, int first_pass, int shift,
     const uint32_t* __restrict__ hist, uint32_t stride, const uint32_t* __restrict__ totals,
     uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out) {
@@ -138,6 +139,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
   const SortRange rg = sort_range(desc, n_host, keys_in, first_pass);
   const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
   const uint64_t* kin = rg.keys;
+  const uint8_t* ops = (first_pass && desc) ? desc->ops : ops_in;  // read on the first pass only
   const int lane = lane_id();
   const int wave = threadIdx.x / WAVE;
   {
@@ -161,7 +163,12 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
       const bool valid = le < tile_n;
       const uint64_t e = tile0 + le;
       k[r] = valid ? kin[e] : KEY_EMPTY;
-      id[r] = valid ? (idx_in ? idx_in[e] : (uint32_t)e) : 0u;
+      uint32_t pl = (uint32_t)e;
+      if (valid && !idx_in && ops) {
+        const uint32_t o = ops[e];
+        pl |= (o > 3u ? 1u : o) << POS_BITS;
+      }
+      id[r] = valid ? (idx_in ? idx_in[e] : pl) : 0u;
       const uint32_t d = (uint32_t)(k[r] >> shift) & (RADIX - 1);
       // Lanes holding the same digit: intersect 8 bit-ballots.
       uint64_t peers = ballot(valid);
@@ -231,7 +238,8 @@ size_t sort_hist_words(uint64_t n) {
 }
 
 template <int ITEMS>
-static hipError_t sort_passes(const uint64_t* keys_in, const BatchDesc* desc, uint32_t n_host,
+static hipError_t sort_passes(const uint64_t* keys_in, const uint8_t* ops_in, const BatchDesc* desc,
+                              uint32_t n_host,
                               uint32_t n_max, uint64_t* keys_tmp, uint64_t* keys_out,
                               uint32_t* idx_tmp, uint32_t* idx_out, int begin_bit, int end_bit,
                               uint32_t* hist_words, hipStream_t stream) {
@@ -255,7 +263,8 @@ static hipError_t sort_passes(const uint64_t* keys_in, const BatchDesc* desc, ui
     const int first = p == 0 ? 1 : 0;
     k_hist<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, desc, n_host, first, shift, hist, stride);
     k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
-    k_scatter<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, desc, n_host, first, shift, hist, stride,
+    k_scatter<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, first ? ops_in : nullptr, desc,
+                                                      n_host, first, shift, hist, stride,
                                                       totals, kout, iout);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -265,25 +274,26 @@ static hipError_t sort_passes(const uint64_t* keys_in, const BatchDesc* desc, ui
   return hipSuccess;
 }
 
-hipError_t radix_sort_pairs_desc(const uint64_t* keys_in, const BatchDesc* desc, uint32_t n_max,
+hipError_t radix_sort_pairs_desc(const uint64_t* keys_in, const uint8_t* ops_in,
+                                 const BatchDesc* desc, uint32_t n_max,
                                  uint64_t* keys_tmp, uint64_t* keys_out, uint32_t* idx_tmp,
                                  uint32_t* idx_out, int begin_bit, int end_bit, uint32_t* hist_words,
                                  int tile_items, hipStream_t stream) {
   if (n_max == 0) return hipSuccess;
   const uint32_t n_host = desc ? 0u : n_max;
   if (tile_items == 8)
-    return sort_passes<8>(keys_in, desc, n_host, n_max, keys_tmp, keys_out, idx_tmp, idx_out,
-                          begin_bit, end_bit, hist_words, stream);
-  return sort_passes<16>(keys_in, desc, n_host, n_max, keys_tmp, keys_out, idx_tmp, idx_out,
-                         begin_bit, end_bit, hist_words, stream);
+    return sort_passes<8>(keys_in, ops_in, desc, n_host, n_max, keys_tmp, keys_out, idx_tmp,
+                          idx_out, begin_bit, end_bit, hist_words, stream);
+  return sort_passes<16>(keys_in, ops_in, desc, n_host, n_max, keys_tmp, keys_out, idx_tmp,
+                         idx_out, begin_bit, end_bit, hist_words, stream);
 }
 
 hipError_t radix_sort_pairs(const uint64_t* keys_in, uint64_t* keys_tmp, uint64_t* keys_out,
                             uint32_t* idx_tmp, uint32_t* idx_out, uint32_t n, int begin_bit,
                             int end_bit, uint32_t* hist_words, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  return sort_passes<16>(keys_in, nullptr, n, n, keys_tmp, keys_out, idx_tmp, idx_out, begin_bit,
-                         end_bit, hist_words, stream);
+  return sort_passes<16>(keys_in, nullptr, nullptr, n, n, keys_tmp, keys_out, idx_tmp, idx_out,
+                         begin_bit, end_bit, hist_words, stream);
 }
 
 }  // namespace stm

@@ -218,7 +218,10 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
                                                 const uint64_t* __restrict__ fences,
                                                 const uint32_t* __restrict__ lids,
                                                 const SegModel* __restrict__ segs,
-                                                const BatchDesc* __restrict__ desc) {
+                                                const BatchDesc* __restrict__ desc,
+                                                uint32_t* __restrict__ uleaf,
+                                                uint64_t* __restrict__ uver,
+                                                uint32_t* __restrict__ upar) {
   constexpr int K = F / G;
   const int lane = lane_id();
   const int g = lane & (G - 1);
@@ -226,7 +229,10 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   const uint64_t grp0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / G;
   const uint64_t ngroups = (uint64_t)gridDim.x * blockDim.x / G;
   const uint32_t U = cnt->n_unique;
-  if (grp0 * chunk >= U) return;
+  // Every lane group walks one contiguous span of at least `chunk` sorted groups, so its
+  // cached leaf, parent and directory cursor carry over for the whole span.
+  const uint64_t span = max<uint64_t>(chunk, (U + ngroups - 1) / ngroups);
+  if (grp0 * span >= U) return;
   if (desc) {
     out_vals = desc->out_vals;
     out_found = desc->out_found;
@@ -246,11 +252,9 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   uint32_t hits = 0, misses = 0;
   if constexpr (LEARNED) rv = route_view(rh, fences, lids, segs, t.dir);
 
-  // Grid-stride over chunks: a lane group's later chunks hold larger keys, so its cached
-  // leaf, parent and directory cursor stay usable.
-  for (uint64_t grp = grp0; grp * chunk < U; grp += ngroups) {
-  const uint32_t i0 = (uint32_t)(grp * chunk);
-  const uint32_t i1 = (uint32_t)min<uint64_t>((uint64_t)i0 + chunk, U);
+  {
+  const uint32_t i0 = (uint32_t)(grp0 * span);
+  const uint32_t i1 = (uint32_t)min<uint64_t>((uint64_t)i0 + span, U);
   for (uint32_t base = i0; base < i1; base += G) {
     // Lane g owns op base + g: its metadata comes in one coalesced round trip, and the
     // sorted position of its first waiting op in a second one that overlaps the walk.
@@ -261,12 +265,15 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
     const uint32_t gs_l = valid ? gstart[j] : 0;
     const uint32_t np_l = valid ? npend[j] : 0;
     const bool mine = op_l == OP_GET;
-    const uint32_t pos_l = (mine && np_l) ? sidx[gs_l] : 0;
+    // Merged INSERTs are routed too: their leaf is the upsert kernel's starting hint.
+    const bool walk = mine || (uleaf && op_l == OP_INSERT);
+    const uint32_t pos_l = (mine && np_l) ? pos_of(sidx[gs_l]) : 0;
     uint64_t vidx_l = 0;  // value slot of op base + g
+    uint32_t leaf_l = NODE_NONE, par_l = NODE_NONE;
     bool found_l = false;
     const uint32_t n_sub = min(i1 - base, (uint32_t)G);
     for (uint32_t jj = 0; jj < n_sub; ++jj) {
-      if (__shfl((int)mine, gbase + (int)jj, WAVE) == 0) continue;
+      if (__shfl((int)walk, gbase + (int)jj, WAVE) == 0) continue;
       const uint64_t key = shfl64(key_l, gbase + (int)jj);
       if (leaf == NODE_NONE || key >= leaf_hi) {
         bool routed = false;
@@ -337,7 +344,16 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
       if (g == (int)jj) {
         found_l = s >= 0 && at == key;
         vidx_l = (uint64_t)leaf * F + (uint32_t)max(s, 0);
+        leaf_l = leaf;
+        par_l = par;
       }
+    }
+    if (walk && !mine) {
+      // Hints for the upsert kernels: the leaf, its version now (no writer runs during this
+      // kernel) and the level-1 node last used for routing.
+      uleaf[j] = leaf_l;
+      uver[j] = leaf_l < t.capacity ? t.meta[leaf_l].lock : 1ull;
+      upar[j] = par_l;
     }
     // All values of the sub-chunk in one parallel round trip, then the answers.
     if (mine) {
@@ -347,19 +363,17 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
         out_found[pos_l] = found_l ? 1 : 0;
       }
       for (uint32_t p = 1; p < np_l; ++p) {
-        const uint32_t pos = sidx[gs_l + p];
+        const uint32_t pos = pos_of(sidx[gs_l + p]);
         out_vals[pos] = val;
         out_found[pos] = found_l ? 1 : 0;
       }
     }
   }
-  }  // chunks
+  }  // span
   if constexpr (LEARNED) {
-    // Spread over ROUTE_STAT_SLOTS counters so no single address serializes the kernel.
     if (g == 0) {
-      const uint32_t slot = (uint32_t)(grp0 % ROUTE_STAT_SLOTS);
-      if (hits) atomicAdd(t.stats + ST_NUM + slot, (unsigned long long)hits);
-      if (misses) atomicAdd(t.stats + ST_NUM + ROUTE_STAT_SLOTS + slot, (unsigned long long)misses);
+      stat_add(t.stats, ST_ROUTE_HITS, (uint32_t)grp0, hits);
+      stat_add(t.stats, ST_ROUTE_MISSES, (uint32_t)grp0, misses);
     }
   }
 }
@@ -787,17 +801,26 @@ template <int F>
 struct ParentCache {
   WNode<F> n;
   bool valid = false;
+  bool fresh = false;  // read (version, then content) during the current attempt
 };
 
 // One attempt of the upsert transaction for (key, val).
 template <int F>
 STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val, bool& found,
-                                   uint64_t& old, TxCounters& c, ParentCache<F>& pc) {
+                                   uint64_t& old, TxCounters& c, ParentCache<F>& pc,
+                                   uint32_t par_hint) {
   WNode<F> par, cur;
   bool has_par = false;
   bool par_fresh = false;  // par was read with its version during this attempt
   uint64_t hver = 0;
   uint32_t leaf_id;
+  pc.fresh = false;
+  if (!(pc.valid && key >= pc.n.low && key < pc.n.high) && par_hint < t.capacity) {
+    // The level-1 node the lookup kernel routed this key through: one read instead of a
+    // descent from the root.  Its fences and level check it; the leaf's fences check the route.
+    pc.valid = tx_read<F>(t, par_hint, pc.n) && pc.n.level == 1 && key >= pc.n.low && key < pc.n.high;
+    pc.fresh = pc.valid;
+  }
   if (pc.valid && key >= pc.n.low && key < pc.n.high) {
     // Route through the cached level-1 node; the tree never loses a level, so no header
     // read is needed.  The leaf's fences validate the route.
@@ -809,6 +832,7 @@ STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val
     leaf_id = (uint32_t)wave_at<F>(pc.n.v, cle - 1);
     par = pc.n;
     has_par = true;
+    par_fresh = pc.fresh;
   } else {
     hver = uniform64(ld_relaxed(&t.hdr->lock));
     if (hver & 1) return TX_ABORT;
@@ -873,12 +897,14 @@ template <int F>
 STM_DEV bool run_upsert(const TreeView& t, uint64_t key, uint64_t val, uint32_t gs, uint32_t np,
                         const uint32_t* __restrict__ sidx, uint64_t* __restrict__ out_vals,
                         uint8_t* __restrict__ out_found, uint32_t max_attempts, TxCounters& c,
-                        ParentCache<F>& pc) {
+                        ParentCache<F>& pc, uint32_t par_hint = NODE_NONE) {
   bool found = false;
   uint64_t old = 0;
   uint32_t aborts = 0, restarts = 0;
   for (;;) {
-    const TxResult r = tx_upsert_attempt<F>(t, key, val, found, old, c, pc);
+    // The hint is only tried on the first attempt; retries descend from the root.
+    const TxResult r = tx_upsert_attempt<F>(t, key, val, found, old, c, pc,
+                                            aborts + restarts == 0 ? par_hint : NODE_NONE);
     if (r == TX_DONE) break;
     if (r == TX_FAIL) return true;  // pool exhausted: error bit raised, op dropped
     if (r == TX_ABORT) {
@@ -891,7 +917,7 @@ STM_DEV bool run_upsert(const TreeView& t, uint64_t key, uint64_t val, uint32_t 
   }
   if (out_vals) {
     for (uint32_t p = lane_id(); p < np; p += WAVE) {
-      const uint32_t pos = sidx[gs + p];
+      const uint32_t pos = pos_of(sidx[gs + p]);
       out_vals[pos] = old;
       out_found[pos] = found ? 1 : 0;
     }
@@ -899,23 +925,224 @@ STM_DEV bool run_upsert(const TreeView& t, uint64_t key, uint64_t val, uint32_t 
   return true;
 }
 
+STM_DEV uint32_t wave_stripe() { return blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; }
+
 STM_DEV void flush_counters(const TreeView& t, const TxCounters& c) {
   if (lane_id() == 0) {
-    if (c.commits) atomicAdd(t.stats + ST_COMMITS, (unsigned long long)c.commits);
-    if (c.aborts) atomicAdd(t.stats + ST_ABORTS, (unsigned long long)c.aborts);
-    if (c.leaf_splits) atomicAdd(t.stats + ST_LEAF_SPLITS, (unsigned long long)c.leaf_splits);
-    if (c.inner_splits) atomicAdd(t.stats + ST_INNER_SPLITS, (unsigned long long)c.inner_splits);
-    if (c.root_splits) atomicAdd(t.stats + ST_ROOT_SPLITS, (unsigned long long)c.root_splits);
-    if (c.new_keys) {
-      atomicAdd(t.stats + ST_NEW_KEYS, (unsigned long long)c.new_keys);
-      atomicAdd((unsigned long long*)&t.hdr->n_keys, (unsigned long long)c.new_keys);
-    }
+    const uint32_t w = wave_stripe();
+    stat_add(t.stats, ST_COMMITS, w, c.commits);
+    stat_add(t.stats, ST_ABORTS, w, c.aborts);
+    stat_add(t.stats, ST_LEAF_SPLITS, w, c.leaf_splits);
+    stat_add(t.stats, ST_INNER_SPLITS, w, c.inner_splits);
+    stat_add(t.stats, ST_ROOT_SPLITS, w, c.root_splits);
+    stat_add(t.stats, ST_KEYS, w, c.new_keys);
   }
 }
 
-// Each wavefront owns a contiguous range of the batch's key-sorted groups and runs the
-// merged INSERTs in it in key order, so consecutive upserts mostly reuse the cached
-// parent and different wavefronts work on disjoint key ranges.
+// Lane-group fast path: NG = 64 / G merged INSERTs of one wavefront at once, each by G lanes
+// holding its leaf strided (slot e*G + g in element e of lane g, so every load and store
+// instruction covers G consecutive slots).  It is the leaf-only transaction of the wave-wide
+// path (test-and-set of the leaf's lock word, read, update in place or shift-insert, stores
+// acknowledged, unlock with version + 2) run for NG ops in lock-step, started from the leaf the
+// lookup kernel routed the key to.  It never waits for a lock and never splits: an op whose
+// leaf is locked, whose hint is stale (the leaf split since: its fences no longer hold the key)
+// or whose leaf is full is handed back to the wave-wide path.
+// src = round lane of this group's op (-1: none).  Returns the round lanes whose op committed.
+template <int F, int G>
+STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uint64_t v_l,
+                                   uint32_t leaf_l, uint64_t ver_l, uint32_t gs_l, uint32_t np_l,
+                                   const uint32_t* __restrict__ sidx,
+                                   uint64_t* __restrict__ out_vals,
+                                   uint8_t* __restrict__ out_found, TxCounters& c) {
+  constexpr int K = F / G;
+  constexpr int NG = WAVE / G;
+  const int lane = lane_id();
+  const int g = lane & (G - 1);
+  const int gbase = lane - g;
+  const int sl = src >= 0 ? src : 0;
+  const uint64_t key = shfl64(k_l, sl);
+  const uint64_t val = shfl64(v_l, sl);
+  const uint32_t leaf = (uint32_t)__shfl((int)leaf_l, sl, WAVE);
+  const uint32_t gs = (uint32_t)__shfl((int)gs_l, sl, WAVE);
+  const uint32_t np = (uint32_t)__shfl((int)np_l, sl, WAVE);
+  const uint64_t ver = shfl64(ver_l, sl);
+  const bool act = src >= 0 && leaf < t.capacity;
+  NodeMeta* m = t.meta + (act ? leaf : 0u);
+  uint64_t* kp = t.keys + (size_t)leaf * F + g;
+  uint64_t* vp = t.vals + (size_t)leaf * F + g;
+  uint64_t lk[K], lv[K];
+  uint64_t info = 0, low = 0, high = 0, word = 0;
+  // Test-and-set and the content loads go out together.  The content is the leaf's if the
+  // test-and-set returns exactly the version the lookup kernel saw (unlocked): every commit
+  // changes the version and an aborted writer writes nothing, so no writer touched the leaf
+  // since this kernel started.  Otherwise, with the lock held, it is read again.
+  if (act) {
+    if (g == 0)
+      word = __hip_atomic_fetch_or(&m->lock, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    info = ld_relaxed(&m->info);
+    low = ld_relaxed(&m->low);
+    high = ld_relaxed(&m->high);
+#pragma unroll
+    for (int e = 0; e < K; ++e) {
+      lk[e] = ld_relaxed(kp + e * G);
+      lv[e] = ld_relaxed(vp + e * G);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < K; ++e) {
+      lk[e] = KEY_EMPTY;
+      lv[e] = 0;
+    }
+  }
+  word = shfl64(word, gbase);
+  const bool held = act && !(word & 1ull);
+  if (ballot(held && word != ver)) {
+    wave_order();  // the reload issues only after the lock is held
+    if (held && word != ver) {
+      info = ld_relaxed(&m->info);
+      low = ld_relaxed(&m->low);
+      high = ld_relaxed(&m->high);
+#pragma unroll
+      for (int e = 0; e < K; ++e) {
+        lk[e] = ld_relaxed(kp + e * G);
+        lv[e] = ld_relaxed(vp + e * G);
+      }
+    }
+  }
+  if (!held) {
+#pragma unroll
+    for (int e = 0; e < K; ++e) {
+      lk[e] = KEY_EMPTY;
+      lv[e] = 0;
+    }
+    info = 0;
+  }
+  const uint32_t count = meta_count(info);
+  const bool fits = held && meta_level(info) == 0 && count <= (uint32_t)F && key >= low && key < high;
+  uint32_t le = 0;
+#pragma unroll
+  for (int e = 0; e < K; ++e) le += lk[e] <= key ? 1u : 0u;
+  const int cle = (int)group_sum<G>(le);
+  const int s = cle > 0 ? cle - 1 : 0;
+  const uint64_t at_k = shfl64(select_k<K>(lk, s / G), gbase + s % G);
+  const uint64_t at_v = shfl64(select_k<K>(lv, s / G), gbase + s % G);
+  const bool found = fits && cle > 0 && at_k == key;
+  const bool insert = fits && !found && count < (uint32_t)F;
+  const bool commit = found || insert;
+  const uint64_t old = found ? at_v : 0;
+  if (found) {
+    if (g == s % G) st_relaxed(t.vals + (size_t)leaf * F + s, val);
+  }
+  // Shift insert at pos = cle: slot x takes old[x] below pos, the new pair at pos and
+  // old[x - 1] above; only slots pos .. count change.  old[x - 1] is element e of lane g - 1,
+  // or for g = 0 element e - 1 of lane G - 1.
+  const int pos = cle;
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    const uint64_t pk_in = shfl64(lk[e], g > 0 ? lane - 1 : lane);
+    const uint64_t pv_in = shfl64(lv[e], g > 0 ? lane - 1 : lane);
+    const uint64_t pk_wr = shfl64(lk[e > 0 ? e - 1 : 0], gbase + G - 1);
+    const uint64_t pv_wr = shfl64(lv[e > 0 ? e - 1 : 0], gbase + G - 1);
+    const int x = e * G + g;
+    if (insert && x >= pos && x <= (int)count) {
+      const uint64_t ok_ = g > 0 ? pk_in : pk_wr;
+      const uint64_t ov_ = g > 0 ? pv_in : pv_wr;
+      st_relaxed(kp + e * G, x == pos ? key : ok_);
+      st_relaxed(vp + e * G, x == pos ? val : ov_);
+    }
+  }
+  if (insert && g == 0) st_relaxed(&m->info, make_info(count + 1, 0));
+  wave_order();  // every content store is acknowledged before the new versions
+  if (held && g == 0) st_relaxed(&m->lock, commit ? word + 2 : word);
+  if (commit) {
+    for (uint32_t p = (uint32_t)g; p < np; p += G) {
+      const uint32_t pos_o = pos_of(sidx[gs + p]);
+      out_vals[pos_o] = old;
+      out_found[pos_o] = found ? 1 : 0;
+    }
+  }
+  const uint64_t leaders = ballot(commit && g == 0);
+  const uint64_t news = ballot(insert && g == 0);
+  c.commits += (uint32_t)__popcll(leaders);
+  c.new_keys += (uint32_t)__popcll(news);
+  uint64_t done = 0;
+#pragma unroll
+  for (int q = 0; q < NG; ++q) {
+    const int sq = __builtin_amdgcn_readlane(src, q * G);
+    if ((leaders >> (q * G)) & 1ull) done |= 1ull << sq;
+  }
+  return done;
+}
+
+// Fast-path kernel over the batch's merged INSERTs (ilist, n_upserts entries): each wavefront
+// owns a contiguous part and runs it NG ops at a time on the lane-group fast path; the ops it
+// hands back are appended to slist (one atomic per wavefront round that has any) for k_upsert.
+template <int F, int G>
+__global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t* __restrict__ ukey,
+                                                     const uint64_t* __restrict__ uval,
+                                                     const uint32_t* __restrict__ gstart,
+                                                     const uint32_t* __restrict__ npend,
+                                                     const uint32_t* __restrict__ sidx,
+                                                     const uint32_t* __restrict__ ilist,
+                                                     const uint32_t* __restrict__ uleaf,
+                                                     const uint64_t* __restrict__ uver,
+                                                     BatchCounters* cnt, uint32_t* __restrict__ slist,
+                                                     uint64_t* __restrict__ out_vals,
+                                                     uint8_t* __restrict__ out_found,
+                                                     const BatchDesc* __restrict__ desc) {
+  constexpr int NG = WAVE / G;
+  const uint32_t N = cnt->n_upserts;
+  if (desc) {
+    out_vals = desc->out_vals;
+    out_found = desc->out_found;
+  }
+  const uint32_t nw = gridDim.x * (blockDim.x / WAVE);
+  const uint32_t w = uniform32((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const uint32_t per = (N + nw - 1) / nw;
+  const uint32_t e0 = min(w * per, N);
+  const uint32_t e1 = min(e0 + per, N);
+  TxCounters c;
+  uint32_t upserts = 0;
+  for (uint32_t base = e0; base < e1; base += WAVE) {
+    const uint32_t e = base + lane_id();
+    const bool mine = e < e1;
+    const uint32_t gi = mine ? ilist[e] : 0;
+    uint64_t todo = ballot(mine);
+    const uint64_t k_l = mine ? ukey[gi] : 0;
+    const uint64_t v_l = mine ? uval[gi] : 0;
+    const uint32_t gs_l = mine ? gstart[gi] : 0;
+    const uint32_t np_l = mine ? npend[gi] : 0;
+    const uint32_t leaf_l = mine ? uleaf[gi] : NODE_NONE;
+    const uint64_t ver_l = mine ? uver[gi] : 1ull;
+    for (uint64_t rest = todo; rest;) {
+      // group q takes the q-th remaining op of the round
+      int my_src = -1;
+#pragma unroll
+      for (int q = 0; q < NG; ++q) {
+        const int l = rest ? __builtin_ctzll(rest) : -1;
+        if (l >= 0) rest &= rest - 1;
+        if (lane_id() / G == q) my_src = l;
+      }
+      const uint64_t done = fast_leaf_upserts<F, G>(t, my_src, k_l, v_l, leaf_l, ver_l, gs_l,
+                                                    np_l, sidx, out_vals, out_found, c);
+      upserts += (uint32_t)__popcll(done);
+      todo &= ~done;
+    }
+    if (todo) {  // handed back
+      uint32_t at = 0;
+      if (lane_id() == 0) at = atomicAdd(&cnt->n_slow, (uint32_t)__popcll(todo));
+      at = (uint32_t)__shfl((int)at, 0, WAVE);
+      if ((todo >> lane_id()) & 1ull) slist[at + (uint32_t)__popcll(todo & lanemask_lt())] = gi;
+    }
+  }
+  flush_counters(t, c);
+  if (lane_id() == 0) stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
+}
+
+// Wave-wide transactions for the ops the fast path handed back (slist[0 .. n_slow)), or for
+// every merged INSERT of the batch when slist is null.  Each wavefront owns a contiguous part
+// of that sequence (key order within a part), so consecutive upserts mostly reuse the cached
+// parent.
 template <int F>
 __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __restrict__ ukey,
                                                 const uint64_t* __restrict__ uval,
@@ -923,12 +1150,15 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
                                                 const uint32_t* __restrict__ gstart,
                                                 const uint32_t* __restrict__ npend,
                                                 const uint32_t* __restrict__ sidx,
+                                                const uint32_t* __restrict__ slist,
+                                                const uint32_t* __restrict__ upar,
                                                 BatchCounters* cnt, uint32_t* __restrict__ dlist,
                                                 uint32_t max_attempts,
                                                 uint64_t* __restrict__ out_vals,
                                                 uint8_t* __restrict__ out_found,
                                                 const BatchDesc* __restrict__ desc) {
-  const uint32_t U = cnt->n_unique;
+  const uint32_t U = slist ? cnt->n_slow : cnt->n_unique;
+  if (U == 0) return;
   if (desc) {
     out_vals = desc->out_vals;
     out_found = desc->out_found;
@@ -942,14 +1172,16 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
   ParentCache<F> pc;
   uint32_t upserts = 0, deferred = 0;
   for (uint32_t base = g0; base < g1; base += WAVE) {
-    // Op data of the round's upserts, loaded together (coalesced) and read lane by lane.
-    const uint32_t gi = base + lane_id();
-    const bool mine = gi < g1 && uop[gi] == OP_INSERT;
+    // Op data of the round's upserts, loaded together and read lane by lane.
+    const uint32_t e = base + lane_id();
+    const uint32_t gi = e < g1 ? (slist ? slist[e] : e) : 0;
+    const bool mine = e < g1 && (slist || uop[gi] == OP_INSERT);
     uint64_t todo = ballot(mine);
     const uint64_t k_l = mine ? ukey[gi] : 0;
     const uint64_t v_l = mine ? uval[gi] : 0;
     const uint32_t gs_l = mine ? gstart[gi] : 0;
     const uint32_t np_l = mine ? npend[gi] : 0;
+    const uint32_t par_l = (mine && upar) ? upar[gi] : NODE_NONE;
     while (todo) {
       const int l = __builtin_ctzll(todo);
       todo &= todo - 1;
@@ -957,20 +1189,22 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
       const uint64_t val = readlane64(v_l, l);
       const uint32_t gs = __builtin_amdgcn_readlane(gs_l, l);
       const uint32_t np = __builtin_amdgcn_readlane(np_l, l);
+      const uint32_t g = __builtin_amdgcn_readlane(gi, l);
+      const uint32_t ph = __builtin_amdgcn_readlane(par_l, l);
       upserts++;
-      if (!run_upsert<F>(t, key, val, gs, np, sidx, out_vals, out_found, max_attempts, c, pc)) {
+      if (!run_upsert<F>(t, key, val, gs, np, sidx, out_vals, out_found, max_attempts, c, pc, ph)) {
         uint32_t slot = 0;
         if (lane_id() == 0) slot = atomicAdd(&cnt->n_deferred, 1u);
         slot = uniform32(__shfl(slot, 0, WAVE));
-        if (lane_id() == 0) dlist[slot] = base + (uint32_t)l;
+        if (lane_id() == 0) dlist[slot] = g;
         deferred++;
       }
     }
   }
   flush_counters(t, c);
   if (lane_id() == 0) {
-    if (upserts) atomicAdd(t.stats + ST_UPSERTS, (unsigned long long)upserts);
-    if (deferred) atomicAdd(t.stats + ST_DEFERRED, (unsigned long long)deferred);
+    stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
+    stat_add(t.stats, ST_DEFERRED, wave_stripe(), deferred);
   }
 }
 
@@ -1144,17 +1378,19 @@ template <int F, int G>
 static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipStream_t s) {
   const uint64_t groups = (a.n_max + a.chunk - 1) / a.chunk;
   const uint64_t threads = groups * G;
-  // Persistent grid: at most LOOKUP_GRID_CAP workgroups, each lane group loops over chunks.
+  // Persistent grid: at most LOOKUP_GRID_CAP workgroups; each lane group walks one contiguous span.
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((threads + 255) / 256, LOOKUP_GRID_CAP);
   if (blocks == 0) return hipSuccess;
   if (a.route_hdr)
     k_lookup<F, G, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                  a.counters, a.chunk, a.out_vals, a.out_found,
-                                                 a.route_hdr, a.fences, a.lids, a.segs, a.desc);
+                                                 a.route_hdr, a.fences, a.lids, a.segs, a.desc,
+                                                 a.uleaf, a.uver, a.upar);
   else
     k_lookup<F, G, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                   a.counters, a.chunk, a.out_vals, a.out_found,
-                                                  nullptr, nullptr, nullptr, nullptr, a.desc);
+                                                  nullptr, nullptr, nullptr, nullptr, a.desc,
+                                                  a.uleaf, a.uver, a.upar);
   return hipGetLastError();
 }
 
@@ -1173,12 +1409,31 @@ hipError_t launch_lookup(const TreeView& t, const LookupArgs& a, hipStream_t s) 
   STM_DISPATCH_F(t.fanout, launch_lookup_f<F>(t, a, s));
 }
 
+template <int F, int G>
+static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
+  k_upsert_fast<F, G><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx,
+                                                a.ilist, a.uleaf, a.uver, a.counters, a.slist,
+                                                a.out_vals, a.out_found, a.desc);
+  return hipGetLastError();
+}
+
 template <int F>
 static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  const bool fast = a.uleaf && a.uver && a.ilist && a.slist;
+  if (fast) {
+    switch (a.lanes) {
+      case 8: e = launch_upsert_fast<F, (F >= 8 ? 8 : F)>(t, a, s); break;
+      case 16: e = launch_upsert_fast<F, (F >= 16 ? 16 : F)>(t, a, s); break;
+      case 32: e = launch_upsert_fast<F, (F >= 32 ? 32 : F)>(t, a, s); break;
+      default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+  }
   k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
-                                        a.counters, a.dlist, a.max_attempts, a.out_vals,
-                                        a.out_found, a.desc);
-  hipError_t e = hipGetLastError();
+                                        fast ? a.slist : nullptr, a.upar, a.counters, a.dlist,
+                                        a.max_attempts, a.out_vals, a.out_found, a.desc);
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_upsert_serial<F><<<1, 64, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.dlist,
                                       a.counters, a.out_vals, a.out_found, a.desc);

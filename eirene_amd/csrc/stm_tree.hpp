@@ -24,6 +24,9 @@ struct LookupArgs {
   const uint32_t* lids;
   const SegModel* segs;
   const BatchDesc* desc;  // optional: out_vals / out_found are taken from here
+  uint32_t* uleaf;        // optional: leaf of every merged INSERT (the upsert kernels' hint),
+  uint64_t* uver;         //   that leaf's lock word when the lookup kernel ran,
+  uint32_t* upar;         //   and the level-1 node that routed it (NODE_NONE if none)
 };
 
 struct UpsertArgs {
@@ -40,6 +43,12 @@ struct UpsertArgs {
   uint64_t* out_vals;
   uint8_t* out_found;
   const BatchDesc* desc;
+  const uint32_t* uleaf;  // leaf hints from the lookup kernel; with slist: the fast path runs
+  const uint64_t* uver;   // the hinted leaf's lock word when the lookup kernel ran
+  const uint32_t* upar;   // level-1 node hint for the wave-wide path
+  const uint32_t* ilist;  // the merged INSERT groups (counters->n_upserts)
+  uint32_t* slist;        // ops the fast path hands back (capacity: groups)
+  uint32_t lanes;         // lanes per op of the fast path: 8, 16 or 32
 };
 
 hipError_t launch_build_leaves(const TreeView& t, const uint64_t* ukey, const uint64_t* uval,

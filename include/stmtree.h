@@ -71,7 +71,9 @@ typedef struct stm_config {
                                  keys; default 16) */
     uint32_t use_graphs;  /* stm_run_batches_device: 1 = replay one captured hipGraph per batch
                              size class (default 1), 0 = launch every kernel from the host */
-    uint32_t reserved[1];
+    uint32_t upsert_lanes;/* lanes per insert on the upsert kernel's lane-group fast path:
+                             8, 16 or 32, at most fanout (64 / this many inserts per
+                             wavefront at once; default 16) */
 } stm_config;
 
 /* Per-phase timing slots of stm_stats.phase_ms / phase_launches. */
