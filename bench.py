@@ -218,11 +218,12 @@ def main():
         tree.sync()
         elapsed = time.perf_counter() - t_start
         st = tree.stats()
+        info = tree.route_info() if route == "learned" else None
         tree.close()
         mops = args.steps * n_batch / elapsed / 1e6
         log(f"[{route}] {args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s; "
             f"phases {st['phase_ms']}")
-        return {"mops": mops, "elapsed": elapsed, "lat": lat, "st": st, "first": first}
+        return {"mops": mops, "elapsed": elapsed, "lat": lat, "st": st, "first": first, "info": info}
 
     bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
@@ -290,7 +291,9 @@ def main():
                 "p99_batch_ms": round(float(np.percentile(x["lat"], 99)) * 1e3, 3),
                 "phase_ms_per_batch": {k: round(v / args.steps, 4) for k, v in s_["phase_ms"].items()
                                        if s_["phase_launches"][k]},
-                "route_hits": s_["route_hits"], "route_misses": s_["route_misses"]}
+                "model_window_hits": s_["route_hits"], "model_window_misses": s_["route_misses"],
+                "fence_window_hit_rate": (round(x["info"]["window_hits"] / max(1, x["info"]["leaves"]), 4)
+                                          if x["info"] else None)}
 
     uneven = None
     if args.uneven_batches > 0:

@@ -178,6 +178,7 @@ struct stm_tree {
   RouteState route;
   bool route_alloc = false;
   bool route_valid = false;       // directory + model describe a recent tree
+  bool route_warm = false;        // the segment models were trained on this tree
   uint32_t batches_since_train = 0;
   // one instantiated graph per batch size class (null until first needed)
   hipGraphExec_t graphs[GRAPH_CLASSES] = {};
@@ -310,18 +311,27 @@ void free_router(stm_tree* t) {
   t->view.dir = nullptr;
   t->route_alloc = false;
   t->route_valid = false;
+  t->route_warm = false;
 }
 
-// Rebuilds the leaf directory from the tree and trains the router (phase "train").
-int train_router(stm_tree* t, uint32_t steps) {
+// Rebuilds the leaf directory from the tree and trains the router (phase "train").  A
+// periodic retrain of models already trained on this tree starts from their parameters and
+// runs an eighth of the steps (at least 16): a few batches move few leaves.
+int train_router(stm_tree* t, uint32_t steps, bool periodic) {
   int rc = alloc_router(t);
   if (rc) return rc;
   TreeHeader h;
   rc = read_header(t, &h);
   if (rc) return rc;
   const uint32_t nn = (uint32_t)std::min<uint64_t>(h.n_nodes, t->capacity);
+  const bool warm = periodic && t->route_warm;
+  if (warm) steps = std::max<uint32_t>(steps / 8, 16);
+  // Adam's first steps move every parameter by about the learning rate whatever the
+  // gradient's size, so fine-tuning a converged model uses a tenth of it.
+  const float lr = warm ? 0.002f : 0.02f;
   phase_begin(t, STM_PHASE_TRAIN);
-  HIP_TRY(route_build(t->view, t->route, nn, steps, 0.02f, steps > 0, t->stream));
+  HIP_TRY(route_build(t->view, t->route, nn, steps, lr, steps > 0, warm, t->stream));
+  t->route_warm = steps > 0;
   phase_end(t, STM_PHASE_TRAIN);
   if (t->cfg.timing) {  // every retrain is counted, also between graph replays
     HIP_TRY(hipStreamSynchronize(t->stream));
@@ -472,7 +482,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
 int maybe_retrain(stm_tree* t) {
   if (t->cfg.route != STM_ROUTE_LEARNED) return STM_OK;
   if (t->route_valid && t->batches_since_train < t->cfg.retrain_every) return STM_OK;
-  return train_router(t, t->cfg.train_steps);
+  return train_router(t, t->cfg.train_steps, true);
 }
 
 // Scan output area of a chunk (device pointers); null = op 3 runs as GET.
@@ -726,6 +736,7 @@ int build_from_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, s
                             sizeof(unsigned long long) * STAT_STRIPES, st);
       if (r2 == hipSuccess) r2 = hipStreamSynchronize(st);
       t->route_valid = false;  // the router describes the previous tree
+      t->route_warm = false;
     }
     (void)hipFree(lows_a);
     (void)hipFree(lows_b);
@@ -1205,7 +1216,7 @@ int stm_retrain(stm_tree* t, uint32_t train_steps) {
   if (!t) return fail(STM_ERR_INVALID, "null tree");
   int rc = set_device(t);
   if (rc) return rc;
-  rc = train_router(t, train_steps);
+  rc = train_router(t, train_steps, false);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(t->stream));
   phase_collect(t);

@@ -84,7 +84,8 @@ __global__ __launch_bounds__(RT_BLOCK) void k_route_root_partial(const uint64_t*
 }
 
 __global__ void k_route_root_final(const double* __restrict__ partial, uint32_t nblocks,
-                                   const uint64_t* __restrict__ fences, RouteHeader* __restrict__ rh) {
+                                   const uint64_t* __restrict__ fences, RouteHeader* __restrict__ rh,
+                                   uint32_t keep_root) {
   if (threadIdx.x != 0) return;
   double sx = 0, sy = 0, sxx = 0, sxy = 0;
   for (uint32_t b = 0; b < nblocks; ++b) {
@@ -98,20 +99,22 @@ __global__ void k_route_root_final(const double* __restrict__ partial, uint32_t 
   const double var = sxx * n - sx * sx;
   if (n >= 2 && var > 0) a = (sxy * n - sx * sy) / var;  // least-squares slope (diagnostic)
   rh->ls_slope = a * 5.421010862427522e-20;
-  // The root is the line through the first and last fence: the least-squares line of a
-  // skewed key set leaves a large share of the keys left of its zero crossing, and clamping
-  // them into segment 0 would make that one segment hold most of the leaves.
+  // The root is the line through the first and last fence, in segment units: the
+  // least-squares line of a skewed key set leaves a large share of the keys left of its zero
+  // crossing, and clamping them into segment 0 would make that one segment hold most of the
+  // leaves.  A warm retrain keeps it (keys beyond the old last fence go to the last segment).
+  if (keep_root) return;
   const uint32_t L = rh->L;
   a = 0;
   b = 0;
   if (L >= 2) {
     const double k0 = (double)fences[0], k1 = (double)fences[L - 1];
     if (k1 > k0) {
-      a = (double)(L - 1) / (k1 - k0);
+      a = (double)ROUTE_SEGMENTS / (k1 - k0);
       b = -a * k0;
     }
   }
-  rh->a = a;  // rank per key unit
+  rh->a = a;  // segments per key unit
   rh->b = b;
 }
 
@@ -315,7 +318,7 @@ hipError_t route_evaluate(const RouteState& rs, uint64_t* d_hm, uint32_t n_max, 
 size_t route_partial_blocks() { return 256; }
 
 hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint32_t steps,
-                       float lr, bool train, hipStream_t s) {
+                       float lr, bool train, bool warm, hipStream_t s) {
   if (n_nodes == 0 || n_nodes > t.capacity) return hipErrorInvalidValue;
   k_route_lows<<<(t.capacity + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.lows, rs.rank_of);
   hipError_t e = hipGetLastError();
@@ -326,8 +329,10 @@ hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint
   k_route_directory<<<(n_nodes + 255) / 256, 256, 0, s>>>(rs.sorted_lows, rs.order, n_nodes, rs.hdr,
                                                           rs.fences, rs.lids, rs.dir, rs.rank_of);
   k_route_root_partial<<<(uint32_t)route_partial_blocks(), RT_BLOCK, 0, s>>>(rs.fences, rs.hdr, rs.partial);
-  k_route_root_final<<<1, 64, 0, s>>>(rs.partial, (uint32_t)route_partial_blocks(), rs.fences, rs.hdr);
-  k_route_segments<<<(ROUTE_SEGMENTS + 255) / 256, 256, 0, s>>>(rs.hdr, rs.fences, rs.lids, rs.segs, 1u);
+  k_route_root_final<<<1, 64, 0, s>>>(rs.partial, (uint32_t)route_partial_blocks(), rs.fences, rs.hdr,
+                                      warm ? 1u : 0u);
+  k_route_segments<<<(ROUTE_SEGMENTS + 255) / 256, 256, 0, s>>>(rs.hdr, rs.fences, rs.lids, rs.segs,
+                                                                warm ? 0u : 1u);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (train && steps > 0) {

@@ -1,9 +1,10 @@
 // stm_learned.hpp — learned leaf routing: model layout and the device-side predictor.
 //
 // The tree emits a leaf directory: the low fences of all leaves in key order (fences[r],
-// with fences[L] = +inf) and their node ids (lids[r]).  A linear root maps a key to a
-// predicted rank (least squares over (fence, rank)), which selects one of S segments; a
-// per-segment MLP 1 -> 16 -> 1 (ReLU, fp32) refines the rank inside the segment.  The
+// with fences[L] = +inf) and their node ids (lids[r]).  A linear root maps a key to one of S
+// segments (the line through the first and last fence, in segment units); a per-segment MLP
+// 1 -> 16 -> 1 (ReLU, fp32) predicts the rank inside the segment.  The root is kept across
+// periodic retrains, so each segment keeps its key range and its model a warm start.  The
 // lookup verifies the window [p-2, p+2] exactly against the directory fences and the
 // leaf's current high fence, and falls back to a root-to-leaf descent on a miss.
 #pragma once
@@ -32,7 +33,7 @@ static_assert(sizeof(SegModel) == 256, "segment record is 256 bytes");
 struct RouteHeader {
   uint32_t L;   // leaves in the directory
   uint32_t S;   // segments
-  double a, b;  // root line: rank ~ a * key + b
+  double a, b;  // root line: segment ~ a * key + b
   double ls_slope;  // least-squares slope of rank over key (reported, not used to route)
 };
 
@@ -43,7 +44,7 @@ struct RouteView {
   const DirEntry* dir;     // [L + 1], dir[L].fence = KEY_EMPTY
   uint32_t L;
   uint32_t S;
-  double a, b;             // root: rank ~ a * key + b
+  double a, b;             // root: segment ~ a * key + b
 };
 
 // MLP forward on a normalized input: returns the normalized output.
@@ -58,10 +59,9 @@ __host__ __device__ inline float mlp_forward(const float* p, float x) {
 
 __host__ __device__ inline uint32_t root_segment(const RouteView& r, uint64_t key) {
   double y = r.a * (double)key + r.b;
-  if (!(y > 0.0)) y = 0.0;
-  const double L = (double)r.L;
-  if (y > L - 1.0) y = L - 1.0;
-  uint32_t s = (uint32_t)(y * (double)r.S / L);
+  if (!(y > 0.0)) y = 0.0;  // also maps NaN to 0
+  if (y >= (double)r.S) return r.S - 1;
+  const uint32_t s = (uint32_t)y;
   return s >= r.S ? r.S - 1 : s;
 }
 
@@ -95,9 +95,10 @@ struct RouteState {
 
 size_t route_partial_blocks();
 // Builds the directory and the root; trains the segment MLPs for `steps` Adam steps when
-// train is set (otherwise the segments keep their identity-like initial parameters).
+// train is set.  warm keeps the segments' current parameters as the starting point (a
+// retrain after a few batches); otherwise they restart from the identity-like initial ones.
 hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint32_t steps, float lr,
-                       bool train, hipStream_t s);
+                       bool train, bool warm, hipStream_t s);
 hipError_t mlp_grad(const float* d_x, const float* d_t, uint32_t n, const float* d_params,
                     float* d_grad, float* d_loss, hipStream_t s);
 // d_hm[0] += ranks whose own fence is predicted within the window, d_hm[1] += the others.

@@ -127,82 +127,81 @@ STM_DEV float group_sumf(float v) {
   return v;
 }
 
-// Per lane-group state of the learned path: the directory entry of the current leaf and a
-// prefetched copy of its successor entry (keys arrive sorted, so the next leaf is usually
-// the next entry and needs no model evaluation).
+// Per lane-group state of the learned path: a window of G consecutive directory entries,
+// entry base + g in lane g (fence, leaf id, split flag).  Keys arrive sorted, so most keys
+// route inside the window with no memory access; the window is refilled from the next
+// entries when a key passes its end, and from the model's prediction otherwise.
 struct RouteCursor {
-  uint32_t r = NODE_NONE;   // directory entry of the current leaf
-  uint64_t dir_hi = 0;      // fences[r + 1]
-  uint64_t succ_hi = 0;     // fences[r + 2]; 0 when entry r + 1 does not exist
-  uint32_t succ_lid = NODE_NONE;
-  uint32_t succ_dirty = 1;
+  bool valid = false;
+  uint32_t base = 0;
+  uint64_t f = KEY_EMPTY;  // fence of entry base + g (KEY_EMPTY past the last entry)
+  uint32_t lid = NODE_NONE;
+  uint32_t dirty = 1;
 };
 
-// Learned routing of `key` to a directory entry on a tree nobody writes.  On a hit sets the
-// entry's leaf id and dirty flag (the leaf split since the directory was built) and returns
-// true; on a miss returns false and the caller descends from the root.  The successor entry
-// is tried first, then the model's window [p-2, p+2]: one load per lane brings the fence,
-// leaf id and dirty flag of one window entry.
+template <int G>
+STM_DEV void window_load(const RouteView& rv, uint32_t base, int g, RouteCursor& c) {
+  const uint64_t idx = (uint64_t)base + (uint32_t)g;
+  c.valid = true;
+  c.base = base;
+  c.f = KEY_EMPTY;
+  c.lid = NODE_NONE;
+  c.dirty = 1;
+  if (idx < rv.L) {
+    const DirEntry e = rv.dir[idx];
+    c.f = e.fence;
+    c.lid = e.lid;
+    c.dirty = e.dirty;
+  }
+}
+
+// Entry of `key` inside the window: true when some entry base + j holds it with its
+// successor's fence also in the window (j <= G - 2).
+template <int G>
+STM_DEV bool window_find(const RouteCursor& c, uint64_t key, int gbase, uint32_t& lid, bool& dirty,
+                         uint64_t& hi, uint32_t& cnt_out) {
+  const uint32_t cnt = group_sum<G>(c.valid && c.f <= key ? 1u : 0u);
+  cnt_out = cnt;
+  const uint32_t j = cnt > 0 ? cnt - 1 : 0;
+  lid = (uint32_t)__shfl((int)c.lid, gbase + (int)j, WAVE);
+  dirty = __shfl((int)c.dirty, gbase + (int)j, WAVE) != 0;
+  hi = shfl64(c.f, gbase + (int)min(cnt, (uint32_t)G - 1));
+  return c.valid && cnt >= 1 && cnt <= (uint32_t)G - 1;
+}
+
+// Learned routing of `key` to a directory entry on a tree nobody writes.  Returns true with
+// the entry's leaf id, split flag and upper fence; false sends the caller to a root-to-leaf
+// descent.  model: 0 = routed without the model, 1 = the model's +/-2 window held the key,
+// 2 = the model missed.
 template <int F, int G>
 STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, RouteCursor& c,
-                           uint32_t& lid, bool& dirty) {
-  const uint32_t L = rv.L;
-  if (L == 0) return false;
-  uint32_t r = NODE_NONE;
-  if (c.r != NODE_NONE && key >= c.dir_hi && key < c.succ_hi) {
-    r = c.r + 1;
-    lid = c.succ_lid;
-    dirty = c.succ_dirty != 0;
-    c.dir_hi = c.succ_hi;
-  } else {
-    const SegModel* m = rv.segs + root_segment(rv, key);
-    const float x = (float)(((double)key - (double)m->klo) * (double)m->inv_w);
-    float acc = 0.0f;
-    for (int j = g; j < MLP_HIDDEN; j += G)
-      acc += m->p[2 * MLP_HIDDEN + j] * fmaxf(m->p[j] * x + m->p[MLP_HIDDEN + j], 0.0f);
-    const float y = group_sumf<G>(acc) + m->p[3 * MLP_HIDDEN];
-    float rank = (float)m->r0 + y * m->n;
-    rank = fminf(fmaxf(rank, 0.0f), (float)(L - 1));  // also maps NaN to 0
-    const int64_t p = (int64_t)(rank + 0.5f);
-    const int64_t idx = p - ROUTE_WINDOW + g;
-    bool le = false;
-    uint64_t f = KEY_EMPTY;
-    uint32_t wl = NODE_NONE, wd = 1;
-    if (g < 2 * ROUTE_WINDOW + 2) {
-      if (idx < 0) {
-        le = true;
-      } else if (idx < (int64_t)L) {
-        const DirEntry e = rv.dir[idx];
-        f = e.fence;
-        wl = e.lid;
-        wd = e.dirty;
-        le = f <= key;
-      }
-    }
-    const int cnt = (int)group_sum<G>(le ? 1u : 0u);
-    // lanes 0..cnt-1 hold fences <= key: entry r = p-2+cnt-1 ends at lane cnt's fence
-    const uint32_t lid_w = (uint32_t)__shfl((int)wl, gbase + max(cnt - 1, 0), WAVE);
-    const uint32_t dirty_w = (uint32_t)__shfl((int)wd, gbase + max(cnt - 1, 0), WAVE);
-    const uint64_t hi_w = shfl64(f, gbase + min(cnt, 2 * ROUTE_WINDOW + 1));
-    if (cnt < 1 || cnt > 2 * ROUTE_WINDOW + 1) {
-      c.r = NODE_NONE;
-      return false;
-    }
-    r = (uint32_t)(p - ROUTE_WINDOW + cnt - 1);
-    lid = lid_w;
-    dirty = dirty_w != 0;
-    c.dir_hi = hi_w;
+                           uint32_t& lid, bool& dirty, uint64_t& hi, int& model) {
+  model = 0;
+  if (rv.L == 0) return false;
+  uint32_t cnt = 0;
+  if (window_find<G>(c, key, gbase, lid, dirty, hi, cnt)) return true;
+  if (c.valid && cnt == (uint32_t)G && c.base + G - 1 < rv.L) {
+    // Past the window: the next entries (one coalesced load) usually hold the key.
+    window_load<G>(rv, c.base + G - 1, g, c);
+    if (window_find<G>(c, key, gbase, lid, dirty, hi, cnt)) return true;
   }
-  c.r = r;
-  if (r + 1 < L) {  // prefetch the successor entry; its loads overlap the leaf's
-    const DirEntry e = rv.dir[r + 1];
-    c.succ_hi = rv.dir[r + 2].fence;
-    c.succ_lid = e.lid;
-    c.succ_dirty = e.dirty;
-  } else {
-    c.succ_hi = 0;
-  }
-  return true;
+  // The model: root line -> segment -> MLP -> rank p; entries [p-2, p-2+G) are loaded and the
+  // key must lie in [p-2, p+2] for a model hit.
+  const SegModel* m = rv.segs + root_segment(rv, key);
+  const float x = (float)(((double)key - (double)m->klo) * (double)m->inv_w);
+  float acc = 0.0f;
+  for (int j = g; j < MLP_HIDDEN; j += G)
+    acc += m->p[2 * MLP_HIDDEN + j] * fmaxf(m->p[j] * x + m->p[MLP_HIDDEN + j], 0.0f);
+  const float y = group_sumf<G>(acc) + m->p[3 * MLP_HIDDEN];
+  float rank = (float)m->r0 + y * m->n;
+  rank = fminf(fmaxf(rank, 0.0f), (float)(rv.L - 1));  // also maps NaN to 0
+  const int64_t p = (int64_t)(rank + 0.5f);
+  const uint32_t base = (uint32_t)max<int64_t>(p - ROUTE_WINDOW, 0);
+  window_load<G>(rv, base, g, c);
+  const bool ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt);
+  const int64_t r = (int64_t)base + (int64_t)cnt - 1;
+  model = (ok && r >= p - ROUTE_WINDOW && r <= p + ROUTE_WINDOW) ? 1 : 2;
+  return ok;
 }
 
 template <int F, int G, bool LEARNED>
@@ -249,6 +248,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   uint64_t pk[K], pv[K];
   RouteView rv{};
   RouteCursor cur;
+  uint64_t cur_hi = 0;  // upper fence of the current directory entry
   uint32_t hits = 0, misses = 0;
   if constexpr (LEARNED) rv = route_view(rh, fences, lids, segs, t.dir);
 
@@ -280,10 +280,13 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
         if constexpr (LEARNED) {
           uint32_t lid = leaf;
           bool dirty = true;
+          int model = 0;
           // Still inside the current directory entry: the leaf split, walk right from it.
-          const bool same_entry = cur.r != NODE_NONE && leaf != NODE_NONE && key < cur.dir_hi;
-          routed = same_entry || learned_route<F, G>(rv, key, g, gbase, cur, lid, dirty);
+          const bool same_entry = leaf != NODE_NONE && key < cur_hi;
+          uint64_t hi = cur_hi;
+          routed = same_entry || learned_route<F, G>(rv, key, g, gbase, cur, lid, dirty, hi, model);
           if (routed) {
+            cur_hi = hi;
             leaf = min(lid, cap - 1);
             uint64_t mh = KEY_EMPTY;
             if (dirty) mh = t.meta[leaf].high;
@@ -299,12 +302,14 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
                 }
               }
               if (routed) load_slots<F, G>(t.keys, leaf, g, lk);
-              else cur.r = NODE_NONE;
+              else cur_hi = 0;
             }
-            leaf_hi = min(cur.dir_hi, mh);
+            leaf_hi = min(cur_hi, mh);
+          } else {
+            cur_hi = 0;
           }
-          hits += routed ? 1 : 0;
-          misses += routed ? 0 : 1;
+          hits += model == 1 ? 1 : 0;
+          misses += model == 2 ? 1 : 0;
         }
         if (!routed) {
           if (height <= 1) {
