@@ -14,7 +14,9 @@ events the library records on its own stream around every phase (stm_config.timi
 A second section ("uneven") runs the uneven-batching workload: 2^26 Normal(2^62, 2^58)
 keys and a stream of batches with sizes log-uniform in [2^10, 2^22], replayed from captured
 hipGraphs with launch parameters chosen by eirene_amd.autotune, beside the same stream
-launched batch by batch from the host with the library defaults.
+launched batch by batch from the host with the library defaults.  A third ("serving") runs the
+continuous-batching request loop open-loop: 2^25 uniform keys, Poisson arrivals of requests
+of 1..2^16 ops (90% lookups, 8% inserts, 2% range scans), request latency percentiles.
 
     python bench.py [--steps K] [--warmup W] [--route descent|learned] [--no-cpu]
                     [--uneven-batches N] [--tuning FILE]
@@ -58,7 +60,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU-baseline budget (batches run until it is spent)")
     ap.add_argument("--no-headline", action="store_true",
-                    help="skip the learned-vs-plain section (profiling the uneven stream alone)")
+                    help="skip the learned-vs-plain section (profiling the other sections alone)")
+    ap.add_argument("--serve-requests", type=int, default=8000,
+                    help="serving loop: requests offered (0 = skip)")
+    ap.add_argument("--serve-mops", type=float, default=200.0,
+                    help="serving loop: offered load in Mops/s (Poisson arrivals)")
     ap.add_argument("--uneven-batches", type=int, default=256,
                     help="uneven stream: batches log-uniform in [2^10, 2^22] (0 = skip)")
     ap.add_argument("--tuning", default=None,
@@ -156,6 +162,44 @@ def run_uneven(args, ea, w):
     }
 
 
+def run_serving(args, ea, w):
+    """Serving loop: 2^25 uniform keys resident in HBM; open-loop Poisson arrivals at
+    serve_mops; requests of floor(2^(16u)) ops (1 .. 2^16), 90% lookups, 8% inserts, 2% range
+    scans of length 16..256; FIFO admission by HBM-budget slots; request latency percentiles."""
+    from eirene_amd.serving import open_loop
+    n_load = 1 << 25
+    t0 = time.time()
+    load = w.uniform_keys(n_load, args.seed + 5)
+    lv = w.values_for(load)
+    srt = w.sorted_copy(load)
+    cfg = ea.StmConfig(max_keys=n_load + (1 << 24), max_batch=1 << 22)
+    with ea.StmTree(cfg) as t:
+        dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(lv)
+        t.bulk_load_device(dk, dv)
+        dk.free()
+        dv.free()
+        log(f"[serve] loaded 2^25 keys in {time.time() - t0:.1f}s")
+        rep = open_loop(t, srt, requests=args.serve_requests, offered_mops=args.serve_mops,
+                        ops_log2=(0, 16), lookup_frac=0.90, insert_frac=0.08, scan_len=(16, 256),
+                        seed=args.seed + 7)
+        bad = t.check()
+    log(f"[serve] {rep['requests']} requests, {rep['mops']:.1f} Mops/s, p50 {rep['p50_ms']:.2f} ms, "
+        f"p99 {rep['p99_ms']:.2f} ms, {rep['batches']} batches")
+    return {
+        "workload": "2^25 uniform keys; open-loop Poisson arrivals; requests of 1..2^16 ops "
+                    "(log-uniform), 90% lookup / 8% insert / 2% scan (16..256)",
+        "offered_mops_target": args.serve_mops,
+        "offered_mops": round(rep["offered_mops"], 2), "mops": round(rep["mops"], 2),
+        "requests": rep["requests"], "ops": rep["ops"], "failed": rep["failed"],
+        "latency_ms": {k: round(rep[k + "_ms"], 3) for k in ("p50", "p90", "p99", "p999", "max", "mean")},
+        "batches": rep["batches"], "mean_batch_ops": round(rep["mean_batch_ops"], 1),
+        "max_pending": rep["max_pending"], "busy_frac": round(rep["busy_frac"], 3),
+        "scans": rep["scans"], "scan_pairs": rep["scan_pairs"],
+        "hbm_budget_bytes": rep["hbm_budget"], "slot_ops": rep["slot_ops"],
+        "tree_check_violations": bad,
+    }
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -170,7 +214,8 @@ def main():
     from eirene_amd import workloads as w
 
     if args.no_headline:
-        print(json.dumps({"uneven": run_uneven(args, ea, w) if args.uneven_batches > 0 else None}),
+        print(json.dumps({"uneven": run_uneven(args, ea, w) if args.uneven_batches > 0 else None,
+                          "serving": run_serving(args, ea, w) if args.serve_requests > 0 else None}),
               flush=True)
         return
 
@@ -296,9 +341,12 @@ def main():
                                           if x["info"] else None)}
 
     uneven = None
+    serving = None
+    bk.free(); bv.free(); bo.free(); out_v.free(); out_f.free()
     if args.uneven_batches > 0:
-        bk.free(); bv.free(); bo.free(); out_v.free(); out_f.free()
         uneven = run_uneven(args, ea, w)
+    if args.serve_requests > 0:
+        serving = run_serving(args, ea, w)
 
     out = {
         "metric": METRIC, "value": round(mops, 2), "unit": "Mops/s", "n_gpus": 1,
@@ -321,6 +369,7 @@ def main():
                                    "root_splits", "deferred", "n_keys", "height")},
         "roofline": roofline, "cpu_baseline": cpu, "cpu_gpu_first_batch_agree": agree,
         "uneven": uneven,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -6,6 +6,7 @@ Search space (every point is a valid StmConfig):
     ops_per_group        sorted ops walked per lane group 8, 16, 32
     sort_tile_items      radix-sort elements per thread  8, 16  (tiles of 2048 / 4096 keys)
     upsert_ops_per_wave  batch slots per upsert wavefront 256, 512, 1024
+    upsert_lanes         lanes per insert on the upsert fast path 8, 16, 32 (at most fanout)
 
 Method: for each fanout (the tree must be rebuilt for it), coordinate descent over the other
 parameters from the library defaults, two sweeps, keeping a change only when it is faster by
@@ -39,6 +40,7 @@ SPACE = {
     "ops_per_group": [8, 16, 32],
     "sort_tile_items": [8, 16],
     "upsert_ops_per_wave": [256, 512, 1024],
+    "upsert_lanes": [8, 16, 32],
 }
 FANOUTS = [32, 64, 128]
 TUNED_FIELDS = ["fanout"] + list(SPACE)
@@ -105,7 +107,8 @@ def tune(sample: Sample, base: StmConfig, fanouts=FANOUTS, sweeps: int = 2,
     best_cfg, best_mops = base, default_mops
     for F in fanouts:
         cur = StmConfig(**{**asdict(base), "fanout": F,
-                           "lanes_per_op": min(base.lanes_per_op, F)})
+                           "lanes_per_op": min(base.lanes_per_op, F),
+                           "upsert_lanes": min(base.upsert_lanes, F)})
         cur_mops = measure(cur)
         for _ in range(sweeps):
             changed = False
@@ -113,7 +116,7 @@ def tune(sample: Sample, base: StmConfig, fanouts=FANOUTS, sweeps: int = 2,
                 for v in values:
                     if v == getattr(cur, field):
                         continue
-                    if field == "lanes_per_op" and v > F:
+                    if field in ("lanes_per_op", "upsert_lanes") and v > F:
                         continue
                     cand = StmConfig(**{**asdict(cur), field: v})
                     m = measure(cand)

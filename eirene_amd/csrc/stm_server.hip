@@ -10,12 +10,15 @@
 //              time one slot is forming, one is running on the tree, the others drain.
 //   admission  FIFO.  A request joins the forming slot if its ops and scan pairs fit;
 //              otherwise it and every later request wait until a slot frees up.  Admission
-//              copies the request into the slot's pinned buffers (the caller's thread).
+//              reserves the request's place under the server lock; the copy into the slot's
+//              pinned buffers happens outside it, in the admitting thread, so several
+//              submitting threads copy at once (the runner waits for a slot's copies).
 //   runner     when the tree is idle it closes the forming slot (the next free slot starts
 //              forming), copies it to HBM on its own stream, runs it as one batch (scan mode
 //              when it holds scans) and queues the answer copies on a third stream.
-//   completer  waits for a slot's answers, scatters them into the requests' buffers, marks
-//              the requests done and returns the slot.
+//   completer  waits for a slot's answers, scatters them into the requests' buffers (split
+//              over helper threads for large slots), marks the requests done and returns the
+//              slot.
 // Requests of one batch are concatenated in admission order, and batches run in order, so
 // every request sees exactly the state it would see if the requests ran one by one in
 // admission order: a result is the same alone and in the mix.
@@ -60,6 +63,7 @@ double now_s() {
 
 constexpr uint64_t BYTES_PER_OP = 8 + 8 + 1 + 8 + 1;  // key, value, op in; value, found out
 constexpr uint64_t BYTES_PER_PAIR = 16;
+constexpr unsigned SCATTER_THREADS = 8;
 
 struct Req {
   uint64_t ticket = 0;
@@ -74,6 +78,7 @@ struct Req {
   uint64_t pairs = 0;  // sum of its scan lengths
   uint64_t scans = 0;
   double t_arrival = 0, t_admit = 0, t_done = 0;
+  uint64_t off = 0, poff = 0;  // place in its slot (ops, scan pairs)
   int status = STM_OK;
   std::string error;
   bool done = false;
@@ -90,12 +95,14 @@ struct Slot {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::vector<Req*> reqs;
   uint64_t n = 0, pairs = 0, scans = 0;
+  uint32_t copies = 0;  // admitted requests whose inputs are still being copied in
   int status = STM_OK;
   std::string error;
 
   void reset() {
     reqs.clear();
     n = pairs = scans = 0;
+    copies = 0;
     status = STM_OK;
     error.clear();
   }
@@ -130,8 +137,11 @@ struct stm_server {
   uint64_t st_max_pending = 0;
   double st_busy = 0;
 
-  // Admission of queued requests into the forming slot (mu held).
-  void admit_locked() {
+  std::condition_variable cv_copies;  // a slot's input copies finished
+
+  // Admission of queued requests into the forming slot (mu held): reserves their places and
+  // returns them in `out`; the caller copies them in (copy_in) after releasing the lock.
+  void admit_locked(std::vector<std::pair<Slot*, Req*>>& out) {
     if (!forming && !free_slots.empty()) {
       forming = free_slots.front();
       free_slots.pop_front();
@@ -141,20 +151,40 @@ struct stm_server {
       Slot* s = forming;
       if (s->n + r->n > s->cap_ops || s->pairs + r->pairs > s->cap_pairs) break;
       pending.pop_front();
-      std::memcpy(s->h_keys + s->n, r->keys, r->n * 8);
-      std::memcpy(s->h_vals + s->n, r->vals, r->n * 8);
-      std::memcpy(s->h_ops + s->n, r->ops, r->n);
+      r->off = s->n;
+      r->poff = s->pairs;
       s->n += r->n;
       s->pairs += r->pairs;
       s->scans += r->scans;
+      s->copies += 1;
       r->t_admit = now_s();
       s->reqs.push_back(r);
+      out.emplace_back(s, r);
     }
-    if (forming && forming->n) cv_run.notify_one();
+  }
+
+  // Copies admitted requests into their slots (mu not held), then releases them.
+  void copy_in(std::vector<std::pair<Slot*, Req*>>& adm) {
+    if (adm.empty()) return;
+    for (auto& sr : adm) {
+      Slot* s = sr.first;
+      Req* r = sr.second;
+      std::memcpy(s->h_keys + r->off, r->keys, r->n * 8);
+      std::memcpy(s->h_vals + r->off, r->vals, r->n * 8);
+      std::memcpy(s->h_ops + r->off, r->ops, r->n);
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto& sr : adm) sr.first->copies -= 1;
+    }
+    adm.clear();
+    cv_copies.notify_all();
+    cv_run.notify_one();
   }
 
   void run_loop() {
     (void)hipSetDevice(device);
+    std::vector<std::pair<Slot*, Req*>> adm;
     for (;;) {
       Slot* s = nullptr;
       {
@@ -166,8 +196,10 @@ struct stm_server {
         }
         s = forming;
         forming = nullptr;
-        admit_locked();  // the next free slot starts forming
+        admit_locked(adm);  // the next free slot starts forming
+        cv_copies.wait(lk, [&] { return s->copies == 0; });  // the slot's inputs are in place
       }
+      copy_in(adm);
       const double t0 = now_s();
       hipError_t e = hipMemcpyAsync(s->d_keys, s->h_keys, s->n * 8, hipMemcpyHostToDevice, s_in);
       if (e == hipSuccess) e = hipMemcpyAsync(s->d_vals, s->h_vals, s->n * 8, hipMemcpyHostToDevice, s_in);
@@ -229,20 +261,32 @@ struct stm_server {
         s->status = STM_ERR_DEVICE;
         s->error = std::string("answer copy: ") + hipGetErrorString(e);
       }
-      uint64_t off = 0, poff = 0;
-      for (Req* r : s->reqs) {
-        if (s->status == STM_OK) {
-          std::memcpy(r->out_vals, s->h_ovals + off, r->n * 8);
-          std::memcpy(r->out_found, s->h_ofound + off, r->n);
-          if (r->pairs) {
-            std::memcpy(r->scan_keys, s->h_skeys + poff, r->pairs * 8);
-            std::memcpy(r->scan_vals, s->h_svals + poff, r->pairs * 8);
+      if (s->status == STM_OK) {
+        // Large slots are scattered by several threads (each takes every k-th request).
+        const uint64_t bytes = s->n * 9 + s->pairs * 16;
+        const unsigned k = bytes > (8u << 20) ? SCATTER_THREADS : 1u;
+        auto scatter = [s, k](unsigned part) {
+          for (size_t i = part; i < s->reqs.size(); i += k) {
+            Req* r = s->reqs[i];
+            std::memcpy(r->out_vals, s->h_ovals + r->off, r->n * 8);
+            std::memcpy(r->out_found, s->h_ofound + r->off, r->n);
+            if (r->pairs) {
+              std::memcpy(r->scan_keys, s->h_skeys + r->poff, r->pairs * 8);
+              std::memcpy(r->scan_vals, s->h_svals + r->poff, r->pairs * 8);
+            }
           }
+        };
+        if (k == 1) {
+          scatter(0);
+        } else {
+          std::vector<std::thread> th;
+          for (unsigned part = 1; part < k; ++part) th.emplace_back(scatter, part);
+          scatter(0);
+          for (auto& x : th) x.join();
         }
-        off += r->n;
-        poff += r->pairs;
       }
       const double t = now_s();
+      std::vector<std::pair<Slot*, Req*>> adm;
       {
         std::lock_guard<std::mutex> lk(mu);
         for (Req* r : s->reqs) {
@@ -258,8 +302,9 @@ struct stm_server {
         }
         s->reset();
         free_slots.push_back(s);
-        admit_locked();
+        admit_locked(adm);
       }
+      copy_in(adm);
       cv_done.notify_all();
     }
   }
@@ -410,16 +455,20 @@ int stm_server_submit(stm_server* s, const uint64_t* keys, const uint64_t* vals,
   r->pairs = pairs;
   r->scans = scans;
   r->t_arrival = arrival_s >= 0 ? arrival_s : now_s();
-  std::lock_guard<std::mutex> lk(s->mu);
-  if (s->stop) return sfail(STM_ERR_STATE, "server is shutting down");
-  r->ticket = s->next_ticket++;
-  *ticket = r->ticket;
-  Req* rp = r.get();
-  s->reqs.emplace(rp->ticket, std::move(r));
-  s->pending.push_back(rp);
-  s->outstanding += 1;
-  s->st_max_pending = std::max<uint64_t>(s->st_max_pending, s->pending.size());
-  s->admit_locked();
+  std::vector<std::pair<Slot*, Req*>> adm;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return sfail(STM_ERR_STATE, "server is shutting down");
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    Req* rp = r.get();
+    s->reqs.emplace(rp->ticket, std::move(r));
+    s->pending.push_back(rp);
+    s->outstanding += 1;
+    s->st_max_pending = std::max<uint64_t>(s->st_max_pending, s->pending.size());
+    s->admit_locked(adm);
+  }
+  s->copy_in(adm);
   return STM_OK;
 }
 
@@ -537,18 +586,31 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
   stm_server* srv = nullptr;
   int rc = stm_server_create(tree, cfg, &srv);
   if (rc) return rc;
+  // CLIENTS threads submit the requests (client c takes every CLIENTS-th), each at its
+  // arrival time, so copying requests in does not delay later arrivals.
+  constexpr unsigned CLIENTS = 4;
   std::vector<uint64_t> tickets(R, 0);
+  std::atomic<int> submit_rc{STM_OK};
   const double t0 = now_s() + 0.005;
-  for (size_t r = 0; r < R && rc == STM_OK; ++r) {
-    const double due = t0 + arrival[r];
-    for (double n = now_s(); n < due; n = now_s())
-      if (due - n > 300e-6) std::this_thread::sleep_for(std::chrono::microseconds(100));
-    rc = stm_server_submit(srv, keys.data() + off[r], vals.data() + off[r], ops.data() + off[r],
-                           off[r + 1] - off[r], ovals.data() + off[r], ofound.data() + off[r],
-                           skeys.data() + poff[r], svals.data() + poff[r], poff[r + 1] - poff[r], due,
-                           &tickets[r]);
+  {
+    std::vector<std::thread> clients;
+    for (unsigned cl = 0; cl < CLIENTS; ++cl)
+      clients.emplace_back([&, cl] {
+        for (size_t r = cl; r < R && submit_rc.load() == STM_OK; r += CLIENTS) {
+          const double due = t0 + arrival[r];
+          for (double n = now_s(); n < due; n = now_s())
+            if (due - n > 300e-6) std::this_thread::sleep_for(std::chrono::microseconds(100));
+          const int rcs = stm_server_submit(srv, keys.data() + off[r], vals.data() + off[r],
+                                            ops.data() + off[r], off[r + 1] - off[r],
+                                            ovals.data() + off[r], ofound.data() + off[r],
+                                            skeys.data() + poff[r], svals.data() + poff[r],
+                                            poff[r + 1] - poff[r], due, &tickets[r]);
+          if (rcs != STM_OK) submit_rc = rcs;
+        }
+      });
+    for (auto& x : clients) x.join();
   }
-  const int submit_rc = rc;
+  rc = submit_rc.load();
   std::vector<double> lat;
   lat.reserve(R);
   uint64_t failed = 0;
@@ -563,7 +625,7 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
   stm_server_stats st{};
   stm_server_get_stats(srv, &st);
   stm_server_destroy(srv);
-  if (submit_rc) return submit_rc;
+  if (rc) return rc;
   std::vector<double> sorted_lat = lat;
   std::sort(sorted_lat.begin(), sorted_lat.end());
   auto pct = [&](double q) {
