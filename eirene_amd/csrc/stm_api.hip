@@ -116,6 +116,7 @@ struct Scratch {
   uint32_t* upar = nullptr;   // level-1 node hint of every merged INSERT
   uint64_t* uver = nullptr;   // version of the hinted leaf
   BatchCounters* counters = nullptr;
+  BatchCounters* h_counters = nullptr;  // pinned host copy for the per-batch readback
   // range scans
   uint64_t *soff = nullptr, *scan_tiles = nullptr;
   int32_t* lastins = nullptr;
@@ -145,6 +146,7 @@ struct Scratch {
     A_(upar, cap);
     A_(uver, cap);
     A_(counters, 1);
+    if ((e = hipHostMalloc((void**)&h_counters, sizeof(BatchCounters), 0)) != hipSuccess) return e;
     if (staging) {
       A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
       A_(soff, cap); A_(lastins, cap); A_(scan_tiles, scan_tile_words(cap));
@@ -158,6 +160,7 @@ struct Scratch {
                   desc_table, desc_live, cursor, acc, soff, lastins, scan_tiles};
     for (void* p : ps)
       if (p) (void)hipFree(p);
+    if (h_counters) (void)hipHostFree(h_counters);
     *this = Scratch();
   }
 };
@@ -241,15 +244,19 @@ int read_header(stm_tree* t, TreeHeader* h) {
   return STM_OK;
 }
 
+int error_bits_status(stm_tree* t, uint64_t error) {
+  if (error & ERR_POOL_FULL)
+    return fail(STM_ERR_NOMEM, "node pool exhausted (capacity %llu nodes); raise max_keys",
+                (unsigned long long)t->capacity);
+  if (error & ERR_LIVELOCK) return fail(STM_ERR_DEVICE, "serial upsert pass could not commit");
+  return STM_OK;
+}
+
 int check_device_error(stm_tree* t) {
   TreeHeader h;
   int rc = read_header(t, &h);
   if (rc) return rc;
-  if (h.error & ERR_POOL_FULL)
-    return fail(STM_ERR_NOMEM, "node pool exhausted (capacity %llu nodes); raise max_keys",
-                (unsigned long long)t->capacity);
-  if (h.error & ERR_LIVELOCK) return fail(STM_ERR_DEVICE, "serial upsert pass could not commit");
-  return STM_OK;
+  return error_bits_status(t, h.error);
 }
 
 // Writes an empty tree: one empty leaf as root.
@@ -508,9 +515,10 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
     in.scan_cap = so->cap;
   }
   HIP_TRY(enqueue_pipeline(t, in, true));
-  BatchCounters bc;
-  HIP_TRY(hipMemcpyAsync(&bc, s.counters, sizeof(bc), hipMemcpyDeviceToHost, st));
+  // One readback per batch, into pinned memory: counters and the tree's error bits.
+  HIP_TRY(hipMemcpyAsync(s.h_counters, s.counters, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  const BatchCounters bc = *s.h_counters;
   phase_collect(t);
   t->st.batches += 1;
   t->st.ops += n;
@@ -524,7 +532,7 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
                   "scan output area (%llu pairs)", bc.scan_bad, SCAN_MAX_LEN,
                   (unsigned long long)so->cap);
   }
-  return check_device_error(t);
+  return error_bits_status(t, bc.hdr_error);
 }
 
 // ------------------------------------------------------------------ graph replay of streams

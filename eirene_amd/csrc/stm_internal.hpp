@@ -61,7 +61,9 @@ struct BatchCounters {
   uint32_t scan_bad;    // scan ops rejected (length or output area)
   uint64_t scan_total;  // sum of the scan lengths (pairs the output area must hold)
   uint32_t n_slow;      // merged INSERTs the fast path handed to the transactional kernel
-  uint32_t pad[7];
+  uint32_t pad0;
+  uint64_t hdr_error;   // the tree's sticky error bits after the batch (one readback per batch)
+  uint32_t pad[4];
 };
 static_assert(sizeof(BatchCounters) == 64, "one 64-byte block");
 

@@ -1241,6 +1241,9 @@ __global__ __launch_bounds__(64) void k_upsert_serial(TreeView t, const uint64_t
     }
   }
   flush_counters(t, c);
+  // The last kernel of a batch: the tree's error bits travel back with the batch counters.
+  wave_order();
+  if (lane_id() == 0) const_cast<BatchCounters*>(cnt)->hdr_error = ld_relaxed(&t.hdr->error);
 }
 
 // ================================================================== dump / check
