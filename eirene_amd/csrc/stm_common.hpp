@@ -80,13 +80,15 @@ STM_DEV void stat_add(unsigned long long* stats, int slot, uint32_t stripe, uint
 }
 
 // Learned-routing directory entry r: the low fence and node id of the r-th leaf in key order
-// when the directory was built, and whether that leaf has split since (its live high fence
-// must then be read).  16 bytes, so one load brings all three.
+// when the directory was built, its level-1 parent then (a hint for splits) and whether the
+// leaf has split since (its live high fence must then be read).  16 bytes, one load.
 struct alignas(16) DirEntry {
   uint64_t fence;
   uint32_t lid;
-  uint32_t dirty;
+  uint32_t pflags;  // parent id in bits 0..30 (DIR_PAR_NONE: unknown), DIR_SPLIT in bit 31
 };
+constexpr uint32_t DIR_SPLIT = 1u << 31;
+constexpr uint32_t DIR_PAR_NONE = DIR_SPLIT - 1;
 
 struct TreeView {
   uint64_t* keys;
@@ -97,7 +99,7 @@ struct TreeView {
   uint32_t capacity;
   uint32_t fanout;
   // Learned-routing directory hooks (null without a router): a committed leaf split marks
-  // the leaf's directory entry dirty so lookups re-read its live high fence.
+  // DIR_SPLIT in the leaf's directory entry so lookups re-read its live high fence.
   const uint32_t* rank_of;
   DirEntry* dir;
 };

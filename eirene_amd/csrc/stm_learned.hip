@@ -45,13 +45,29 @@ __global__ void k_route_directory(const uint64_t* __restrict__ sorted_lows,
   if (low == KEY_EMPTY) return;
   fences[r] = low;
   lids[r] = order[r];
-  dir[r] = DirEntry{low, order[r], 0u};
+  dir[r] = DirEntry{low, order[r], DIR_PAR_NONE};
   rank_of[order[r]] = r;
   if (r + 1 == n || sorted_lows[r + 1] == KEY_EMPTY) {
     rh->L = r + 1;
     rh->S = ROUTE_SEGMENTS;
     fences[r + 1] = KEY_EMPTY;
-    dir[r + 1] = DirEntry{KEY_EMPTY, NODE_NONE, 1u};
+    dir[r + 1] = DirEntry{KEY_EMPTY, NODE_NONE, DIR_SPLIT | DIR_PAR_NONE};
+  }
+}
+
+// Parent hints: every level-1 node writes its id into its children's directory entries.
+__global__ void k_route_parents(TreeView t, uint32_t n_nodes, DirEntry* __restrict__ dir) {
+  const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n_nodes) return;
+  const NodeMeta m = t.meta[id];
+  if (meta_level(m.info) != 1) return;
+  const uint32_t cnt = min(meta_count(m.info), t.fanout);
+  const uint32_t par = id < DIR_PAR_NONE ? id : DIR_PAR_NONE;
+  for (uint32_t s = 0; s < cnt; ++s) {
+    const uint64_t child = t.vals[(uint64_t)id * t.fanout + s];
+    if (child >= t.capacity) continue;
+    const uint32_t r = t.rank_of[child];
+    if (r != NODE_NONE) dir[r].pflags = par;
   }
 }
 
@@ -328,6 +344,7 @@ hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint
   if (e != hipSuccess) return e;
   k_route_directory<<<(n_nodes + 255) / 256, 256, 0, s>>>(rs.sorted_lows, rs.order, n_nodes, rs.hdr,
                                                           rs.fences, rs.lids, rs.dir, rs.rank_of);
+  k_route_parents<<<(n_nodes + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.dir);
   k_route_root_partial<<<(uint32_t)route_partial_blocks(), RT_BLOCK, 0, s>>>(rs.fences, rs.hdr, rs.partial);
   k_route_root_final<<<1, 64, 0, s>>>(rs.partial, (uint32_t)route_partial_blocks(), rs.fences, rs.hdr,
                                       warm ? 1u : 0u);

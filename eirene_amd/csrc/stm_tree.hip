@@ -136,7 +136,7 @@ struct RouteCursor {
   uint32_t base = 0;
   uint64_t f = KEY_EMPTY;  // fence of entry base + g (KEY_EMPTY past the last entry)
   uint32_t lid = NODE_NONE;
-  uint32_t dirty = 1;
+  uint32_t pflags = DIR_SPLIT | DIR_PAR_NONE;
 };
 
 template <int G>
@@ -146,12 +146,12 @@ STM_DEV void window_load(const RouteView& rv, uint32_t base, int g, RouteCursor&
   c.base = base;
   c.f = KEY_EMPTY;
   c.lid = NODE_NONE;
-  c.dirty = 1;
+  c.pflags = DIR_SPLIT | DIR_PAR_NONE;
   if (idx < rv.L) {
     const DirEntry e = rv.dir[idx];
     c.f = e.fence;
     c.lid = e.lid;
-    c.dirty = e.dirty;
+    c.pflags = e.pflags;
   }
 }
 
@@ -159,12 +159,14 @@ STM_DEV void window_load(const RouteView& rv, uint32_t base, int g, RouteCursor&
 // successor's fence also in the window (j <= G - 2).
 template <int G>
 STM_DEV bool window_find(const RouteCursor& c, uint64_t key, int gbase, uint32_t& lid, bool& dirty,
-                         uint64_t& hi, uint32_t& cnt_out) {
+                         uint64_t& hi, uint32_t& cnt_out, uint32_t& par) {
   const uint32_t cnt = group_sum<G>(c.valid && c.f <= key ? 1u : 0u);
   cnt_out = cnt;
   const uint32_t j = cnt > 0 ? cnt - 1 : 0;
   lid = (uint32_t)__shfl((int)c.lid, gbase + (int)j, WAVE);
-  dirty = __shfl((int)c.dirty, gbase + (int)j, WAVE) != 0;
+  const uint32_t pf = (uint32_t)__shfl((int)c.pflags, gbase + (int)j, WAVE);
+  dirty = (pf & DIR_SPLIT) != 0;
+  par = (pf & DIR_PAR_NONE) == DIR_PAR_NONE ? NODE_NONE : (pf & DIR_PAR_NONE);
   hi = shfl64(c.f, gbase + (int)min(cnt, (uint32_t)G - 1));
   return c.valid && cnt >= 1 && cnt <= (uint32_t)G - 1;
 }
@@ -175,15 +177,15 @@ STM_DEV bool window_find(const RouteCursor& c, uint64_t key, int gbase, uint32_t
 // 2 = the model missed.
 template <int F, int G>
 STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, RouteCursor& c,
-                           uint32_t& lid, bool& dirty, uint64_t& hi, int& model) {
+                           uint32_t& lid, bool& dirty, uint64_t& hi, int& model, uint32_t& par) {
   model = 0;
   if (rv.L == 0) return false;
   uint32_t cnt = 0;
-  if (window_find<G>(c, key, gbase, lid, dirty, hi, cnt)) return true;
+  if (window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par)) return true;
   if (c.valid && cnt == (uint32_t)G && c.base + G - 1 < rv.L) {
     // Past the window: the next entries (one coalesced load) usually hold the key.
     window_load<G>(rv, c.base + G - 1, g, c);
-    if (window_find<G>(c, key, gbase, lid, dirty, hi, cnt)) return true;
+    if (window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par)) return true;
   }
   // The model: root line -> segment -> MLP -> rank p; entries [p-2, p-2+G) are loaded and the
   // key must lie in [p-2, p+2] for a model hit.
@@ -198,7 +200,7 @@ STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, 
   const int64_t p = (int64_t)(rank + 0.5f);
   const uint32_t base = (uint32_t)max<int64_t>(p - ROUTE_WINDOW, 0);
   window_load<G>(rv, base, g, c);
-  const bool ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt);
+  const bool ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par);
   const int64_t r = (int64_t)base + (int64_t)cnt - 1;
   model = (ok && r >= p - ROUTE_WINDOW && r <= p + ROUTE_WINDOW) ? 1 : 2;
   return ok;
@@ -281,11 +283,14 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
           uint32_t lid = leaf;
           bool dirty = true;
           int model = 0;
+          uint32_t rpar = par;
           // Still inside the current directory entry: the leaf split, walk right from it.
           const bool same_entry = leaf != NODE_NONE && key < cur_hi;
           uint64_t hi = cur_hi;
-          routed = same_entry || learned_route<F, G>(rv, key, g, gbase, cur, lid, dirty, hi, model);
+          routed = same_entry ||
+                   learned_route<F, G>(rv, key, g, gbase, cur, lid, dirty, hi, model, rpar);
           if (routed) {
+            par = rpar;  // the level-1 hint recorded for merged INSERTs
             cur_hi = hi;
             leaf = min(lid, cap - 1);
             uint64_t mh = KEY_EMPTY;
@@ -790,7 +795,7 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
   }
   if (t.rank_of && lane_id() == 0) {
     const uint32_t r = t.rank_of[n.id];
-    if (r != NODE_NONE) t.dir[r].dirty = 1;
+    if (r != NODE_NONE) t.dir[r].pflags |= DIR_SPLIT;
   }
   c.leaf_splits++;
   c.commits++;
