@@ -293,9 +293,21 @@ def main():
     mix_bytes = 0.95 * BYTES_LOOKUP + 0.05 * BYTES_INSERT
     kname = "k_lookup<%d,%d,%s>" % (args.fanout, args.lanes,
                                     "true" if "learned" in res else "false")
+    # Measured L2<->memory bytes per launch of this kernel for this workload (rocprofv3 PMC
+    # FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel calibration), committed with the profile.
+    traffic = None
+    tf = ROOT / "profiles" / "hbm_traffic_gamma_2p26_batch_2p22.json"
+    if tf.exists() and args.fanout == 64 and args.lanes == 16 and args.load_log2 == 26 \
+            and args.batch_log2 == 22:
+        rt = "learned" if "learned" in res else "descent"
+        kt = json.loads(tf.read_text())["routes"][rt]
+        kk = next((k for k in kt if k.startswith("k_lookup")), None)
+        if kk:
+            traffic = kt[kk]["read_bytes"] + kt[kk]["written_bytes"]
     roofline = {
         "kernel": kname, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "traffic_source": "profiles/hbm_traffic_gamma_2p26_batch_2p22.json" if traffic else None,
         "avg_launch_ms": round(lk_ms, 4), "lookups_per_launch": round(lookups / lk_launch),
         "bytes_per_lookup": BYTES_LOOKUP,
         "whole_batch_gbs": round(total_ops * mix_bytes / elapsed / 1e9, 1),
