@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--group", type=int, default=16)
     ap.add_argument("--route", choices=["descent", "learned", "both"], default="both")
+    ap.add_argument("--tuning-headline", default=str(ROOT / "eirene_amd" / "tuning" /
+                                                     "gamma_2p26_batch_2p22.json"),
+                    help="tuning record (eirene_amd.autotune) for the tuned routes; '' = none")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -237,11 +240,19 @@ def main():
                                                  0.9, w.DIST_GAMMA, 2.0, 2.0 ** 56)
     log(f"generated {n_batches} batches in {time.time() - t0:.1f}s")
 
+    tuned = None
+    if args.tuning_headline and Path(args.tuning_headline).exists():
+        from eirene_amd import autotune as at
+        tuned = at.load_tuning(args.tuning_headline)["best"]
+
     def run_route(route):
         cfg = ea.StmConfig(fanout=args.fanout, max_keys=n_load + n_batches * n_batch + (1 << 20),
                            max_batch=n_batch, lanes_per_op=args.lanes, ops_per_group=args.group,
-                           route=ea.ROUTE_LEARNED if route == "learned" else ea.ROUTE_DESCENT,
+                           route=ea.ROUTE_LEARNED if route.startswith("learned") else ea.ROUTE_DESCENT,
                            retrain_every=8, timing=True)
+        if route.endswith("_tuned"):
+            from eirene_amd import autotune as at
+            cfg = at.apply_tuning(cfg, tuned)
         tree = ea.StmTree(cfg)
         t0 = time.time()
         dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(load_vals)
@@ -263,7 +274,7 @@ def main():
         tree.sync()
         elapsed = time.perf_counter() - t_start
         st = tree.stats()
-        info = tree.route_info() if route == "learned" else None
+        info = tree.route_info() if route.startswith("learned") else None
         tree.close()
         mops = args.steps * n_batch / elapsed / 1e6
         log(f"[{route}] {args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s; "
@@ -274,15 +285,19 @@ def main():
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
     out_f = ea.DeviceBuffer(keys.size, np.uint8)
     routes = ["descent", "learned"] if args.route == "both" else [args.route]
+    if tuned:
+        routes += [r + "_tuned" for r in list(routes)]
     res = {r: run_route(r) for r in routes}
-    head = res["learned"] if "learned" in res else res[routes[0]]
+    head_name = next((r for r in ("learned_tuned", "learned") if r in res), routes[0])
+    head = res[head_name]
     st, elapsed, lat = head["st"], head["elapsed"], head["lat"]
     mops = head["mops"]
     total_ops = args.steps * n_batch
     same = None
-    if len(res) == 2 and res["descent"]["first"] is not None:
-        a_, b_ = res["descent"]["first"], res["learned"]["first"]
-        same = bool(np.array_equal(a_[0], b_[0]) and np.array_equal(a_[1], b_[1]))
+    firsts = [x["first"] for x in res.values() if x["first"] is not None]
+    if len(firsts) >= 2:
+        same = bool(all(np.array_equal(firsts[0][0], f[0]) and np.array_equal(firsts[0][1], f[1])
+                        for f in firsts[1:]))
 
     # Roofline of the read-only lookup kernel (the tree kernel every lookup takes).
     lookups = st["unique_ops"] - st["upserts"]
@@ -291,17 +306,17 @@ def main():
     lk_bytes = lookups / lk_launch * BYTES_LOOKUP
     achieved = lk_bytes / (lk_ms * 1e-3) / 1e9 if lk_ms > 0 else 0.0
     mix_bytes = 0.95 * BYTES_LOOKUP + 0.05 * BYTES_INSERT
-    kname = "k_lookup<%d,%d,%s>" % (args.fanout, args.lanes,
-                                    "true" if "learned" in res else "false")
+    head_fanout = tuned["fanout"] if head_name.endswith("_tuned") else args.fanout
+    head_lanes = tuned["lanes_per_op"] if head_name.endswith("_tuned") else args.lanes
+    kname = "k_lookup<%d,%d,%s>" % (head_fanout, head_lanes,
+                                    "true" if head_name.startswith("learned") else "false")
     # Measured L2<->memory bytes per launch of this kernel for this workload (rocprofv3 PMC
     # FETCH_SIZE / WRITE_SIZE, scaled by the copy-kernel calibration), committed with the profile.
     traffic = None
     tf = ROOT / "profiles" / "hbm_traffic_gamma_2p26_batch_2p22.json"
-    if tf.exists() and args.fanout == 64 and args.lanes == 16 and args.load_log2 == 26 \
-            and args.batch_log2 == 22:
-        rt = "learned" if "learned" in res else "descent"
-        kt = json.loads(tf.read_text())["routes"][rt]
-        kk = next((k for k in kt if k.startswith("k_lookup")), None)
+    if tf.exists() and args.load_log2 == 26 and args.batch_log2 == 22:
+        kt = json.loads(tf.read_text())["routes"].get(head_name)
+        kk = next((k for k in kt if k.startswith(kname)), None) if kt else None
         if kk:
             traffic = kt[kk]["read_bytes"] + kt[kk]["written_bytes"]
     roofline = {
@@ -368,14 +383,17 @@ def main():
         "config": {"workload": f"learned_gamma: bulk-load 2^{args.load_log2} Gamma(k=2, theta=2^56) "
                                f"keys, {args.steps} batches of 2^{args.batch_log2} ops, 95% lookup "
                                f"(90% hits) / 5% insert, router retrained every 8 batches",
-                   "global_batch": n_batch, "value_route": "learned" if "learned" in res else routes[0],
-                   "fanout": args.fanout, "lanes_per_op": args.lanes,
-                   "ops_per_group": args.group, "parallelism": "single GPU"},
+                   "global_batch": n_batch, "value_route": head_name,
+                   "defaults": {"fanout": args.fanout, "lanes_per_op": args.lanes,
+                                "ops_per_group": args.group},
+                   "tuned": tuned, "tuning_record": (str(Path(args.tuning_headline).relative_to(ROOT))
+                                                     if tuned else None),
+                   "parallelism": "single GPU"},
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "routes": {r: summary(r) for r in res},
-        "learned_over_plain": round(res["learned"]["mops"] / res["descent"]["mops"], 4)
-                              if len(res) == 2 else None,
+        "learned_over_plain": {k: round(res["learned" + k]["mops"] / res["descent" + k]["mops"], 4)
+                               for k in ("", "_tuned") if "learned" + k in res and "descent" + k in res},
         "learned_and_plain_results_identical": same,
         "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
                                    "root_splits", "deferred", "n_keys", "height")},

@@ -170,23 +170,30 @@ def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", required=True)
     ap.add_argument("--keys-log2", type=int, default=24)
+    ap.add_argument("--dist", choices=["normal", "gamma"], default="normal")
     ap.add_argument("--batches", type=int, default=48)
     ap.add_argument("--lo", type=float, default=10)
     ap.add_argument("--hi", type=float, default=22)
     ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
     n = 1 << args.keys_log2
-    load = w.normal_keys(n, args.seed)
+    if args.dist == "gamma":
+        dist, p0, p1, dname = w.DIST_GAMMA, 2.0, 2.0 ** 56, "Gamma(k=2, theta=2^56)"
+        load = w.gamma_keys(n, args.seed, p0, p1)
+    else:
+        dist, p0, p1, dname = w.DIST_NORMAL, 2.0 ** 62, 2.0 ** 58, "Normal(2^62, 2^58)"
+        load = w.normal_keys(n, args.seed, p0, p1)
     srt = w.sorted_copy(load)
     keys, vals, ops, offsets = uneven_sample(srt, args.batches, args.seed, args.lo, args.hi,
-                                             w.DIST_NORMAL, 2.0 ** 62, 2.0 ** 58)
+                                             dist, p0, p1)
     sample = Sample(load, w.values_for(load), keys, vals, ops, offsets)
     base = StmConfig(max_keys=n + int(offsets[-1]) + (1 << 20), max_batch=1 << int(args.hi),
                      route=ROUTE_DESCENT)
     rec = tune(sample, base, log=print)
     sample.free()
-    rec["workload"] = (f"2^{args.keys_log2} Normal(2^62, 2^58) keys, {args.batches} batches "
-                       f"log-uniform in [2^{args.lo:g}, 2^{args.hi:g}], 95/5 lookup/insert")
+    size = (f"of 2^{args.hi:g} ops" if args.lo == args.hi
+            else f"log-uniform in [2^{args.lo:g}, 2^{args.hi:g}]")
+    rec["workload"] = f"2^{args.keys_log2} {dname} keys, {args.batches} batches {size}, 95/5 lookup/insert"
     Path(args.out).write_text(json.dumps(rec, indent=1) + "\n")
     print(json.dumps({k: rec[k] for k in ("best", "best_mops", "default_mops")}))
 
