@@ -285,10 +285,11 @@ def main():
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
     out_f = ea.DeviceBuffer(keys.size, np.uint8)
     routes = ["descent", "learned"] if args.route == "both" else [args.route]
-    if tuned:
-        routes += [r + "_tuned" for r in list(routes)]
+    if tuned and "descent" in routes:  # the record was tuned with root-to-leaf descent
+        routes.append("descent_tuned")
     res = {r: run_route(r) for r in routes}
-    head_name = next((r for r in ("learned_tuned", "learned") if r in res), routes[0])
+    # The workload is the learned-routing configuration: `value` is its learned route.
+    head_name = "learned" if "learned" in res else routes[0]
     head = res[head_name]
     st, elapsed, lat = head["st"], head["elapsed"], head["lat"]
     mops = head["mops"]
@@ -386,8 +387,9 @@ def main():
                    "global_batch": n_batch, "value_route": head_name,
                    "defaults": {"fanout": args.fanout, "lanes_per_op": args.lanes,
                                 "ops_per_group": args.group},
-                   "tuned": tuned, "tuning_record": (str(Path(args.tuning_headline).relative_to(ROOT))
-                                                     if tuned else None),
+                   "descent_tuned": tuned,
+                   "tuning_record": (str(Path(args.tuning_headline).relative_to(ROOT))
+                                     if tuned else None),
                    "parallelism": "single GPU"},
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
