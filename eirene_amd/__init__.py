@@ -108,6 +108,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_sync": (C.c_int, [_P]),
         "stm_retrain": (C.c_int, [_P, C.c_uint32]),
         "stm_route_info": (C.c_int, [_P, _U64P, _U64P, _U64P]),
+        "stm_route_directory": (C.c_int, [_P, _P, _P, C.c_uint64, _U64P]),
         "stm_mlp_gradients": (C.c_int, [_P, _P, C.c_size_t, _P, _P, C.POINTER(C.c_float)]),
         # workload generators (host only)
         "stm_splitmix64": (C.c_uint64, [C.c_uint64]),
@@ -381,6 +382,15 @@ class StmTree:
         n, h, m = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(self._lib.stm_route_info(self._h, C.byref(n), C.byref(h), C.byref(m)))
         return {"leaves": n.value, "window_hits": h.value, "window_misses": m.value}
+
+    def route_directory(self):
+        """The leaf directory: (low fences uint64[L], leaf ids uint32[L]) in key order."""
+        n = C.c_uint64()
+        _check(self._lib.stm_route_directory(self._h, None, None, 0, C.byref(n)))
+        f = np.empty(n.value, dtype=np.uint64)
+        ids = np.empty(n.value, dtype=np.uint32)
+        _check(self._lib.stm_route_directory(self._h, _ptr(f), _ptr(ids), n.value, C.byref(n)))
+        return f, ids
 
     def stream(self) -> int:
         p = C.c_void_p()

@@ -298,7 +298,7 @@ int alloc_router(stm_tree* t) {
       dalloc(&r.segs, ROUTE_SEGMENTS) || dalloc(&r.dir, cap + 1) || dalloc(&r.rank_of, cap) ||
       dalloc(&r.lows, cap) || dalloc(&r.lows_tmp, cap) || dalloc(&r.sorted_lows, cap) ||
       dalloc(&r.order, cap) || dalloc(&r.order_tmp, cap) || dalloc(&r.hist, sort_hist_words(cap)) ||
-      dalloc(&r.partial, 4 * route_partial_blocks()))
+      dalloc(&r.partial, 4 * route_partial_blocks()) || dalloc(&r.dir_alt, cap + 1))
     return fail(STM_ERR_NOMEM, "learned-routing buffers for %llu nodes", (unsigned long long)cap);
   t->route_alloc = true;
   t->view.rank_of = r.rank_of;
@@ -310,7 +310,7 @@ int alloc_router(stm_tree* t) {
 void free_router(stm_tree* t) {
   RouteState& r = t->route;
   void* ps[] = {r.hdr, r.fences, r.lids, r.segs, r.dir, r.rank_of, r.lows, r.lows_tmp,
-                r.sorted_lows, r.order, r.order_tmp, r.hist, r.partial};
+                r.sorted_lows, r.order, r.order_tmp, r.hist, r.partial, r.dir_alt};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   t->route = RouteState();
@@ -1250,6 +1250,29 @@ int stm_route_info(stm_tree* t, uint64_t* n_leaves, uint64_t* window_hits, uint6
   *n_leaves = rh.L;
   *window_hits = hm[0];
   *window_misses = hm[1];
+  return STM_OK;
+}
+
+int stm_route_directory(stm_tree* t, uint64_t* fences, uint32_t* leaf_ids, uint64_t max,
+                        uint64_t* n_leaves) {
+  if (!t || !n_leaves || (!fences != !leaf_ids)) return fail(STM_ERR_INVALID, "null argument");
+  if (!t->route_valid) return fail(STM_ERR_STATE, "no router has been trained (stm_retrain)");
+  int rc = set_device(t);
+  if (rc) return rc;
+  RouteHeader rh{};
+  HIP_TRY(hipMemcpyAsync(&rh, t->route.hdr, sizeof(rh), hipMemcpyDeviceToHost, t->stream));
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  *n_leaves = rh.L;
+  if (!fences) return STM_OK;  // size query
+  if (rh.L > max) return fail(STM_ERR_INVALID, "directory holds %u leaves, room for %llu", rh.L,
+                              (unsigned long long)max);
+  if (rh.L) {
+    HIP_TRY(hipMemcpyAsync(fences, t->route.fences, sizeof(uint64_t) * rh.L, hipMemcpyDeviceToHost,
+                           t->stream));
+    HIP_TRY(hipMemcpyAsync(leaf_ids, t->route.lids, sizeof(uint32_t) * rh.L, hipMemcpyDeviceToHost,
+                           t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+  }
   return STM_OK;
 }
 

@@ -71,6 +71,155 @@ __global__ void k_route_parents(TreeView t, uint32_t n_nodes, DirEntry* __restri
   }
 }
 
+// ------------------------------------------------------------------ warm directory rewrite
+// Since the last directory build, leaves only split: every new leaf sits in the right-sibling
+// chain that starts at an entry flagged DIR_SPLIT and ends before the next entry's leaf.  The
+// rewrite counts each chain, ranks the chains by an exclusive scan and writes the new entries
+// in place of a sort of every node.
+constexpr int DS_BLOCK = 256;
+constexpr int DS_ITEMS = 16;
+constexpr int DS_TILE = DS_BLOCK * DS_ITEMS;
+
+__global__ void k_dir_count(TreeView t, const RouteHeader* __restrict__ rh,
+                            const DirEntry* __restrict__ dir, uint32_t* __restrict__ counts) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t L = rh->L;
+  if (r >= L) return;
+  const DirEntry e = dir[r];
+  uint32_t c = 1;
+  if (e.pflags & DIR_SPLIT) {
+    const uint32_t stop = r + 1 < L ? dir[r + 1].lid : NODE_NONE;
+    uint32_t node = e.lid;
+    for (uint32_t guard = 0; guard < t.capacity; ++guard) {
+      const uint64_t nx = t.meta[node].next;
+      if (nx == stop || nx >= t.capacity) break;
+      node = (uint32_t)nx;
+      ++c;
+    }
+  }
+  counts[r] = c;
+}
+
+STM_DEV uint32_t block_exclusive_u32(uint32_t v, uint32_t* s_w /*[4]*/, uint32_t& total) {
+  const int lane = lane_id();
+  const int wave = threadIdx.x / WAVE;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, WAVE);
+    if (lane >= off) x += y;
+  }
+  if (lane == WAVE - 1) s_w[wave] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < DS_BLOCK / WAVE; ++w) {
+    const uint32_t t = s_w[w];
+    if (w < wave) before += t;
+    total += t;
+  }
+  __syncthreads();
+  return before + x - v;
+}
+
+// Exclusive scan of counts[0 .. L) into offs (three kernels: tile sums, their scan, apply).
+__global__ __launch_bounds__(DS_BLOCK) void k_dir_scan_reduce(const RouteHeader* __restrict__ rh,
+                                                              const uint32_t* __restrict__ counts,
+                                                              uint32_t* __restrict__ tiles) {
+  __shared__ uint32_t s_w[DS_BLOCK / WAVE];
+  const uint32_t L = rh->L;
+  const uint32_t b = blockIdx.x;
+  if ((uint64_t)b * DS_TILE >= L) return;  // uniform over the block
+  uint32_t acc = 0;
+  for (int i = 0; i < DS_ITEMS; ++i) {
+    const uint64_t r = (uint64_t)b * DS_TILE + (uint64_t)i * DS_BLOCK + threadIdx.x;
+    if (r < L) acc += counts[r];
+  }
+  uint32_t total;
+  (void)block_exclusive_u32(acc, s_w, total);
+  if (threadIdx.x == 0) tiles[b] = total;
+}
+
+__global__ __launch_bounds__(DS_BLOCK) void k_dir_scan_tiles(RouteHeader* __restrict__ rh,
+                                                             uint32_t* __restrict__ tiles) {
+  __shared__ uint32_t s_w[DS_BLOCK / WAVE];
+  const uint32_t nt = (rh->L + DS_TILE - 1) / DS_TILE;
+  const uint32_t per = (nt + DS_BLOCK - 1) / DS_BLOCK;
+  const uint32_t b0 = min(threadIdx.x * per, nt);
+  const uint32_t b1 = min(b0 + per, nt);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; ++b) sum += tiles[b];
+  uint32_t total;
+  uint32_t run = block_exclusive_u32(sum, s_w, total);
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t c = tiles[b];
+    tiles[b] = run;
+    run += c;
+  }
+  if (threadIdx.x == 0) rh->L_next = total;
+}
+
+__global__ __launch_bounds__(DS_BLOCK) void k_dir_scan_apply(const RouteHeader* __restrict__ rh,
+                                                             const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ tiles,
+                                                             uint32_t* __restrict__ offs) {
+  __shared__ uint32_t s_w[DS_BLOCK / WAVE];
+  const uint32_t L = rh->L;
+  const uint32_t b = blockIdx.x;
+  if ((uint64_t)b * DS_TILE >= L) return;
+  uint32_t carry = tiles[b];
+  for (int i = 0; i < DS_ITEMS; ++i) {
+    const uint64_t r = (uint64_t)b * DS_TILE + (uint64_t)i * DS_BLOCK + threadIdx.x;
+    const uint32_t c = r < L ? counts[r] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_u32(c, s_w, total);
+    if (r < L) offs[r] = carry + ex;
+    carry += total;
+  }
+}
+
+// New entries of chain r at offs[r] ..: the chain's first leaf keeps its fence; the others take
+// their low fence from the node; all keep the old entry's parent hint.
+__global__ void k_dir_rewrite(TreeView t, const RouteHeader* __restrict__ rh,
+                              const DirEntry* __restrict__ dir, const uint32_t* __restrict__ counts,
+                              const uint32_t* __restrict__ offs, DirEntry* __restrict__ dir_alt,
+                              uint32_t* __restrict__ rank_of) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rh->L) return;
+  const DirEntry e = dir[r];
+  const uint32_t par = e.pflags & DIR_PAR_NONE;
+  const uint32_t base = offs[r];
+  const uint32_t c = counts[r];
+  uint32_t node = e.lid;
+  for (uint32_t j = 0; j < c; ++j) {
+    const uint64_t low = j == 0 ? e.fence : t.meta[node].low;
+    dir_alt[base + j] = DirEntry{low, node, par};
+    rank_of[node] = base + j;
+    if (j + 1 < c) node = (uint32_t)t.meta[node].next;
+  }
+}
+
+__global__ void k_dir_install(RouteHeader* __restrict__ rh, const DirEntry* __restrict__ dir_alt,
+                              DirEntry* __restrict__ dir, uint64_t* __restrict__ fences,
+                              uint32_t* __restrict__ lids) {
+  const uint32_t L = rh->L_next;
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < L) {
+    const DirEntry e = dir_alt[r];
+    dir[r] = e;
+    fences[r] = e.fence;
+    lids[r] = e.lid;
+  } else if (r == L) {
+    dir[L] = DirEntry{KEY_EMPTY, NODE_NONE, DIR_SPLIT | DIR_PAR_NONE};
+    fences[L] = KEY_EMPTY;
+  }
+}
+
+__global__ void k_dir_set_len(RouteHeader* __restrict__ rh) {
+  if (threadIdx.x == 0) rh->L = rh->L_next;
+}
+
 // Per-block sums of (x, y, xx, xy) with x = fence / 2^64 and y = rank.
 __global__ __launch_bounds__(RT_BLOCK) void k_route_root_partial(const uint64_t* __restrict__ fences,
                                                                  const RouteHeader* __restrict__ rh,
@@ -336,15 +485,34 @@ size_t route_partial_blocks() { return 256; }
 hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint32_t steps,
                        float lr, bool train, bool warm, hipStream_t s) {
   if (n_nodes == 0 || n_nodes > t.capacity) return hipErrorInvalidValue;
-  k_route_lows<<<(t.capacity + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.lows, rs.rank_of);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  e = radix_sort_pairs(rs.lows, rs.lows_tmp, rs.sorted_lows, rs.order_tmp, rs.order, n_nodes, 0, 64,
-                       rs.hist, s);
-  if (e != hipSuccess) return e;
-  k_route_directory<<<(n_nodes + 255) / 256, 256, 0, s>>>(rs.sorted_lows, rs.order, n_nodes, rs.hdr,
-                                                          rs.fences, rs.lids, rs.dir, rs.rank_of);
-  k_route_parents<<<(n_nodes + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.dir);
+  hipError_t e;
+  if (warm) {
+    // Rewrite the existing directory: leaves at most n_nodes, so every grid is sized by it.
+    const uint32_t g = (n_nodes + 255) / 256;
+    const uint32_t gt = (n_nodes + DS_TILE - 1) / DS_TILE;
+    uint32_t* counts = rs.order;
+    uint32_t* offs = rs.order_tmp;
+    uint32_t* tiles = rs.hist;
+    k_dir_count<<<g, 256, 0, s>>>(t, rs.hdr, rs.dir, counts);
+    k_dir_scan_reduce<<<gt, DS_BLOCK, 0, s>>>(rs.hdr, counts, tiles);
+    k_dir_scan_tiles<<<1, DS_BLOCK, 0, s>>>(rs.hdr, tiles);
+    k_dir_scan_apply<<<gt, DS_BLOCK, 0, s>>>(rs.hdr, counts, tiles, offs);
+    k_dir_rewrite<<<g, 256, 0, s>>>(t, rs.hdr, rs.dir, counts, offs, rs.dir_alt, rs.rank_of);
+    k_dir_install<<<(n_nodes + 1 + 255) / 256, 256, 0, s>>>(rs.hdr, rs.dir_alt, rs.dir, rs.fences, rs.lids);
+    k_dir_set_len<<<1, 64, 0, s>>>(rs.hdr);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  } else {
+    k_route_lows<<<(t.capacity + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.lows, rs.rank_of);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = radix_sort_pairs(rs.lows, rs.lows_tmp, rs.sorted_lows, rs.order_tmp, rs.order, n_nodes, 0, 64,
+                         rs.hist, s);
+    if (e != hipSuccess) return e;
+    k_route_directory<<<(n_nodes + 255) / 256, 256, 0, s>>>(rs.sorted_lows, rs.order, n_nodes, rs.hdr,
+                                                            rs.fences, rs.lids, rs.dir, rs.rank_of);
+    k_route_parents<<<(n_nodes + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.dir);
+  }
   k_route_root_partial<<<(uint32_t)route_partial_blocks(), RT_BLOCK, 0, s>>>(rs.fences, rs.hdr, rs.partial);
   k_route_root_final<<<1, 64, 0, s>>>(rs.partial, (uint32_t)route_partial_blocks(), rs.fences, rs.hdr,
                                       warm ? 1u : 0u);

@@ -33,6 +33,8 @@ static_assert(sizeof(SegModel) == 256, "segment record is 256 bytes");
 struct RouteHeader {
   uint32_t L;   // leaves in the directory
   uint32_t S;   // segments
+  uint32_t L_next;  // leaves of the directory being rewritten
+  uint32_t pad;
   double a, b;  // root line: segment ~ a * key + b
   double ls_slope;  // least-squares slope of rank over key (reported, not used to route)
 };
@@ -91,6 +93,7 @@ struct RouteState {
   uint64_t *lows = nullptr, *lows_tmp = nullptr, *sorted_lows = nullptr;
   uint32_t *order = nullptr, *order_tmp = nullptr, *hist = nullptr;
   double* partial = nullptr;
+  DirEntry* dir_alt = nullptr;  // capacity + 1: the directory being rewritten (warm retrain)
 };
 
 size_t route_partial_blocks();

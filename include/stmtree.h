@@ -208,6 +208,11 @@ int stm_get_stream(stm_tree* tree, void** stream);
  * current model's +/-2 window contains it (hits) or not (misses). */
 int stm_retrain(stm_tree* tree, uint32_t train_steps);
 int stm_route_info(stm_tree* tree, uint64_t* n_leaves, uint64_t* window_hits, uint64_t* window_misses);
+/* Copies the leaf directory (low fence and leaf id of every leaf, in key order) to host
+ * arrays of max entries; *n_leaves receives its size.  STM_ERR_INVALID when max is smaller;
+ * with both arrays null only the size is returned. */
+int stm_route_directory(stm_tree* tree, uint64_t* fences, uint32_t* leaf_ids, uint64_t max,
+                        uint64_t* n_leaves);
 
 /* One forward/backward pass of the router's segment MLP (1 -> 16 -> 1, ReLU, fp32) on the
  * device, for checking the training kernels: loss = mean of 0.5 * (mlp(x_i) - t_i)^2 and its

@@ -59,6 +59,33 @@ def test_learned_routing_results_match_oracle(oracle_mod):
     t.close()
 
 
+def test_warm_directory_rewrite_matches_cold_build():
+    """A periodic retrain rewrites the directory from the split chains; after insert-heavy
+    batches it must list exactly the leaves a rebuild from every node lists."""
+    import eirene_amd as ea
+    from eirene_amd import workloads as w
+    t, load, vals = gamma_tree(ea, w, 1 << 18, 7, retrain_every=1, train_steps=32)
+    t.retrain()  # the directory of the loaded tree; later retrains rewrite it
+    f0, _ = t.route_directory()
+    srt = w.sorted_copy(load)
+    probe = np.ascontiguousarray(srt[:1024])
+    for b in range(3):
+        keys, bv, ops = w.mixed_batch(srt, 1 << 18, 70 + b, 0.2, 0.5, w.DIST_GAMMA, 2.0, 2.0 ** 56)
+        t.run_batch(keys, bv, ops)
+        # the retrain runs before each batch: a lookup-only batch rewrites the directory
+        # and splits nothing
+        t.run_batch(probe, np.zeros_like(probe), np.full(len(probe), ea.OP_GET, dtype=np.uint8))
+        fw, iw = t.route_directory()
+        assert np.all(fw[1:] > fw[:-1])
+    assert len(fw) > len(f0) + 1000, (len(f0), len(fw))
+    assert t.check() == 0
+    t.retrain(0)  # rebuild from every node
+    fc, ic = t.route_directory()
+    np.testing.assert_array_equal(fw, fc)
+    np.testing.assert_array_equal(iw, ic)
+    t.close()
+
+
 def test_trained_router_beats_untrained():
     import eirene_amd as ea
     from eirene_amd import workloads as w
