@@ -394,8 +394,8 @@ def main():
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "routes": {r: summary(r) for r in res},
-        "learned_over_plain": {k: round(res["learned" + k]["mops"] / res["descent" + k]["mops"], 4)
-                               for k in ("", "_tuned") if "learned" + k in res and "descent" + k in res},
+        "learned_over_plain": {("descent" + k): round(res["learned"]["mops"] / res["descent" + k]["mops"], 4)
+                               for k in ("", "_tuned") if "learned" in res and "descent" + k in res},
         "learned_and_plain_results_identical": same,
         "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
                                    "root_splits", "deferred", "n_keys", "height")},

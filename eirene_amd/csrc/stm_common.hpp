@@ -116,6 +116,14 @@ struct TreeView {
 // invalidate (buffer_inv sc1), which only matter for non-atomic accesses; none exist on
 // this path.  The read-only lookup kernel uses plain loads and runs only when no writer
 // is active; kernel boundaries order it against the transactional kernels.
+// Plain load through a pointer known to address device global memory.  Pointers read from
+// memory (a batch descriptor's) are generic to the compiler, which then emits flat loads; this
+// lets it use global loads with a scalar base and immediate offsets.
+template <typename T>
+STM_DEV T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 STM_DEV uint64_t ld_relaxed(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -10,7 +10,8 @@
 //   k_scan_hist exclusive scan per digit over tiles, digit totals
 //   k_scatter   stable in-tile ranking with 64-lane ballots (a wavefront owns a contiguous
 //               segment, so (wave, round, lane) order is input order), LDS staging into
-//               digit order, then coalesced runs to global memory.
+//               digit order (keys, then payloads, through one buffer), then coalesced runs
+//               to global memory.
 // The element count and the input keys can come from a device-side batch descriptor, and
 // the tile kernels loop over tiles with a fixed grid, so one launch sequence (and one
 // captured hipGraph) serves any batch size up to the one the scratch was sized for.  The
@@ -27,6 +28,7 @@ constexpr int SORT_BLOCK = 256;
 constexpr int SORT_WAVES = SORT_BLOCK / WAVE;  // 4
 constexpr int RADIX = 256;
 constexpr uint32_t SORT_GRID_CAP = 2048;       // workgroups of the tile kernels
+constexpr uint32_t SCAN_REG = 8;               // tiles per thread the histogram scan keeps in registers
 
 // Exclusive scan of one u32 per thread over a 256-thread block; returns the block total.
 STM_DEV uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t& total) {
@@ -50,6 +52,14 @@ STM_DEV uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, u
   }
   __syncthreads();
   return wbase + x - v;
+}
+
+// Digit (8 bits from bit `shift`, uniform) of a key with 32-bit operations only.
+STM_DEV uint32_t digit_of(uint64_t k, int shift) {
+  const uint32_t lo = (uint32_t)k, hi = (uint32_t)(k >> 32);
+  const uint32_t w = shift < 32 ? __builtin_amdgcn_alignbit(hi, lo, (uint32_t)shift)
+                                : hi >> (shift - 32);
+  return w & (RADIX - 1);
 }
 
 struct SortRange {
@@ -85,7 +95,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_hist(const uint64_t* __restrict_
 #pragma unroll 4
     for (int r = 0; r < ITEMS; ++r) {
       const uint64_t e = base + (uint64_t)r * SORT_BLOCK + threadIdx.x;
-      if (e < rg.n) atomicAdd(&s_h[wave][(uint32_t)(kp[e] >> shift) & (RADIX - 1)], 1u);
+      if (e < rg.n) atomicAdd(&s_h[wave][digit_of(gload(kp + e), shift)], 1u);
     }
     __syncthreads();
     const int d = threadIdx.x;  // SORT_BLOCK == RADIX
@@ -107,14 +117,31 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scan_hist(uint32_t* __restrict__
   const uint32_t per = (ntiles + SORT_BLOCK - 1) / SORT_BLOCK;
   const uint32_t b0 = min(threadIdx.x * per, ntiles);
   const uint32_t b1 = min(b0 + per, ntiles);
-  uint32_t sum = 0;
-  for (uint32_t b = b0; b < b1; ++b) sum += row[b];
   uint32_t total;
-  uint32_t run = block_exclusive_scan_256(sum, s_tmp, total);
-  for (uint32_t b = b0; b < b1; ++b) {
-    uint32_t c = row[b];
-    row[b] = run;
-    run += c;
+  if (per <= SCAN_REG) {
+    // Up to SCAN_REG tiles per thread held in registers: one read and one write of the row.
+    uint32_t c[SCAN_REG];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_REG; ++j) {
+      c[j] = b0 + j < b1 ? row[b0 + j] : 0u;
+      sum += c[j];
+    }
+    uint32_t run = block_exclusive_scan_256(sum, s_tmp, total);
+#pragma unroll
+    for (int j = 0; j < SCAN_REG; ++j) {
+      if (b0 + j < b1) row[b0 + j] = run;
+      run += c[j];
+    }
+  } else {
+    uint32_t sum = 0;
+    for (uint32_t b = b0; b < b1; ++b) sum += row[b];
+    uint32_t run = block_exclusive_scan_256(sum, s_tmp, total);
+    for (uint32_t b = b0; b < b1; ++b) {
+      uint32_t c = row[b];
+      row[b] = run;
+      run += c;
+    }
   }
   if (threadIdx.x == 0) totals[blockIdx.x] = total;
 }
@@ -133,8 +160,10 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
   __shared__ uint32_t s_tbase[RADIX];             // global position of each digit's first element
   __shared__ uint32_t s_gbase[RADIX];             // global position of this tile's run of each digit
   __shared__ uint32_t s_tmp[SORT_WAVES];
-  __shared__ uint64_t s_keys[TILE];
-  __shared__ uint32_t s_idx[TILE];
+  // Keys, then payloads, staged in digit order through one buffer: 8 bytes per element of
+  // LDS instead of 12 lets four 4096-element tiles share a CU.
+  __shared__ uint64_t s_stage[TILE];
+  uint32_t* s_idx = reinterpret_cast<uint32_t*>(s_stage);
 
   const SortRange rg = sort_range(desc, n_host, keys_in, first_pass);
   const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
@@ -154,22 +183,34 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
     s_gbase[threadIdx.x] = s_tbase[threadIdx.x] + hist[(uint64_t)threadIdx.x * stride + b];
     __syncthreads();
 
+    // Registers bound the tile: keys and ranks live through the ranking, payloads are loaded
+    // only once the keys are staged.
     uint64_t k[ITEMS];
-    uint32_t id[ITEMS];
     uint32_t rk[ITEMS];
+    // Per-tile copies of the lane's indices: opaque to the compiler, so it recomputes the
+    // per-item offsets in each tile instead of hoisting 32 of them into registers.
+    uint32_t lane0 = (uint32_t)wave * SEG + lane;  // tile-local index of item 0
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(lane0), "+v"(tid));
+    // Item r of a lane is tile element lane0 + r * WAVE.  Loads of a partial tile clamp the
+    // index into the tile instead of branching; the clamped values are never used.
+    const uint32_t last = tile_n - 1;
+    const uint64_t* kt = kin + tile0;
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-      const uint32_t le = (uint32_t)wave * SEG + (uint32_t)r * WAVE + lane;  // tile-local
-      const bool valid = le < tile_n;
-      const uint64_t e = tile0 + le;
-      k[r] = valid ? kin[e] : KEY_EMPTY;
-      uint32_t pl = (uint32_t)e;
-      if (valid && !idx_in && ops) {
-        const uint32_t o = ops[e];
-        pl |= (o > 3u ? 1u : o) << POS_BITS;
-      }
-      id[r] = valid ? (idx_in ? idx_in[e] : pl) : 0u;
-      const uint32_t d = (uint32_t)(k[r] >> shift) & (RADIX - 1);
+      const uint32_t le = lane0 + (uint32_t)r * WAVE;
+      const uint64_t v = gload(kt + min(le, last));
+      k[r] = le < tile_n ? v : KEY_EMPTY;
+    }
+    asm volatile("" : "+v"(lane0));  // each phase recomputes its offsets
+    uint32_t dk[(ITEMS + 3) / 4];  // the items' digits, four to a register
+#pragma unroll
+    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dk[q] = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
+      const uint32_t d = digit_of(k[r], shift);
+      dk[r / 4] |= d << (8 * (r % 4));
       // Lanes holding the same digit: intersect 8 bit-ballots.
       uint64_t peers = ballot(valid);
 #pragma unroll
@@ -180,6 +221,9 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
       }
       uint32_t base = valid ? s_wcnt[wave][d] : 0u;
       rk[r] = base + (uint32_t)__popcll(peers & lt);
+      // Materialise the rank here: left to itself the compiler keeps the 64-bit peer mask of
+      // every item alive across the barrier below, which costs occupancy.
+      asm volatile("" : "+v"(rk[r]));
       const bool leader = valid && (lane == 63 - __clzll(peers));
       if (leader) s_wcnt[wave][d] = base + (uint32_t)__popcll(peers);
     }
@@ -196,23 +240,62 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
       s_wcnt[3][d] = c0 + c1 + c2;
     }
     __syncthreads();
+    asm volatile("" : "+v"(lane0));
+    // rk becomes the element's position in the tile's digit order
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-      const uint32_t le = (uint32_t)wave * SEG + (uint32_t)r * WAVE + lane;
-      if (le < tile_n) {
-        const uint32_t d = (uint32_t)(k[r] >> shift) & (RADIX - 1);
-        const uint32_t p = s_dbase[d] + s_wcnt[wave][d] + rk[r];
-        s_keys[p] = k[r];
-        s_idx[p] = id[r];
+      if (lane0 + (uint32_t)r * WAVE < tile_n) {
+        const uint32_t d = (dk[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
+        rk[r] += s_dbase[d] + s_wcnt[wave][d];
+        s_stage[rk[r]] = k[r];
+      }
+    }
+    asm volatile("" : "+v"(lane0));
+    // Payloads: the input's, or on the first pass the position with the op code above it.
+    uint32_t id[ITEMS];
+    if (idx_in) {
+      const uint32_t* it = idx_in + tile0;
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) id[r] = gload(it + min(lane0 + (uint32_t)r * WAVE, last));
+    } else if (ops) {
+      const uint8_t* ot = ops + tile0;
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) {
+        const uint32_t le = lane0 + (uint32_t)r * WAVE;
+        const uint32_t o = gload(ot + min(le, last));
+        id[r] = ((uint32_t)tile0 + le) | ((o > 3u ? 1u : o) << POS_BITS);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) id[r] = (uint32_t)tile0 + lane0 + (uint32_t)r * WAVE;
+    }
+    __syncthreads();
+    // The digit of every position a thread writes, four to a register, for the payload phase.
+    uint32_t dg[(ITEMS + 3) / 4];
+#pragma unroll
+    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dg[q] = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
+      if (p < tile_n) {
+        const uint64_t kk = s_stage[p];
+        const uint32_t d = digit_of(kk, shift);
+        dg[r / 4] |= d << (8 * (r % 4));
+        keys_out[s_gbase[d] + (p - s_dbase[d])] = kk;
       }
     }
     __syncthreads();
-    for (uint32_t p = threadIdx.x; p < tile_n; p += SORT_BLOCK) {
-      const uint64_t kk = s_keys[p];
-      const uint32_t d = (uint32_t)(kk >> shift) & (RADIX - 1);
-      const uint64_t dst = (uint64_t)s_gbase[d] + (p - s_dbase[d]);
-      keys_out[dst] = kk;
-      idx_out[dst] = s_idx[p];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      if (lane0 + (uint32_t)r * WAVE < tile_n) s_idx[rk[r]] = id[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
+      if (p < tile_n) {
+        const uint32_t d = (dg[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
+        idx_out[s_gbase[d] + (p - s_dbase[d])] = s_idx[p];
+      }
     }
     __syncthreads();
   }
