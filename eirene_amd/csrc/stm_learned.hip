@@ -355,33 +355,82 @@ STM_DEV float mlp_accumulate(const float* p, float x, float t, float scale, floa
   return 0.5f * e * e * scale;
 }
 
-// Sum over the wavefront of g[0..MLP_PARAMS) into s_out (LDS, MLP_PARAMS floats).  The
-// LDS hand-off between lanes of one wavefront only needs the writes to have completed
-// (wave_order waits for them); no workgroup barrier is involved, so wavefronts of a block
-// may leave at different times.
-STM_DEV void wave_reduce_grad(const float* g, float* s_part /*[64][MLP_PARAMS]*/, float* s_out) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int q = 0; q < MLP_PARAMS; ++q) s_part[lane * MLP_PARAMS + q] = g[q];
-  wave_order();
-  if (lane < MLP_PARAMS) {
-    float acc = 0.0f;
-    for (int l = 0; l < WAVE; ++l) acc += s_part[l * MLP_PARAMS + lane];
-    s_out[lane] = acc;
+// Gradient reduction without LDS: a recursive-halving reduce-scatter over the wavefront.  At
+// each of six stages (lane distance 32 .. 1) a lane keeps one half of its values, sends the
+// other half to its partner and adds what the partner sent; after the last stage lane L holds
+// the wavefront's sum of parameter param_of_lane(L) (lanes that map past MLP_PARAMS hold
+// padding).  52 shuffles instead of 64 x 49 LDS words per step.
+constexpr int RS_C0 = MLP_PARAMS;        // 49
+constexpr int RS_H0 = (RS_C0 + 1) / 2;   // 25 values after the distance-32 stage
+constexpr int RS_H1 = (RS_H0 + 1) / 2;   // 13
+constexpr int RS_H2 = (RS_H1 + 1) / 2;   // 7
+constexpr int RS_H3 = (RS_H2 + 1) / 2;   // 4
+constexpr int RS_H4 = (RS_H3 + 1) / 2;   // 2
+constexpr int RS_H5 = (RS_H4 + 1) / 2;   // 1
+
+__host__ __device__ constexpr int param_of_lane(int lane) {
+  return ((lane & 32) ? RS_H0 : 0) + ((lane & 16) ? RS_H1 : 0) + ((lane & 8) ? RS_H2 : 0) +
+         ((lane & 4) ? RS_H3 : 0) + ((lane & 2) ? RS_H4 : 0) + ((lane & 1) ? RS_H5 : 0);
+}
+__host__ __device__ constexpr int lane_of_param(int q) {
+  int lane = 0, c = RS_C0;
+  for (int dist = 32; dist >= 1; dist >>= 1) {
+    const int h = (c + 1) / 2;
+    if (q >= h) {
+      lane |= dist;
+      q -= h;
+    }
+    c = h;
   }
-  wave_order();
+  return lane;
+}
+static_assert(param_of_lane(lane_of_param(0)) == 0 && param_of_lane(lane_of_param(48)) == 48 &&
+                  param_of_lane(lane_of_param(24)) == 24 && param_of_lane(lane_of_param(25)) == 25,
+              "reduce-scatter lane map");
+
+template <int C, int DIST>
+STM_DEV void rs_stage(float* a, int lane) {
+  constexpr int H = (C + 1) / 2;
+  const bool up = (lane & DIST) != 0;
+#pragma unroll
+  for (int q = 0; q < H; ++q) {
+    const float lo = a[q];
+    const float hi = q + H < C ? a[q + H] : 0.0f;
+    const float send = up ? lo : hi;
+    const float keep = up ? hi : lo;
+    a[q] = keep + __shfl_xor(send, DIST, WAVE);
+  }
+}
+
+// g[0..MLP_PARAMS) per lane -> the wavefront sum of parameter param_of_lane(lane).
+STM_DEV float wave_reduce_scatter(float* g) {
+  const int lane = lane_id();
+  rs_stage<RS_C0, 32>(g, lane);
+  rs_stage<RS_H0, 16>(g, lane);
+  rs_stage<RS_H1, 8>(g, lane);
+  rs_stage<RS_H2, 4>(g, lane);
+  rs_stage<RS_H3, 2>(g, lane);
+  rs_stage<RS_H4, 1>(g, lane);
+  return g[0];
+}
+
+// Every parameter from the lane that holds it, as wave-uniform values (scalar registers).
+STM_DEV void broadcast_params(float pv, float* p) {
+#pragma unroll
+  for (int q = 0; q < MLP_PARAMS; ++q)
+    p[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pv), lane_of_param(q)));
 }
 
 constexpr int TRAIN_WAVES = 4;
-constexpr uint32_t TRAIN_MAX_PAIRS = 2048;
+constexpr int TRAIN_CACHE = 8;  // pairs per lane kept in registers
+constexpr uint32_t TRAIN_MAX_PAIRS = WAVE * TRAIN_CACHE;  // 512 pairs per segment and step
 
-// One wavefront per segment: full-batch Adam for `steps` steps.
+// One wavefront per segment: full-batch Adam for `steps` steps.  Lane L owns parameter
+// param_of_lane(L): its Adam moments and its value; the forward and backward passes read the
+// parameters as wave-uniform values.
 __global__ __launch_bounds__(TRAIN_WAVES * WAVE) void k_route_train(
     const RouteHeader* __restrict__ rh, const uint64_t* __restrict__ fences,
     SegModel* __restrict__ segs, uint32_t steps, float lr) {
-  __shared__ float s_part[TRAIN_WAVES][WAVE * MLP_PARAMS];
-  __shared__ float s_grad[TRAIN_WAVES][MLP_PARAMS];
-  __shared__ float s_p[TRAIN_WAVES][MLP_PARAMS];
   const int wave = threadIdx.x / WAVE;
   const int lane = lane_id();
   const uint32_t s = blockIdx.x * TRAIN_WAVES + wave;
@@ -389,61 +438,72 @@ __global__ __launch_bounds__(TRAIN_WAVES * WAVE) void k_route_train(
   SegModel& m = segs[s];
   const uint32_t cnt = m.cnt;
   if (cnt < 2 || m.inv_w == 0.0f) return;
-  if (lane < MLP_PARAMS) s_p[wave][lane] = m.p[lane];
-  wave_order();
-  float am = 0.0f, av = 0.0f;  // Adam moments of parameter `lane`
+  const int own = param_of_lane(lane);
+  const bool owner = own < MLP_PARAMS;
+  float pv = owner ? m.p[own] : 0.0f;
+  float am = 0.0f, av = 0.0f;  // Adam moments of the owned parameter
   const float inv_n = 1.0f / (float)cnt;
   const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
   float b1t = 1.0f, b2t = 1.0f;
+  // A segment of at most TRAIN_MAX_PAIRS leaves trains on all its pairs, held in registers.
+  // A larger one (the linear root leaves dense key ranges with several times the mean)
+  // trains on every span-th pair, at most TRAIN_MAX_PAIRS of them, from an offset that moves
+  // every step, so the largest segments cost no more per step than a full one.
+  const uint32_t span = (cnt + TRAIN_MAX_PAIRS - 1) / TRAIN_MAX_PAIRS;
+  const bool cached = span == 1;
+  const uint32_t used = cached ? cnt : 0;  // pairs per step of a cached segment
+  float xc[TRAIN_CACHE], tc[TRAIN_CACHE];
+#pragma unroll
+  for (int c = 0; c < TRAIN_CACHE; ++c) {
+    const uint32_t i = (uint32_t)lane + (uint32_t)c * WAVE;
+    xc[c] = cached && i < used ? seg_x(m, fences[m.r0 + i]) : 0.0f;
+    tc[c] = (float)i * inv_n;
+  }
   for (uint32_t it = 0; it < steps; ++it) {
     float p[MLP_PARAMS];
-#pragma unroll
-    for (int q = 0; q < MLP_PARAMS; ++q) p[q] = s_p[wave][q];
+    broadcast_params(pv, p);
     float g[MLP_PARAMS];
 #pragma unroll
     for (int q = 0; q < MLP_PARAMS; ++q) g[q] = 0.0f;
-    // Large segments train on a strided subsample of at most TRAIN_MAX_PAIRS pairs per
-    // step, shifted every step, so one segment cannot stall the launch.
-    const uint32_t used = min(cnt, TRAIN_MAX_PAIRS);
-    const uint32_t span = cnt / used;
-    const uint32_t shift = span > 1 ? (it * 2654435761u) % span : 0;
-    const float scale = 1.0f / (float)used;
-    for (uint32_t j = lane; j < used; j += WAVE) {
-      const uint32_t i = span > 1 ? j * span + shift : j;
-      const float x = seg_x(m, fences[m.r0 + i]);
-      const float tg = (float)i * inv_n;
-      mlp_accumulate(p, x, tg, scale, g);
+    if (cached) {
+      const float scale = 1.0f / (float)used;
+#pragma unroll
+      for (int c = 0; c < TRAIN_CACHE; ++c)
+        if ((uint32_t)lane + (uint32_t)c * WAVE < used) mlp_accumulate(p, xc[c], tc[c], scale, g);
+    } else {
+      const uint32_t shift = (it * 2654435761u) % span;
+      const uint32_t n_sub = (cnt - shift + span - 1) / span;  // pairs shift, shift + span, ...
+      const float scale = 1.0f / (float)n_sub;
+      for (uint32_t j = lane; j < n_sub; j += WAVE) {
+        const uint32_t i = j * span + shift;
+        mlp_accumulate(p, seg_x(m, fences[m.r0 + i]), (float)i * inv_n, scale, g);
+      }
     }
-    wave_reduce_grad(g, s_part[wave], s_grad[wave]);
-    if (lane < MLP_PARAMS) {
-      const float gr = s_grad[wave][lane];
-      b1t *= beta1;
-      b2t *= beta2;
-      am = beta1 * am + (1.0f - beta1) * gr;
-      av = beta2 * av + (1.0f - beta2) * gr * gr;
-      const float step_lr = lr * (1.0f - 0.9f * (float)it / (float)steps);
-      s_p[wave][lane] = s_p[wave][lane] - step_lr * (am / (1.0f - b1t)) / (sqrtf(av / (1.0f - b2t)) + eps);
-    }
-    wave_order();
+    const float gr = wave_reduce_scatter(g);
+    b1t *= beta1;
+    b2t *= beta2;
+    am = beta1 * am + (1.0f - beta1) * gr;
+    av = beta2 * av + (1.0f - beta2) * gr * gr;
+    const float step_lr = lr * (1.0f - 0.9f * (float)it / (float)steps);
+    if (owner) pv = pv - step_lr * (am / (1.0f - b1t)) / (sqrtf(av / (1.0f - b2t)) + eps);
   }
-  if (lane < MLP_PARAMS) m.p[lane] = s_p[wave][lane];
+  if (owner) m.p[own] = pv;
 }
 
 // Loss and gradient of the mean of 0.5 * (mlp(x_i) - t_i)^2 over n pairs (one wavefront).
 __global__ __launch_bounds__(WAVE) void k_mlp_grad(const float* __restrict__ x, const float* __restrict__ t,
                                                    uint32_t n, const float* __restrict__ params,
                                                    float* __restrict__ grad, float* __restrict__ loss) {
-  __shared__ float s_part[WAVE * MLP_PARAMS];
-  __shared__ float s_grad[MLP_PARAMS];
+  const int own = param_of_lane(lane_id());
   float p[MLP_PARAMS];
-  for (int q = 0; q < MLP_PARAMS; ++q) p[q] = params[q];
+  broadcast_params(own < MLP_PARAMS ? params[own] : 0.0f, p);
   float g[MLP_PARAMS];
   for (int q = 0; q < MLP_PARAMS; ++q) g[q] = 0.0f;
   float l = 0.0f;
   const float inv_n = n ? 1.0f / (float)n : 0.0f;
   for (uint32_t i = lane_id(); i < n; i += WAVE) l += mlp_accumulate(p, x[i], t[i], inv_n, g);
-  wave_reduce_grad(g, s_part, s_grad);
-  if (lane_id() < MLP_PARAMS) grad[lane_id()] = s_grad[lane_id()];
+  const float gr = wave_reduce_scatter(g);
+  if (own < MLP_PARAMS) grad[own] = gr;
   for (int off = 32; off >= 1; off >>= 1) l += __shfl_xor(l, off, WAVE);
   if (lane_id() == 0) *loss = l;
 }

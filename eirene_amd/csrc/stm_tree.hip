@@ -206,6 +206,9 @@ STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, 
   return ok;
 }
 
+// Stripe of the statistics counters a wavefront adds to.
+STM_DEV uint32_t wave_stripe() { return blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; }
+
 template <int F, int G, bool LEARNED>
 __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __restrict__ ukey,
                                                 const uint8_t* __restrict__ uop,
@@ -233,7 +236,9 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   // Every lane group walks one contiguous span of at least `chunk` sorted groups, so its
   // cached leaf, parent and directory cursor carry over for the whole span.
   const uint64_t span = max<uint64_t>(chunk, (U + ngroups - 1) / ngroups);
-  if (grp0 * span >= U) return;
+  // Groups past the end skip the walk but stay for the wave's statistics reduction.
+  const bool active = grp0 * span < U;
+  if (!LEARNED && !active) return;
   if (desc) {
     out_vals = desc->out_vals;
     out_found = desc->out_found;
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   uint32_t hits = 0, misses = 0;
   if constexpr (LEARNED) rv = route_view(rh, fences, lids, segs, t.dir);
 
-  {
+  if (active) {
   const uint32_t i0 = (uint32_t)(grp0 * span);
   const uint32_t i1 = (uint32_t)min<uint64_t>((uint64_t)i0 + span, U);
   for (uint32_t base = i0; base < i1; base += G) {
@@ -381,9 +386,16 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   }
   }  // span
   if constexpr (LEARNED) {
-    if (g == 0) {
-      stat_add(t.stats, ST_ROUTE_HITS, (uint32_t)grp0, hits);
-      stat_add(t.stats, ST_ROUTE_MISSES, (uint32_t)grp0, misses);
+    // One atomic per wavefront and counter: the group leaders' counts summed by readlane.
+    uint32_t h = 0, m = 0;
+#pragma unroll
+    for (int q = 0; q < WAVE; q += G) {
+      h += (uint32_t)__builtin_amdgcn_readlane((int)hits, q);
+      m += (uint32_t)__builtin_amdgcn_readlane((int)misses, q);
+    }
+    if (lane == 0) {
+      if (h) stat_add(t.stats, ST_ROUTE_HITS, wave_stripe(), h);
+      if (m) stat_add(t.stats, ST_ROUTE_MISSES, wave_stripe(), m);
     }
   }
 }
@@ -935,7 +947,6 @@ STM_DEV bool run_upsert(const TreeView& t, uint64_t key, uint64_t val, uint32_t 
   return true;
 }
 
-STM_DEV uint32_t wave_stripe() { return blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; }
 
 STM_DEV void flush_counters(const TreeView& t, const TxCounters& c) {
   if (lane_id() == 0) {
