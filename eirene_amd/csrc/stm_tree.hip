@@ -13,6 +13,7 @@
 //     try-lock commit of the write set with 64-bit CAS, abort and retry with backoff,
 //     preemptive split of full inner nodes on the way down
 //     (reference: src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:162-181).
+#include <atomic>
 #include "stm_common.hpp"
 #include "stm_internal.hpp"
 #include "stm_learned.hpp"
@@ -267,14 +268,23 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
     // sorted position of its first waiting op in a second one that overlaps the walk.
     const uint32_t j = base + g;
     const bool valid = j < i1;
-    const uint8_t op_l = valid ? uop[j] : UOP_SKIP;
-    const uint64_t key_l = valid ? ukey[j] : 0;
-    const uint32_t gs_l = valid ? gstart[j] : 0;
-    const uint32_t np_l = valid ? npend[j] : 0;
+    // Loads from an index clamped into the span (i1 > base), selected afterwards: unconditional
+    // loads issue together instead of one guarded round trip after another.
+    const uint32_t jc = valid ? j : i1 - 1;
+    const uint8_t op_raw = uop[jc];
+    const uint64_t key_raw = ukey[jc];
+    const uint32_t gs_raw = gstart[jc];
+    const uint32_t np_raw = npend[jc];
+    __builtin_amdgcn_sched_barrier(0);  // keep all four loads ahead of their first use
+    const uint8_t op_l = valid ? op_raw : UOP_SKIP;
+    const uint64_t key_l = valid ? key_raw : 0;
+    const uint32_t gs_l = valid ? gs_raw : 0;
+    const uint32_t np_l = valid ? np_raw : 0;
     const bool mine = op_l == OP_GET;
     // Merged INSERTs are routed too: their leaf is the upsert kernel's starting hint.
     const bool walk = mine || (uleaf && op_l == OP_INSERT);
-    const uint32_t pos_l = (mine && np_l) ? pos_of(sidx[gs_l]) : 0;
+    const uint32_t sx_raw = sidx[gs_l];  // gs_l is a sorted position of the batch
+    const uint32_t pos_l = (mine && np_l) ? pos_of(sx_raw) : 0;
     uint64_t vidx_l = 0;  // value slot of op base + g
     uint32_t leaf_l = NODE_NONE, par_l = NODE_NONE;
     bool found_l = false;
@@ -972,9 +982,10 @@ STM_DEV void flush_counters(const TreeView& t, const TxCounters& c) {
 template <int F, int G>
 STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uint64_t v_l,
                                    uint32_t leaf_l, uint64_t ver_l, uint32_t gs_l, uint32_t np_l,
-                                   const uint32_t* __restrict__ sidx,
+                                   uint32_t p0_l, const uint32_t* __restrict__ sidx,
                                    uint64_t* __restrict__ out_vals,
-                                   uint8_t* __restrict__ out_found, TxCounters& c) {
+                                   uint8_t* __restrict__ out_found, TxCounters& c,
+                                   uint64_t*& pend_lock, uint64_t& pend_word) {
   constexpr int K = F / G;
   constexpr int NG = WAVE / G;
   const int lane = lane_id();
@@ -986,6 +997,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   const uint32_t leaf = (uint32_t)__shfl((int)leaf_l, sl, WAVE);
   const uint32_t gs = (uint32_t)__shfl((int)gs_l, sl, WAVE);
   const uint32_t np = (uint32_t)__shfl((int)np_l, sl, WAVE);
+  const uint32_t p0 = (uint32_t)__shfl((int)p0_l, sl, WAVE);
   const uint64_t ver = shfl64(ver_l, sl);
   const bool act = src >= 0 && leaf < t.capacity;
   NodeMeta* m = t.meta + (act ? leaf : 0u);
@@ -1015,6 +1027,11 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
       lv[e] = 0;
     }
   }
+  // The previous round's unlock (lane 0 of each group): one wait covers the acknowledgement
+  // of that round's content stores and this round's test-and-set and loads.
+  wave_order();
+  if (pend_lock) st_relaxed(pend_lock, pend_word);
+  pend_lock = nullptr;
   word = shfl64(word, gbase);
   const bool held = act && !(word & 1ull);
   if (ballot(held && word != ver)) {
@@ -1073,11 +1090,15 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
     }
   }
   if (insert && g == 0) st_relaxed(&m->info, make_info(count + 1, 0));
-  wave_order();  // every content store is acknowledged before the new versions
-  if (held && g == 0) st_relaxed(&m->lock, commit ? word + 2 : word);
+  // The new version is stored once every content store is acknowledged: in the next round
+  // after its wait, or after the last round.
+  if (held && g == 0) {
+    pend_lock = &m->lock;
+    pend_word = commit ? word + 2 : word;
+  }
   if (commit) {
     for (uint32_t p = (uint32_t)g; p < np; p += G) {
-      const uint32_t pos_o = pos_of(sidx[gs + p]);
+      const uint32_t pos_o = p == 0 ? p0 : pos_of(sidx[gs + p]);
       out_vals[pos_o] = old;
       out_found[pos_o] = found ? 1 : 0;
     }
@@ -1124,17 +1145,31 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
   const uint32_t e1 = min(e0 + per, N);
   TxCounters c;
   uint32_t upserts = 0;
+  uint64_t* pend_lock = nullptr;  // a lock this lane still has to release, and its new word
+  uint64_t pend_word = 0;
   for (uint32_t base = e0; base < e1; base += WAVE) {
     const uint32_t e = base + lane_id();
     const bool mine = e < e1;
-    const uint32_t gi = mine ? ilist[e] : 0;
+    // Unconditional loads from an index clamped into the range (e1 > base), selected
+    // afterwards, so the gathers issue together.
+    const uint32_t gi = ilist[mine ? e : e1 - 1];
     uint64_t todo = ballot(mine);
-    const uint64_t k_l = mine ? ukey[gi] : 0;
-    const uint64_t v_l = mine ? uval[gi] : 0;
-    const uint32_t gs_l = mine ? gstart[gi] : 0;
-    const uint32_t np_l = mine ? npend[gi] : 0;
-    const uint32_t leaf_l = mine ? uleaf[gi] : NODE_NONE;
-    const uint64_t ver_l = mine ? uver[gi] : 1ull;
+    const uint64_t k_raw = ukey[gi];
+    const uint64_t v_raw = uval[gi];
+    const uint32_t gs_raw = gstart[gi];
+    const uint32_t np_raw = npend[gi];
+    const uint32_t leaf_raw = uleaf[gi];
+    const uint64_t ver_raw = uver[gi];
+    const uint64_t k_l = mine ? k_raw : 0;
+    const uint64_t v_l = mine ? v_raw : 0;
+    const uint32_t gs_l = mine ? gs_raw : 0;
+    const uint32_t np_l = mine ? np_raw : 0;
+    const uint32_t leaf_l = mine ? leaf_raw : NODE_NONE;
+    const uint64_t ver_l = mine ? ver_raw : 1ull;
+    // The first answered op's input position, loaded with the op data rather than inside
+    // the round (gs_l is a sorted position of the batch).
+    const uint32_t sx_raw = sidx[gs_l];
+    const uint32_t p0_l = (mine && np_l) ? pos_of(sx_raw) : 0u;
     for (uint64_t rest = todo; rest;) {
       // group q takes the q-th remaining op of the round
       int my_src = -1;
@@ -1145,7 +1180,8 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
         if (lane_id() / G == q) my_src = l;
       }
       const uint64_t done = fast_leaf_upserts<F, G>(t, my_src, k_l, v_l, leaf_l, ver_l, gs_l,
-                                                    np_l, sidx, out_vals, out_found, c);
+                                                    np_l, p0_l, sidx, out_vals, out_found, c,
+                                                    pend_lock, pend_word);
       upserts += (uint32_t)__popcll(done);
       todo &= ~done;
     }
@@ -1156,6 +1192,8 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
       if ((todo >> lane_id()) & 1ull) slist[at + (uint32_t)__popcll(todo & lanemask_lt())] = gi;
     }
   }
+  wave_order();
+  if (pend_lock) st_relaxed(pend_lock, pend_word);
   flush_counters(t, c);
   if (lane_id() == 0) stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
 }
@@ -1435,7 +1473,21 @@ hipError_t launch_lookup(const TreeView& t, const LookupArgs& a, hipStream_t s) 
 
 template <int F, int G>
 static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
-  k_upsert_fast<F, G><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx,
+  // Each wavefront takes an equal contiguous share of the merged INSERTs and works through it
+  // in rounds of WAVE / G: a grid of exactly the wavefronts the chip holds at once finishes
+  // in one generation of rounds instead of two.
+  static std::atomic<int> resident{0};  // blocks of this instantiation the device holds at once
+  if (resident.load(std::memory_order_relaxed) == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_upsert_fast<F, G>, 256, 0);
+    if (e != hipSuccess) return e;
+    resident.store(std::max(1, cus * per_cu), std::memory_order_relaxed);
+  }
+  const uint32_t blocks = std::min<uint32_t>(a.blocks, (uint32_t)resident.load(std::memory_order_relaxed));
+  k_upsert_fast<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx,
                                                 a.ilist, a.uleaf, a.uver, a.counters, a.slist,
                                                 a.out_vals, a.out_found, a.desc);
   return hipGetLastError();
