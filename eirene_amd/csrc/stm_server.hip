@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -63,7 +64,83 @@ double now_s() {
 
 constexpr uint64_t BYTES_PER_OP = 8 + 8 + 1 + 8 + 1;  // key, value, op in; value, found out
 constexpr uint64_t BYTES_PER_PAIR = 16;
-constexpr unsigned SCATTER_THREADS = 8;
+constexpr unsigned POOL_THREADS = 8;        // helpers for large copy-ins and answer scatters
+constexpr uint64_t PARALLEL_BYTES = 4u << 20;  // a copy this large is split over the pool
+
+// A fixed set of helper threads that run the parts of one job together with the calling
+// thread.  Jobs from different threads run one after another.
+class Pool {
+ public:
+  explicit Pool(unsigned n) {
+    for (unsigned i = 0; i < n; ++i) th_.emplace_back([this] { work(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  unsigned size() const { return (unsigned)th_.size() + 1; }
+  // Runs f(0) .. f(parts - 1) on the helpers and the caller; returns when all are done.
+  void run(unsigned parts, const std::function<void(unsigned)>& f) {
+    std::lock_guard<std::mutex> job(job_);
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      f_ = &f;
+      parts_ = parts;
+      next_ = 0;
+      done_ = 0;
+      ++gen_;
+    }
+    cv_.notify_all();
+    take();
+    std::unique_lock<std::mutex> lk(m_);
+    cv_done_.wait(lk, [&] { return done_ == parts_; });
+    f_ = nullptr;
+  }
+
+ private:
+  void take() {
+    for (;;) {
+      unsigned p;
+      const std::function<void(unsigned)>* f;
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        if (!f_ || next_ >= parts_) return;
+        p = next_++;
+        f = f_;
+      }
+      (*f)(p);
+      bool last;
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        last = ++done_ == parts_;
+      }
+      if (last) cv_done_.notify_all();
+    }
+  }
+  void work() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+      }
+      take();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_, job_;
+  std::condition_variable cv_, cv_done_;
+  const std::function<void(unsigned)>* f_ = nullptr;
+  unsigned parts_ = 0, next_ = 0, done_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
 
 struct Req {
   uint64_t ticket = 0;
@@ -138,6 +215,7 @@ struct stm_server {
   double st_busy = 0;
 
   std::condition_variable cv_copies;  // a slot's input copies finished
+  std::unique_ptr<Pool> pool;
 
   // Admission of queued requests into the forming slot (mu held): reserves their places and
   // returns them in `out`; the caller copies them in (copy_in) after releasing the lock.
@@ -166,13 +244,21 @@ struct stm_server {
   // Copies admitted requests into their slots (mu not held), then releases them.
   void copy_in(std::vector<std::pair<Slot*, Req*>>& adm) {
     if (adm.empty()) return;
-    for (auto& sr : adm) {
-      Slot* s = sr.first;
-      Req* r = sr.second;
-      std::memcpy(s->h_keys + r->off, r->keys, r->n * 8);
-      std::memcpy(s->h_vals + r->off, r->vals, r->n * 8);
-      std::memcpy(s->h_ops + r->off, r->ops, r->n);
-    }
+    uint64_t ops = 0;
+    for (auto& sr : adm) ops += sr.second->n;
+    // A backlog admitted at once (a slot came free under load) is copied by the pool.
+    const unsigned k = ops * 17 > PARALLEL_BYTES && adm.size() > 1 ? pool->size() : 1u;
+    auto part = [&adm, k](unsigned q) {
+      for (size_t i = q; i < adm.size(); i += k) {
+        Slot* s = adm[i].first;
+        Req* r = adm[i].second;
+        std::memcpy(s->h_keys + r->off, r->keys, r->n * 8);
+        std::memcpy(s->h_vals + r->off, r->vals, r->n * 8);
+        std::memcpy(s->h_ops + r->off, r->ops, r->n);
+      }
+    };
+    if (k == 1) part(0);
+    else pool->run(k, part);
     {
       std::lock_guard<std::mutex> lk(mu);
       for (auto& sr : adm) sr.first->copies -= 1;
@@ -262,9 +348,9 @@ struct stm_server {
         s->error = std::string("answer copy: ") + hipGetErrorString(e);
       }
       if (s->status == STM_OK) {
-        // Large slots are scattered by several threads (each takes every k-th request).
+        // Large slots are scattered by the pool (each part takes every k-th request).
         const uint64_t bytes = s->n * 9 + s->pairs * 16;
-        const unsigned k = bytes > (8u << 20) ? SCATTER_THREADS : 1u;
+        const unsigned k = bytes > PARALLEL_BYTES && s->reqs.size() > 1 ? pool->size() : 1u;
         auto scatter = [s, k](unsigned part) {
           for (size_t i = part; i < s->reqs.size(); i += k) {
             Req* r = s->reqs[i];
@@ -276,14 +362,8 @@ struct stm_server {
             }
           }
         };
-        if (k == 1) {
-          scatter(0);
-        } else {
-          std::vector<std::thread> th;
-          for (unsigned part = 1; part < k; ++part) th.emplace_back(scatter, part);
-          scatter(0);
-          for (auto& x : th) x.join();
-        }
+        if (k == 1) scatter(0);
+        else pool->run(k, scatter);
       }
       const double t = now_s();
       std::vector<std::pair<Slot*, Req*>> adm;
@@ -410,6 +490,7 @@ int stm_server_create(stm_tree* tree, const stm_server_config* cfg_in, stm_serve
     }
     s->free_slots.push_back(p);
   }
+  s->pool = std::make_unique<Pool>(POOL_THREADS);
   s->runner = std::thread([s] { s->run_loop(); });
   s->completer = std::thread([s] { s->complete_loop(); });
   *out = sv.release();
