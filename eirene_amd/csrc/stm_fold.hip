@@ -8,94 +8,46 @@
 // the latest earlier INSERT of the group, otherwise by the tree; the merged op is
 // INSERT(last inserted value) if the group has an insert, GET otherwise.
 //
-// Here one segmented scan replaces RLE + scan + the thread-per-group loop, so skewed
-// (Zipf) groups are spread over many threads and no run count travels to the host.
-// The scan state of a prefix is (has_head, last head position, first and last insert
-// position since that head, number of heads).  Three kernels: tile reduce, scan of the
-// tile aggregates, apply.  Ops that the fold answers (an earlier insert of the same key
-// exists) are written to the outputs here; the first op of each group up to and
-// including its first insert waits for the tree ("npend" ops from "gstart").
+// Here one scan replaces RLE + scan + the thread-per-group loop, so skewed (Zipf) groups
+// are spread over many threads and no run count travels to the host.  Because the groups
+// are contiguous in the sorted order, the whole fold follows from three running values of a
+// prefix: the number of group heads, the position of the last head and the position of the
+// last INSERT.  For element i with head h(i), the latest earlier insert of its key is the
+// prefix's last insert if it is >= h(i); the group's merged op is INSERT iff the last insert
+// up to its end is >= its head.  A wavefront owns 512 consecutive elements as 8 rounds of
+// 64 (coalesced loads and group outputs); within a round the three values of every lane's
+// prefix come from two ballots (heads, inserts): a popcount and a highest set bit.  Three
+// kernels: tile reduce, scan of the tile aggregates, apply.  Ops that the fold answers (an
+// earlier insert of the same key exists) are written to the outputs here; the first op of
+// each group up to and including its first insert waits for the tree ("npend" ops from
+// "gstart").
 #include "stm_common.hpp"
 #include "stm_internal.hpp"
 
 namespace stm {
 
 constexpr int FOLD_BLOCK = 256;
-constexpr int FOLD_ITEMS = 16;
-constexpr int FOLD_TILE = FOLD_BLOCK * FOLD_ITEMS;
 constexpr int FOLD_WAVES = FOLD_BLOCK / WAVE;
+constexpr int FOLD_ROUNDS = 8;                     // rounds of WAVE elements per wavefront
+constexpr int FOLD_CHUNK = FOLD_ROUNDS * WAVE;     // elements of one wavefront
+constexpr int FOLD_TILE = FOLD_WAVES * FOLD_CHUNK;  // elements of one block
 
-struct Seg {
-  int32_t h;      // 1 if the range contains a group head
-  int32_t head;   // position of the last head in the range (-1 if none)
-  int32_t first;  // first insert after the last head (or in the whole range if h == 0)
-  int32_t last;   // last insert after the last head (or in the whole range if h == 0)
-  uint32_t c;     // heads in the range
+struct Acc {
+  uint32_t c;    // group heads in the range
+  int32_t head;  // position of the last head (-1 if none)
+  int32_t ins;   // position of the last INSERT (-1 if none)
 };
 
-STM_DEV Seg seg_identity() { return Seg{0, -1, -1, -1, 0u}; }
-
-STM_DEV Seg seg_combine(const Seg& a, const Seg& b) {
-  Seg r;
-  r.h = a.h | b.h;
-  r.head = b.h ? b.head : a.head;
-  r.first = b.h ? b.first : (a.first >= 0 ? a.first : b.first);
-  r.last = b.h ? b.last : max(a.last, b.last);
-  r.c = a.c + b.c;
-  return r;
+STM_DEV Acc acc_identity() { return Acc{0u, -1, -1}; }
+STM_DEV Acc acc_combine(const Acc& a, const Acc& b) {
+  return Acc{a.c + b.c, max(a.head, b.head), max(a.ins, b.ins)};
 }
 
-STM_DEV Seg seg_shfl_up(const Seg& s, int off) {
-  Seg r;
-  r.h = __shfl_up(s.h, off, WAVE);
-  r.head = __shfl_up(s.head, off, WAVE);
-  r.first = __shfl_up(s.first, off, WAVE);
-  r.last = __shfl_up(s.last, off, WAVE);
-  r.c = __shfl_up(s.c, off, WAVE);
-  return r;
+// Position of the highest lane set in m within a round starting at rb, or `none`.
+STM_DEV int32_t last_set(uint64_t m, uint32_t rb, int32_t none) {
+  return m ? (int32_t)(rb + 63u - (uint32_t)__clzll((long long)m)) : none;
 }
 
-// Exclusive scan of one Seg per thread over the block; also returns the block total.
-STM_DEV Seg block_exclusive_seg(const Seg& v, Seg* s_w /*[FOLD_WAVES]*/, Seg& total) {
-  const int lane = lane_id();
-  const int wave = threadIdx.x / WAVE;
-  Seg x = v;
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    Seg y = seg_shfl_up(x, off);
-    if (lane >= off) x = seg_combine(y, x);
-  }
-  Seg ex = seg_shfl_up(x, 1);
-  if (lane == 0) ex = seg_identity();
-  if (lane == WAVE - 1) s_w[wave] = x;
-  __syncthreads();
-  Seg wp = seg_identity();
-  total = seg_identity();
-#pragma unroll
-  for (int w = 0; w < FOLD_WAVES; ++w) {
-    Seg t = s_w[w];
-    if (w < wave) wp = seg_combine(wp, t);
-    total = seg_combine(total, t);
-  }
-  __syncthreads();
-  return seg_combine(wp, ex);
-}
-
-STM_DEV Seg seg_element(const FoldArgs& a, uint32_t i, uint64_t key, uint64_t prev_key) {
-  const bool head = (i == 0) || key != prev_key;
-  const bool ins = op_of(a.sidx[i]) == OP_INSERT;  // op code packed by the sort
-  Seg e;
-  e.h = head ? 1 : 0;
-  e.head = head ? (int32_t)i : -1;
-  e.first = ins ? (int32_t)i : -1;
-  e.last = e.first;
-  e.c = head ? 1u : 0u;
-  return e;
-}
-
-// A tile is processed in FOLD_ITEMS rounds of FOLD_BLOCK consecutive elements: thread t
-// owns element round * FOLD_BLOCK + t, so every global access of a round is coalesced and
-// the group outputs of a round land in consecutive slots.
 // With a batch descriptor the element count and the caller's arrays come from device memory
 // (one launch sequence serves every batch size).
 STM_DEV void fold_resolve(FoldArgs& a) {
@@ -108,43 +60,134 @@ STM_DEV void fold_resolve(FoldArgs& a) {
   a.out_found = d->out_found;
 }
 
-__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_reduce(FoldArgs a, Seg* tile_agg) {
-  __shared__ Seg s_w[FOLD_WAVES];
+// A wavefront's chunk: element r * WAVE + lane of the chunk in k[r], sx[r] of that lane.
+struct Chunk {
+  uint32_t c0;  // position of the chunk's first element
+  uint64_t k[FOLD_ROUNDS];
+  uint32_t sx[FOLD_ROUNDS];
+  uint64_t pk, nk;  // keys just before and just after the chunk (when they exist)
+};
+
+STM_DEV void load_chunk(const FoldArgs& a, uint32_t c0, Chunk& ch) {
+  const int lane = lane_id();
+  ch.c0 = c0;
+#pragma unroll
+  for (int r = 0; r < FOLD_ROUNDS; ++r) {
+    const uint32_t i = c0 + (uint32_t)r * WAVE + lane;
+    ch.k[r] = i < a.n ? a.skeys[i] : 0;
+    ch.sx[r] = i < a.n ? a.sidx[i] : 0;
+  }
+  ch.pk = (c0 > 0 && c0 <= a.n) ? a.skeys[c0 - 1] : 0;
+  ch.nk = (c0 + FOLD_CHUNK < a.n) ? a.skeys[c0 + FOLD_CHUNK] : 0;
+}
+
+// Head and INSERT flags of round r for this lane, with the key after it.
+STM_DEV void round_flags(const FoldArgs& a, const Chunk& ch, int r, bool& valid, bool& head,
+                         bool& ins, uint64_t& next) {
+  const int lane = lane_id();
+  const uint32_t i = ch.c0 + (uint32_t)r * WAVE + lane;
+  valid = i < a.n;
+  const uint64_t up = shfl64(ch.k[r], lane > 0 ? lane - 1 : 0);
+  const uint64_t prev_last = r > 0 ? readlane64(ch.k[r > 0 ? r - 1 : 0], WAVE - 1) : ch.pk;
+  const uint64_t prev = lane > 0 ? up : prev_last;
+  const uint64_t down = shfl64(ch.k[r], lane < WAVE - 1 ? lane + 1 : WAVE - 1);
+  const uint64_t next_first = r + 1 < FOLD_ROUNDS ? readlane64(ch.k[r + 1 < FOLD_ROUNDS ? r + 1 : r], 0) : ch.nk;
+  next = lane < WAVE - 1 ? down : next_first;
+  head = valid && (i == 0 || ch.k[r] != prev);
+  ins = valid && op_of(ch.sx[r]) == OP_INSERT;
+}
+
+// The chunk's aggregate (wave-uniform).
+STM_DEV Acc chunk_acc(const FoldArgs& a, const Chunk& ch) {
+  Acc t = acc_identity();
+#pragma unroll
+  for (int r = 0; r < FOLD_ROUNDS; ++r) {
+    bool valid, head, ins;
+    uint64_t next;
+    round_flags(a, ch, r, valid, head, ins, next);
+    const uint32_t rb = ch.c0 + (uint32_t)r * WAVE;
+    const uint64_t hm = ballot(head), im = ballot(ins);
+    t.c += (uint32_t)__popcll(hm);
+    t.head = last_set(hm, rb, t.head);
+    t.ins = last_set(im, rb, t.ins);
+  }
+  return t;
+}
+
+// Exclusive prefix over the block's wavefronts of their chunk aggregates, and the total.
+STM_DEV Acc block_wave_prefix(const Acc& w, Acc* s_w /*[FOLD_WAVES]*/, Acc& total) {
+  const int wave = threadIdx.x / WAVE;
+  if (lane_id() == 0) s_w[wave] = w;
+  __syncthreads();
+  Acc before = acc_identity();
+  total = acc_identity();
+#pragma unroll
+  for (int q = 0; q < FOLD_WAVES; ++q) {
+    const Acc t = s_w[q];
+    if (q < wave) before = acc_combine(before, t);
+    total = acc_combine(total, t);
+  }
+  __syncthreads();
+  return before;
+}
+
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_reduce(FoldArgs a, Acc* tile_agg) {
+  __shared__ Acc s_w[FOLD_WAVES];
   fold_resolve(a);
   const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
+  const uint32_t wave = threadIdx.x / WAVE;
   for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
-    const uint32_t tile0 = b * FOLD_TILE;
-    Seg acc = seg_identity();
-    for (int r = 0; r < FOLD_ITEMS; ++r) {
-      const uint32_t r0 = tile0 + r * FOLD_BLOCK;
-      if (r0 >= a.n) break;  // uniform over the block
-      const uint32_t i = r0 + threadIdx.x;
-      Seg e = seg_identity();
-      if (i < a.n) e = seg_element(a, i, a.skeys[i], i ? a.skeys[i - 1] : 0);
-      Seg total;
-      (void)block_exclusive_seg(e, s_w, total);
-      acc = seg_combine(acc, total);
-    }
-    if (threadIdx.x == 0) tile_agg[b] = acc;
+    Chunk ch;
+    load_chunk(a, b * FOLD_TILE + wave * FOLD_CHUNK, ch);
+    Acc total;
+    (void)block_wave_prefix(chunk_acc(a, ch), s_w, total);
+    if (threadIdx.x == 0) tile_agg[b] = total;
   }
 }
 
-__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, FoldArgs a,
+// Exclusive scan of one Acc per thread over the block; also returns the block total.
+STM_DEV Acc block_exclusive_acc(const Acc& v, Acc* s_w /*[FOLD_WAVES]*/, Acc& total) {
+  const int lane = lane_id();
+  const int wave = threadIdx.x / WAVE;
+  Acc x = v;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const Acc y{(uint32_t)__shfl_up((int)x.c, off, WAVE), __shfl_up(x.head, off, WAVE),
+                __shfl_up(x.ins, off, WAVE)};
+    if (lane >= off) x = acc_combine(y, x);
+  }
+  Acc ex{(uint32_t)__shfl_up((int)x.c, 1, WAVE), __shfl_up(x.head, 1, WAVE), __shfl_up(x.ins, 1, WAVE)};
+  if (lane == 0) ex = acc_identity();
+  if (lane == WAVE - 1) s_w[wave] = x;
+  __syncthreads();
+  Acc wp = acc_identity();
+  total = acc_identity();
+#pragma unroll
+  for (int w = 0; w < FOLD_WAVES; ++w) {
+    const Acc t = s_w[w];
+    if (w < wave) wp = acc_combine(wp, t);
+    total = acc_combine(total, t);
+  }
+  __syncthreads();
+  return acc_combine(wp, ex);
+}
+
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Acc* tile_agg, FoldArgs a,
                                                                 BatchCounters* counters) {
-  __shared__ Seg s_w[FOLD_WAVES];
+  __shared__ Acc s_w[FOLD_WAVES];
   fold_resolve(a);
   const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
   const uint32_t per = (ntiles + FOLD_BLOCK - 1) / FOLD_BLOCK;
-  const uint32_t b0 = threadIdx.x * per;
+  const uint32_t b0 = min(threadIdx.x * per, ntiles);
   const uint32_t b1 = min(b0 + per, ntiles);
-  Seg s = seg_identity();
-  for (uint32_t b = b0; b < b1; ++b) s = seg_combine(s, tile_agg[b]);
-  Seg total;
-  Seg run = block_exclusive_seg(s, s_w, total);
+  Acc s = acc_identity();
+  for (uint32_t b = b0; b < b1; ++b) s = acc_combine(s, tile_agg[b]);
+  Acc total;
+  Acc run = block_exclusive_acc(s, s_w, total);
   for (uint32_t b = b0; b < b1; ++b) {
-    Seg t = tile_agg[b];
+    const Acc t = tile_agg[b];
     tile_agg[b] = run;  // becomes the exclusive prefix of tile b
-    run = seg_combine(run, t);
+    run = acc_combine(run, t);
   }
   if (threadIdx.x == 0) {
     counters->n_unique = total.c;
@@ -156,86 +199,91 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Seg* tile_agg, F
   }
 }
 
-__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Seg* tile_prefix) {
-  __shared__ Seg s_w[FOLD_WAVES];
+__global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Acc* tile_prefix) {
+  __shared__ Acc s_w[FOLD_WAVES];
   __shared__ uint32_t s_ins[FOLD_TILE];  // the tile's merged INSERT groups
   __shared__ uint32_t s_nins, s_base;
   fold_resolve(a);
   const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
+  const uint32_t wave = threadIdx.x / WAVE;
+  const uint64_t lt = lanemask_lt();
+  const uint64_t le = lt | (1ull << lane_id());
   for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
-  const uint32_t tile0 = b * FOLD_TILE;
-  if (threadIdx.x == 0) s_nins = 0;
-  __syncthreads();
-  Seg carry = tile_prefix[b];  // state of every element before this round
-  for (int r = 0; r < FOLD_ITEMS; ++r) {
-    const uint32_t r0 = tile0 + r * FOLD_BLOCK;
-    if (r0 >= a.n) break;  // uniform over the block
-    const uint32_t i = r0 + threadIdx.x;
-    const bool valid = i < a.n;
-    uint64_t k = 0, next = 0;
-    Seg e = seg_identity();
-    if (valid) {
-      k = a.skeys[i];
-      next = (i + 1 < a.n) ? a.skeys[i + 1] : k;
-      e = seg_element(a, i, k, i ? a.skeys[i - 1] : 0);
-    }
-    Seg total;
-    const Seg before = seg_combine(carry, block_exclusive_seg(e, s_w, total));
-    carry = seg_combine(carry, total);
-    bool append = false;
-    uint32_t g = 0;
-    if (valid) {
-      const int32_t last_before = e.h ? -1 : before.last;  // latest earlier insert of this key
-      if (a.lastins) a.lastins[i] = e.first >= 0 ? (int32_t)i : last_before;
-      if (a.out_vals && last_before >= 0) {
-        const uint32_t pos = pos_of(a.sidx[i]);
-        a.out_vals[pos] = a.vals[pos_of(a.sidx[last_before])];
-        a.out_found[pos] = 1;
-      }
-      const Seg x = seg_combine(before, e);
-      const bool group_end = (i + 1 == a.n) || next != k;
-      if (group_end) {
-        g = x.c - 1;
-        a.ukey[g] = k;
-        if (a.gstart) {
-          a.gstart[g] = (uint32_t)x.head;
-          a.npend[g] = (uint32_t)((x.first >= 0 ? x.first : (int32_t)i) - x.head + 1);
+    if (threadIdx.x == 0) s_nins = 0;
+    Chunk ch;
+    load_chunk(a, b * FOLD_TILE + wave * FOLD_CHUNK, ch);
+    Acc total;
+    // State before this wavefront's chunk (block_wave_prefix syncs s_nins too).
+    Acc carry = acc_combine(tile_prefix[b], block_wave_prefix(chunk_acc(a, ch), s_w, total));
+#pragma unroll
+    for (int r = 0; r < FOLD_ROUNDS; ++r) {
+      bool valid, is_head, is_ins;
+      uint64_t next;
+      round_flags(a, ch, r, valid, is_head, is_ins, next);
+      const uint32_t rb = ch.c0 + (uint32_t)r * WAVE;
+      const int32_t i = (int32_t)(rb + lane_id());
+      const uint64_t hm = ballot(is_head), im = ballot(is_ins);
+      // This lane's prefix: heads up to and including it, its head, the last insert before it.
+      const uint32_t c_incl = carry.c + (uint32_t)__popcll(hm & le);
+      const int32_t head = last_set(hm & le, rb, carry.head);
+      const int32_t ins_before = last_set(im & lt, rb, carry.ins);
+      bool append = false;
+      uint32_t g = c_incl - 1;
+      if (valid) {
+        const uint64_t key = ch.k[r];
+        const int32_t last_before = ins_before >= head ? ins_before : -1;  // latest earlier insert
+        if (a.lastins) a.lastins[i] = is_ins ? i : last_before;
+        if (a.out_vals && last_before >= 0) {
+          const uint32_t pos = pos_of(ch.sx[r]);
+          a.out_vals[pos] = a.vals[pos_of(a.sidx[last_before])];
+          a.out_found[pos] = 1;
         }
-        uint8_t op;
-        if (k == KEY_EMPTY) {
-          op = UOP_SKIP;
-          atomicAdd(&a.counters->n_bad, (uint32_t)(i - x.head + 1));
-        } else if (x.last >= 0) {
-          op = OP_INSERT;
-          a.uval[g] = a.vals[pos_of(a.sidx[x.last])];
-          append = true;
-        } else {
-          op = OP_GET;
+        if (a.gstart && is_ins && last_before < 0) a.npend[g] = (uint32_t)(i - head + 1);
+        const int32_t ins_incl = is_ins ? i : ins_before;
+        const bool group_end = (uint32_t)i + 1 == a.n || next != key;
+        if (group_end) {
+          a.ukey[g] = key;
+          if (a.gstart) a.gstart[g] = (uint32_t)head;
+          const int32_t last_in = ins_incl >= head ? ins_incl : -1;
+          uint8_t op;
+          if (key == KEY_EMPTY) {
+            op = UOP_SKIP;
+            atomicAdd(&a.counters->n_bad, (uint32_t)(i - head + 1));
+          } else if (last_in >= 0) {
+            op = OP_INSERT;
+            a.uval[g] = a.vals[pos_of(a.sidx[last_in])];
+            append = true;
+          } else {
+            op = OP_GET;
+            if (a.gstart) a.npend[g] = (uint32_t)(i - head + 1);
+          }
+          a.uop[g] = op;
         }
-        a.uop[g] = op;
       }
+      carry.c += (uint32_t)__popcll(hm);
+      carry.head = last_set(hm, rb, carry.head);
+      carry.ins = last_set(im, rb, carry.ins);
+      const uint64_t want = ballot(append);
+      uint32_t wb = 0;
+      if (lane_id() == 0 && want) wb = atomicAdd(&s_nins, (uint32_t)__popcll(want));  // LDS
+      wb = (uint32_t)__shfl((int)wb, 0, WAVE);
+      if (append) s_ins[wb + (uint32_t)__popcll(want & lt)] = g;
     }
-    const uint64_t want = ballot(append);
-    uint32_t wb = 0;
-    if (lane_id() == 0 && want) wb = atomicAdd(&s_nins, (uint32_t)__popcll(want));  // LDS
-    wb = (uint32_t)__shfl((int)wb, 0, WAVE);
-    if (append) s_ins[wb + (uint32_t)__popcll(want & lanemask_lt())] = g;
-  }
-  // One counter update per tile (a hot atomic per wavefront round would serialize the
-  // kernel on one L2 channel), then the tile's INSERT groups go to the batch's list.
-  __syncthreads();
-  if (threadIdx.x == 0) s_base = s_nins ? atomicAdd(&a.counters->n_upserts, s_nins) : 0u;
-  __syncthreads();
-  if (a.ilist)
-    for (uint32_t k = threadIdx.x; k < s_nins; k += FOLD_BLOCK) a.ilist[s_base + k] = s_ins[k];
-  __syncthreads();
+    // One counter update per tile (a hot atomic per wavefront round would serialize the
+    // kernel on one L2 channel), then the tile's INSERT groups go to the batch's list.
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_nins ? atomicAdd(&a.counters->n_upserts, s_nins) : 0u;
+    __syncthreads();
+    if (a.ilist)
+      for (uint32_t k = threadIdx.x; k < s_nins; k += FOLD_BLOCK) a.ilist[s_base + k] = s_ins[k];
+    __syncthreads();
   }  // tiles
 }
 
 size_t fold_scratch_bytes(uint64_t n) {
   uint64_t ntiles = (n + FOLD_TILE - 1) / FOLD_TILE;
   if (ntiles == 0) ntiles = 1;
-  return (size_t)ntiles * sizeof(Seg);
+  return (size_t)ntiles * sizeof(Acc);
 }
 
 // a.n is the element count, or with a.desc the largest count the scratch was sized for.
@@ -243,7 +291,7 @@ hipError_t fold_batch(const FoldArgs& a, hipStream_t stream) {
   const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
   if (a.n == 0) return hipSuccess;
   const uint32_t grid = std::min<uint32_t>(ntiles, 2048);
-  Seg* tiles = (Seg*)a.tile_scratch;
+  Acc* tiles = (Acc*)a.tile_scratch;
   k_fold_reduce<<<grid, FOLD_BLOCK, 0, stream>>>(a, tiles);
   k_fold_scan_tiles<<<1, FOLD_BLOCK, 0, stream>>>(tiles, a, a.counters);
   k_fold_apply<<<grid, FOLD_BLOCK, 0, stream>>>(a, tiles);
