@@ -1008,17 +1008,19 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   // Test-and-set and the content loads go out together.  The content is the leaf's if the
   // test-and-set returns exactly the version the lookup kernel saw (unlocked): every commit
   // changes the version and an aborted writer writes nothing, so no writer touched the leaf
-  // since this kernel started.  Otherwise, with the lock held, it is read again.
+  // since this kernel started, and memory holds what the previous kernels left there, which
+  // plain loads see across the kernel boundary.  Otherwise, with the lock held, it is read
+  // again with device-coherent loads.
   if (act) {
     if (g == 0)
       word = __hip_atomic_fetch_or(&m->lock, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    info = ld_relaxed(&m->info);
-    low = ld_relaxed(&m->low);
-    high = ld_relaxed(&m->high);
+    info = m->info;
+    low = m->low;
+    high = m->high;
 #pragma unroll
     for (int e = 0; e < K; ++e) {
-      lk[e] = ld_relaxed(kp + e * G);
-      lv[e] = ld_relaxed(vp + e * G);
+      lk[e] = kp[e * G];
+      lv[e] = vp[e * G];
     }
   } else {
 #pragma unroll
