@@ -201,9 +201,20 @@ STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, 
   const int64_t p = (int64_t)(rank + 0.5f);
   const uint32_t base = (uint32_t)max<int64_t>(p - ROUTE_WINDOW, 0);
   window_load<G>(rv, base, g, c);
-  const bool ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par);
+  bool ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par);
   const int64_t r = (int64_t)base + (int64_t)cnt - 1;
   model = (ok && r >= p - ROUTE_WINDOW && r <= p + ROUTE_WINDOW) ? 1 : 2;
+  if (!ok) {
+    // A miss is usually near: one more window on the side the key lies (before the window:
+    // no entry <= key; past it: every entry <= key) before the caller's root-to-leaf descent.
+    if (cnt == 0 && base > 0) {
+      window_load<G>(rv, base > (uint32_t)G - 1 ? base - (uint32_t)(G - 1) : 0u, g, c);
+      ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par);
+    } else if (cnt == (uint32_t)G && base + G - 1 < rv.L) {
+      window_load<G>(rv, base + G - 1, g, c);
+      ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par);
+    }
+  }
   return ok;
 }
 
