@@ -846,8 +846,11 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
   if (hipMemsetAsync(t->view.meta, 0, sizeof(NodeMeta) * cap, t->stream) != hipSuccess ||
       hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * STATS_WORDS, t->stream) != hipSuccess)
     return bail(fail(STM_ERR_DEVICE, "pool init"));
+  // Phase events only time device work: without the system-scope fence a record is a
+  // timestamp, not a cache write-back between two kernels (≈10 µs each at a phase boundary).
   for (int p = 0; p < STM_NUM_PHASES; ++p) {
-    if (hipEventCreate(&t->ev_a[p]) != hipSuccess || hipEventCreate(&t->ev_b[p]) != hipSuccess)
+    if (hipEventCreateWithFlags(&t->ev_a[p], hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&t->ev_b[p], hipEventDisableSystemFence) != hipSuccess)
       return bail(fail(STM_ERR_DEVICE, "hipEventCreate"));
   }
   rc = init_empty(t);
