@@ -298,7 +298,7 @@ int alloc_router(stm_tree* t) {
       dalloc(&r.segs, ROUTE_SEGMENTS) || dalloc(&r.dir, cap + 1) || dalloc(&r.rank_of, cap) ||
       dalloc(&r.lows, cap) || dalloc(&r.lows_tmp, cap) || dalloc(&r.sorted_lows, cap) ||
       dalloc(&r.order, cap) || dalloc(&r.order_tmp, cap) || dalloc(&r.hist, sort_hist_words(cap)) ||
-      dalloc(&r.partial, 4 * route_partial_blocks()) || dalloc(&r.dir_alt, cap + 1))
+      dalloc(&r.dir_alt, cap + 1))
     return fail(STM_ERR_NOMEM, "learned-routing buffers for %llu nodes", (unsigned long long)cap);
   t->route_alloc = true;
   t->view.rank_of = r.rank_of;
@@ -310,7 +310,7 @@ int alloc_router(stm_tree* t) {
 void free_router(stm_tree* t) {
   RouteState& r = t->route;
   void* ps[] = {r.hdr, r.fences, r.lids, r.segs, r.dir, r.rank_of, r.lows, r.lows_tmp,
-                r.sorted_lows, r.order, r.order_tmp, r.hist, r.partial, r.dir_alt};
+                r.sorted_lows, r.order, r.order_tmp, r.hist, r.dir_alt};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   t->route = RouteState();

@@ -5,7 +5,7 @@
 // tree emits (key, leaf rank) pairs — the low fence of every leaf in key order — and the
 // router is trained on them on the device:
 //   k_route_directory     leaves sorted by low fence -> fences[r], lids[r], L
-//   k_route_root_*        least-squares line rank ~ a*key + b (fp64, deterministic reduce)
+//   k_route_root          root line segment ~ a*key + b through the first and last fence
 //   k_route_segments      segment s = ranks whose root prediction falls in bucket s
 //   k_route_train         one wavefront per segment: full-batch Adam on the MSE of a
 //                         1 -> 16 -> 1 ReLU MLP (hand-written forward and backward)
@@ -19,7 +19,6 @@
 
 namespace stm {
 
-constexpr int RT_BLOCK = 256;
 
 // Sort keys of the directory build: a leaf's low fence, +inf for every other node, so the
 // leaves come first in key order after the sort (no compaction and no shared counter).
@@ -220,58 +219,14 @@ __global__ void k_dir_set_len(RouteHeader* __restrict__ rh) {
   if (threadIdx.x == 0) rh->L = rh->L_next;
 }
 
-// Per-block sums of (x, y, xx, xy) with x = fence / 2^64 and y = rank.
-__global__ __launch_bounds__(RT_BLOCK) void k_route_root_partial(const uint64_t* __restrict__ fences,
-                                                                 const RouteHeader* __restrict__ rh,
-                                                                 double* __restrict__ partial) {
-  __shared__ double s[4][RT_BLOCK];
-  const uint32_t L = rh->L;
-  double sx = 0, sy = 0, sxx = 0, sxy = 0;
-  for (uint32_t r = blockIdx.x * RT_BLOCK + threadIdx.x; r < L; r += gridDim.x * RT_BLOCK) {
-    const double x = (double)fences[r] * 5.421010862427522e-20;  // 2^-64
-    const double y = (double)r;
-    sx += x;
-    sy += y;
-    sxx += x * x;
-    sxy += x * y;
-  }
-  s[0][threadIdx.x] = sx;
-  s[1][threadIdx.x] = sy;
-  s[2][threadIdx.x] = sxx;
-  s[3][threadIdx.x] = sxy;
-  __syncthreads();
-  for (int off = RT_BLOCK / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off)
-      for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x < 4) partial[blockIdx.x * 4 + threadIdx.x] = s[threadIdx.x][0];
-}
-
-__global__ void k_route_root_final(const double* __restrict__ partial, uint32_t nblocks,
-                                   const uint64_t* __restrict__ fences, RouteHeader* __restrict__ rh,
-                                   uint32_t keep_root) {
+// The root: the line through the first and last fence, in segment units.  A least-squares
+// line of a skewed key set would leave a large share of the keys left of its zero crossing,
+// and clamping them into segment 0 would make that one segment hold most of the leaves.  Warm
+// retrains keep the root (keys beyond the old last fence go to the last segment).
+__global__ void k_route_root(const uint64_t* __restrict__ fences, RouteHeader* __restrict__ rh) {
   if (threadIdx.x != 0) return;
-  double sx = 0, sy = 0, sxx = 0, sxy = 0;
-  for (uint32_t b = 0; b < nblocks; ++b) {
-    sx += partial[4 * b];
-    sy += partial[4 * b + 1];
-    sxx += partial[4 * b + 2];
-    sxy += partial[4 * b + 3];
-  }
-  const double n = (double)rh->L;
-  double a = 0, b = 0;
-  const double var = sxx * n - sx * sx;
-  if (n >= 2 && var > 0) a = (sxy * n - sx * sy) / var;  // least-squares slope (diagnostic)
-  rh->ls_slope = a * 5.421010862427522e-20;
-  // The root is the line through the first and last fence, in segment units: the
-  // least-squares line of a skewed key set leaves a large share of the keys left of its zero
-  // crossing, and clamping them into segment 0 would make that one segment hold most of the
-  // leaves.  A warm retrain keeps it (keys beyond the old last fence go to the last segment).
-  if (keep_root) return;
   const uint32_t L = rh->L;
-  a = 0;
-  b = 0;
+  double a = 0, b = 0;
   if (L >= 2) {
     const double k0 = (double)fences[0], k1 = (double)fences[L - 1];
     if (k1 > k0) {
@@ -540,7 +495,6 @@ hipError_t route_evaluate(const RouteState& rs, uint64_t* d_hm, uint32_t n_max, 
 }
 
 // ------------------------------------------------------------------ host side
-size_t route_partial_blocks() { return 256; }
 
 hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint32_t steps,
                        float lr, bool train, bool warm, hipStream_t s) {
@@ -573,9 +527,7 @@ hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint
                                                             rs.fences, rs.lids, rs.dir, rs.rank_of);
     k_route_parents<<<(n_nodes + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.dir);
   }
-  k_route_root_partial<<<(uint32_t)route_partial_blocks(), RT_BLOCK, 0, s>>>(rs.fences, rs.hdr, rs.partial);
-  k_route_root_final<<<1, 64, 0, s>>>(rs.partial, (uint32_t)route_partial_blocks(), rs.fences, rs.hdr,
-                                      warm ? 1u : 0u);
+  if (!warm) k_route_root<<<1, 64, 0, s>>>(rs.fences, rs.hdr);
   k_route_segments<<<(ROUTE_SEGMENTS + 255) / 256, 256, 0, s>>>(rs.hdr, rs.fences, rs.lids, rs.segs,
                                                                 warm ? 0u : 1u);
   e = hipGetLastError();

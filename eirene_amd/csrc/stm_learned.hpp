@@ -36,7 +36,6 @@ struct RouteHeader {
   uint32_t L_next;  // leaves of the directory being rewritten
   uint32_t pad;
   double a, b;  // root line: segment ~ a * key + b
-  double ls_slope;  // least-squares slope of rank over key (reported, not used to route)
 };
 
 struct RouteView {
@@ -92,11 +91,9 @@ struct RouteState {
   // directory build scratch
   uint64_t *lows = nullptr, *lows_tmp = nullptr, *sorted_lows = nullptr;
   uint32_t *order = nullptr, *order_tmp = nullptr, *hist = nullptr;
-  double* partial = nullptr;
   DirEntry* dir_alt = nullptr;  // capacity + 1: the directory being rewritten (warm retrain)
 };
 
-size_t route_partial_blocks();
 // Builds the directory and the root; trains the segment MLPs for `steps` Adam steps when
 // train is set.  warm keeps the segments' current parameters as the starting point (a
 // retrain after a few batches); otherwise they restart from the identity-like initial ones.
