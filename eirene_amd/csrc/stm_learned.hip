@@ -385,7 +385,7 @@ constexpr uint32_t TRAIN_MAX_PAIRS = WAVE * TRAIN_CACHE;  // 512 pairs per segme
 // parameters as wave-uniform values.
 __global__ __launch_bounds__(TRAIN_WAVES * WAVE) void k_route_train(
     const RouteHeader* __restrict__ rh, const uint64_t* __restrict__ fences,
-    SegModel* __restrict__ segs, uint32_t steps, float lr) {
+    SegModel* __restrict__ segs, uint32_t steps, float lr, uint32_t salt) {
   const int wave = threadIdx.x / WAVE;
   const int lane = lane_id();
   const uint32_t s = blockIdx.x * TRAIN_WAVES + wave;
@@ -400,18 +400,21 @@ __global__ __launch_bounds__(TRAIN_WAVES * WAVE) void k_route_train(
   const float inv_n = 1.0f / (float)cnt;
   const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
   float b1t = 1.0f, b2t = 1.0f;
-  // A segment of at most TRAIN_MAX_PAIRS leaves trains on all its pairs, held in registers.
-  // A larger one (the linear root leaves dense key ranges with several times the mean)
-  // trains on every span-th pair, at most TRAIN_MAX_PAIRS of them, from an offset that moves
-  // every step, so the largest segments cost no more per step than a full one.
+  // The training pairs sit in registers for all steps.  A segment of at most TRAIN_MAX_PAIRS
+  // leaves trains on all of them; a larger one (the linear root leaves dense key ranges with
+  // several times the mean) on every span-th pair, from an offset that differs between builds
+  // (salt), so successive retrains see different pairs and the largest segments cost no more
+  // per step than a full one.
   const uint32_t span = (cnt + TRAIN_MAX_PAIRS - 1) / TRAIN_MAX_PAIRS;
-  const bool cached = span == 1;
-  const uint32_t used = cached ? cnt : 0;  // pairs per step of a cached segment
+  const uint32_t shift = span > 1 ? (salt * 2654435761u) % span : 0u;
+  const uint32_t used = (cnt - shift + span - 1) / span;  // pairs shift, shift + span, ...
+  const float scale = 1.0f / (float)used;
   float xc[TRAIN_CACHE], tc[TRAIN_CACHE];
 #pragma unroll
   for (int c = 0; c < TRAIN_CACHE; ++c) {
-    const uint32_t i = (uint32_t)lane + (uint32_t)c * WAVE;
-    xc[c] = cached && i < used ? seg_x(m, fences[m.r0 + i]) : 0.0f;
+    const uint32_t j = (uint32_t)lane + (uint32_t)c * WAVE;
+    const uint32_t i = j * span + shift;
+    xc[c] = j < used ? seg_x(m, fences[m.r0 + i]) : 0.0f;
     tc[c] = (float)i * inv_n;
   }
   for (uint32_t it = 0; it < steps; ++it) {
@@ -420,20 +423,9 @@ __global__ __launch_bounds__(TRAIN_WAVES * WAVE) void k_route_train(
     float g[MLP_PARAMS];
 #pragma unroll
     for (int q = 0; q < MLP_PARAMS; ++q) g[q] = 0.0f;
-    if (cached) {
-      const float scale = 1.0f / (float)used;
 #pragma unroll
-      for (int c = 0; c < TRAIN_CACHE; ++c)
-        if ((uint32_t)lane + (uint32_t)c * WAVE < used) mlp_accumulate(p, xc[c], tc[c], scale, g);
-    } else {
-      const uint32_t shift = (it * 2654435761u) % span;
-      const uint32_t n_sub = (cnt - shift + span - 1) / span;  // pairs shift, shift + span, ...
-      const float scale = 1.0f / (float)n_sub;
-      for (uint32_t j = lane; j < n_sub; j += WAVE) {
-        const uint32_t i = j * span + shift;
-        mlp_accumulate(p, seg_x(m, fences[m.r0 + i]), (float)i * inv_n, scale, g);
-      }
-    }
+    for (int c = 0; c < TRAIN_CACHE; ++c)
+      if ((uint32_t)lane + (uint32_t)c * WAVE < used) mlp_accumulate(p, xc[c], tc[c], scale, g);
     const float gr = wave_reduce_scatter(g);
     b1t *= beta1;
     b2t *= beta2;
@@ -499,6 +491,7 @@ hipError_t route_evaluate(const RouteState& rs, uint64_t* d_hm, uint32_t n_max, 
 hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint32_t steps,
                        float lr, bool train, bool warm, hipStream_t s) {
   if (n_nodes == 0 || n_nodes > t.capacity) return hipErrorInvalidValue;
+  rs.builds += 1;
   hipError_t e;
   if (warm) {
     // Rewrite the existing directory: leaves at most n_nodes, so every grid is sized by it.
@@ -534,7 +527,7 @@ hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint
   if (e != hipSuccess) return e;
   if (train && steps > 0) {
     k_route_train<<<(ROUTE_SEGMENTS + TRAIN_WAVES - 1) / TRAIN_WAVES, TRAIN_WAVES * WAVE, 0, s>>>(
-        rs.hdr, rs.fences, rs.segs, steps, lr);
+        rs.hdr, rs.fences, rs.segs, steps, lr, rs.builds);
     e = hipGetLastError();
   }
   return e;

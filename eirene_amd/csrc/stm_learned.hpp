@@ -92,6 +92,7 @@ struct RouteState {
   uint64_t *lows = nullptr, *lows_tmp = nullptr, *sorted_lows = nullptr;
   uint32_t *order = nullptr, *order_tmp = nullptr, *hist = nullptr;
   DirEntry* dir_alt = nullptr;  // capacity + 1: the directory being rewritten (warm retrain)
+  uint32_t builds = 0;          // directory builds so far (varies the training subsample)
 };
 
 // Builds the directory and the root; trains the segment MLPs for `steps` Adam steps when
