@@ -995,8 +995,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
                                    uint32_t leaf_l, uint64_t ver_l, uint32_t gs_l, uint32_t np_l,
                                    uint32_t p0_l, const uint32_t* __restrict__ sidx,
                                    uint64_t* __restrict__ out_vals,
-                                   uint8_t* __restrict__ out_found, TxCounters& c,
-                                   uint64_t*& pend_lock, uint64_t& pend_word) {
+                                   uint8_t* __restrict__ out_found, TxCounters& c) {
   constexpr int K = F / G;
   constexpr int NG = WAVE / G;
   const int lane = lane_id();
@@ -1018,10 +1017,14 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   uint64_t info = 0, low = 0, high = 0, word = 0;
   // Test-and-set and the content loads go out together.  The content is the leaf's if the
   // test-and-set returns exactly the version the lookup kernel saw (unlocked): every commit
-  // changes the version and an aborted writer writes nothing, so no writer touched the leaf
-  // since this kernel started, and memory holds what the previous kernels left there, which
-  // plain loads see across the kernel boundary.  Otherwise, with the lock held, it is read
-  // again with device-coherent loads.
+  // changes the version and an aborted writer writes nothing, so no group of this launch has
+  // written the leaf, and memory holds what the previous kernels left there, which plain loads
+  // see across the kernel boundary.  Any other word (locked, or a version a group of this
+  // launch committed) hands the op back to the transactional kernel that runs after this
+  // launch.  So no group here reads keys or values another group here wrote: those stores
+  // need neither device-wide visibility nor ordering before the unlock; the launch boundary
+  // publishes them.  The lock and the count|level word share a line with other groups'
+  // atomics and keep device-coherent stores.
   if (act) {
     if (g == 0)
       word = __hip_atomic_fetch_or(&m->lock, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1040,27 +1043,10 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
       lv[e] = 0;
     }
   }
-  // The previous round's unlock (lane 0 of each group): one wait covers the acknowledgement
-  // of that round's content stores and this round's test-and-set and loads.
-  wave_order();
-  if (pend_lock) st_relaxed(pend_lock, pend_word);
-  pend_lock = nullptr;
   word = shfl64(word, gbase);
   const bool held = act && !(word & 1ull);
-  if (ballot(held && word != ver)) {
-    wave_order();  // the reload issues only after the lock is held
-    if (held && word != ver) {
-      info = ld_relaxed(&m->info);
-      low = ld_relaxed(&m->low);
-      high = ld_relaxed(&m->high);
-#pragma unroll
-      for (int e = 0; e < K; ++e) {
-        lk[e] = ld_relaxed(kp + e * G);
-        lv[e] = ld_relaxed(vp + e * G);
-      }
-    }
-  }
-  if (!held) {
+  const bool fresh = held && word == ver;
+  if (!fresh) {
 #pragma unroll
     for (int e = 0; e < K; ++e) {
       lk[e] = KEY_EMPTY;
@@ -1069,7 +1055,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
     info = 0;
   }
   const uint32_t count = meta_count(info);
-  const bool fits = held && meta_level(info) == 0 && count <= (uint32_t)F && key >= low && key < high;
+  const bool fits = fresh && meta_level(info) == 0 && count <= (uint32_t)F && key >= low && key < high;
   uint32_t le = 0;
 #pragma unroll
   for (int e = 0; e < K; ++e) le += lk[e] <= key ? 1u : 0u;
@@ -1082,7 +1068,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   const bool commit = found || insert;
   const uint64_t old = found ? at_v : 0;
   if (found) {
-    if (g == s % G) st_relaxed(t.vals + (size_t)leaf * F + s, val);
+    if (g == s % G) t.vals[(size_t)leaf * F + s] = val;
   }
   // Shift insert at pos = cle: slot x takes old[x] below pos, the new pair at pos and
   // old[x - 1] above; only slots pos .. count change.  old[x - 1] is element e of lane g - 1,
@@ -1098,17 +1084,12 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
     if (insert && x >= pos && x <= (int)count) {
       const uint64_t ok_ = g > 0 ? pk_in : pk_wr;
       const uint64_t ov_ = g > 0 ? pv_in : pv_wr;
-      st_relaxed(kp + e * G, x == pos ? key : ok_);
-      st_relaxed(vp + e * G, x == pos ? val : ov_);
+      kp[e * G] = x == pos ? key : ok_;
+      vp[e * G] = x == pos ? val : ov_;
     }
   }
   if (insert && g == 0) st_relaxed(&m->info, make_info(count + 1, 0));
-  // The new version is stored once every content store is acknowledged: in the next round
-  // after its wait, or after the last round.
-  if (held && g == 0) {
-    pend_lock = &m->lock;
-    pend_word = commit ? word + 2 : word;
-  }
+  if (held && g == 0) st_relaxed(&m->lock, commit ? word + 2 : word);
   if (commit) {
     for (uint32_t p = (uint32_t)g; p < np; p += G) {
       const uint32_t pos_o = p == 0 ? p0 : pos_of(sidx[gs + p]);
@@ -1158,8 +1139,6 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
   const uint32_t e1 = min(e0 + per, N);
   TxCounters c;
   uint32_t upserts = 0;
-  uint64_t* pend_lock = nullptr;  // a lock this lane still has to release, and its new word
-  uint64_t pend_word = 0;
   for (uint32_t base = e0; base < e1; base += WAVE) {
     const uint32_t e = base + lane_id();
     const bool mine = e < e1;
@@ -1193,8 +1172,7 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
         if (lane_id() / G == q) my_src = l;
       }
       const uint64_t done = fast_leaf_upserts<F, G>(t, my_src, k_l, v_l, leaf_l, ver_l, gs_l,
-                                                    np_l, p0_l, sidx, out_vals, out_found, c,
-                                                    pend_lock, pend_word);
+                                                    np_l, p0_l, sidx, out_vals, out_found, c);
       upserts += (uint32_t)__popcll(done);
       todo &= ~done;
     }
@@ -1205,8 +1183,6 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
       if ((todo >> lane_id()) & 1ull) slist[at + (uint32_t)__popcll(todo & lanemask_lt())] = gi;
     }
   }
-  wave_order();
-  if (pend_lock) st_relaxed(pend_lock, pend_word);
   flush_counters(t, c);
   if (lane_id() == 0) stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
 }
