@@ -18,7 +18,7 @@ launched batch by batch from the host with the library defaults.  A third ("serv
 continuous-batching request loop open-loop: 2^25 uniform keys, Poisson arrivals of requests
 of 1..2^16 ops (90% lookups, 8% inserts, 2% range scans), request latency percentiles.
 
-    python bench.py [--steps K] [--warmup W] [--route descent|learned] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--route descent|learned|index] [--no-cpu]
                     [--uneven-batches N] [--tuning FILE]
 """
 from __future__ import annotations
@@ -54,7 +54,8 @@ def parse():
     ap.add_argument("--fanout", type=int, default=64)
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--group", type=int, default=16)
-    ap.add_argument("--route", choices=["descent", "learned", "both"], default="both")
+    ap.add_argument("--route", choices=["descent", "learned", "index", "both"], default="both",
+                    help="both = descent, learned and the static index")
     ap.add_argument("--tuning-headline", default=str(ROOT / "eirene_amd" / "tuning" /
                                                      "gamma_2p26_batch_2p22.json"),
                     help="tuning record (eirene_amd.autotune) for the tuned routes; '' = none")
@@ -248,7 +249,8 @@ def main():
     def run_route(route):
         cfg = ea.StmConfig(fanout=args.fanout, max_keys=n_load + n_batches * n_batch + (1 << 20),
                            max_batch=n_batch, lanes_per_op=args.lanes, ops_per_group=args.group,
-                           route=ea.ROUTE_LEARNED if route.startswith("learned") else ea.ROUTE_DESCENT,
+                           route={"learned": ea.ROUTE_LEARNED, "index": ea.ROUTE_INDEX}.get(
+                               route.replace("_tuned", ""), ea.ROUTE_DESCENT),
                            retrain_every=8, timing=True)
         if route.endswith("_tuned"):
             from eirene_amd import autotune as at
@@ -284,7 +286,7 @@ def main():
     bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
     out_f = ea.DeviceBuffer(keys.size, np.uint8)
-    routes = ["descent", "learned"] if args.route == "both" else [args.route]
+    routes = ["descent", "learned", "index"] if args.route == "both" else [args.route]
     if tuned and "descent" in routes:  # the record was tuned with root-to-leaf descent
         routes.append("descent_tuned")
     res = {r: run_route(r) for r in routes}
@@ -396,6 +398,9 @@ def main():
         "routes": {r: summary(r) for r in res},
         "learned_over_plain": {("descent" + k): round(res["learned"]["mops"] / res["descent" + k]["mops"], 4)
                                for k in ("", "_tuned") if "learned" in res and "descent" + k in res},
+        # the same leaf directory searched without the model (a rebuilt static index)
+        "learned_over_index": (round(res["learned"]["mops"] / res["index"]["mops"], 4)
+                               if "learned" in res and "index" in res else None),
         "learned_and_plain_results_identical": same,
         "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
                                    "root_splits", "deferred", "n_keys", "height")},

@@ -27,14 +27,14 @@ import numpy as np
 __all__ = [
     "StmTree", "StmError", "StmConfig", "DeviceBuffer", "load_library", "LIB_PATH",
     "OP_INSERT", "OP_GET", "OP_DELETE", "OP_RANGE", "KEY_RESERVED",
-    "ROUTE_DESCENT", "ROUTE_LEARNED", "PHASES",
+    "ROUTE_DESCENT", "ROUTE_LEARNED", "ROUTE_INDEX", "PHASES",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libstmtree.so"
 
 OP_INSERT, OP_GET, OP_DELETE, OP_RANGE = 0, 1, 2, 3
 KEY_RESERVED = 0xFFFFFFFFFFFFFFFF
-ROUTE_DESCENT, ROUTE_LEARNED = 0, 1
+ROUTE_DESCENT, ROUTE_LEARNED, ROUTE_INDEX = 0, 1, 2
 PHASES = ("sort", "merge", "lookup", "upsert", "deferred", "route", "train", "scan")
 
 STATUS = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "STATE"}

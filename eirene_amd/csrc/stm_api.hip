@@ -331,6 +331,8 @@ int train_router(stm_tree* t, uint32_t steps, bool periodic) {
   rc = read_header(t, &h);
   if (rc) return rc;
   const uint32_t nn = (uint32_t)std::min<uint64_t>(h.n_nodes, t->capacity);
+  const bool index = t->cfg.route == STM_ROUTE_INDEX;
+  if (index) steps = 0;  // the static index only needs the directory
   const bool warm = periodic && t->route_warm;
   if (warm) steps = std::max<uint32_t>(steps / 8, 16);
   // Adam's first steps move every parameter by about the learning rate whatever the
@@ -338,7 +340,7 @@ int train_router(stm_tree* t, uint32_t steps, bool periodic) {
   const float lr = warm ? 0.002f : 0.02f;
   phase_begin(t, STM_PHASE_TRAIN);
   HIP_TRY(route_build(t->view, t->route, nn, steps, lr, steps > 0, warm, t->stream));
-  t->route_warm = steps > 0;
+  t->route_warm = steps > 0 || index;
   phase_end(t, STM_PHASE_TRAIN);
   if (t->cfg.timing) {  // every retrain is counted, also between graph replays
     HIP_TRY(hipStreamSynchronize(t->stream));
@@ -424,11 +426,12 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
 
   if (timed) phase_begin(t, STM_PHASE_LOOKUP);
   LookupArgs la{};
-  if (t->cfg.route == STM_ROUTE_LEARNED) {
+  if (t->cfg.route != STM_ROUTE_DESCENT) {
     la.route_hdr = t->route.hdr;
     la.fences = t->route.fences;
     la.lids = t->route.lids;
     la.segs = t->route.segs;
+    la.index = t->cfg.route == STM_ROUTE_INDEX ? 1u : 0u;
   }
   la.ukey = s.ukey;
   la.uop = s.uop;
@@ -485,9 +488,10 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   return hipSuccess;
 }
 
-// Learned routing: (re)build the directory and retrain when it is missing or stale.
+// Learned routing and the static index: (re)build the directory, and retrain, when it is
+// missing or stale.
 int maybe_retrain(stm_tree* t) {
-  if (t->cfg.route != STM_ROUTE_LEARNED) return STM_OK;
+  if (t->cfg.route == STM_ROUTE_DESCENT) return STM_OK;
   if (t->route_valid && t->batches_since_train < t->cfg.retrain_every) return STM_OK;
   return train_router(t, t->cfg.train_steps, true);
 }
@@ -812,7 +816,7 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
   if (cfg.max_attempts == 0) cfg.max_attempts = 64;
   if (cfg.retrain_every == 0) cfg.retrain_every = 8;
   if (cfg.train_steps > 100000) return fail(STM_ERR_INVALID, "train_steps above 100000");
-  if (cfg.route > STM_ROUTE_LEARNED) return fail(STM_ERR_INVALID, "route %u unknown", cfg.route);
+  if (cfg.route > STM_ROUTE_INDEX) return fail(STM_ERR_INVALID, "route %u unknown", cfg.route);
   const uint64_t cap = capacity_for(cfg.max_keys, cfg.fanout);
   if (cap >= NODE_NONE) return fail(STM_ERR_INVALID, "max_keys too large for 32-bit node ids");
 

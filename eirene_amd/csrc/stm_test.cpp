@@ -30,8 +30,9 @@ int main(int argc, char** argv) {
         case 'v': {
           uint32_t route = STM_ROUTE_DESCENT;
           if (std::strcmp(optarg, "learned") == 0) route = STM_ROUTE_LEARNED;
+          else if (std::strcmp(optarg, "index") == 0) route = STM_ROUTE_INDEX;
           else if (std::strcmp(optarg, "38") && std::strcmp(optarg, "39") && std::strcmp(optarg, "plain")) {
-            std::fprintf(stderr, "unknown method %s (38, 39, plain or learned)\n", optarg);
+            std::fprintf(stderr, "unknown method %s (38, 39, plain, learned or index)\n", optarg);
             return 2;
           }
           tree.reset(new stm_host::GPU_STM_Tree(route));

@@ -176,7 +176,7 @@ STM_DEV bool window_find(const RouteCursor& c, uint64_t key, int gbase, uint32_t
 // the entry's leaf id, split flag and upper fence; false sends the caller to a root-to-leaf
 // descent.  model: 0 = routed without the model, 1 = the model's +/-2 window held the key,
 // 2 = the model missed.
-template <int F, int G>
+template <int F, int G, bool INDEX>
 STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, RouteCursor& c,
                            uint32_t& lid, bool& dirty, uint64_t& hi, int& model, uint32_t& par) {
   model = 0;
@@ -187,6 +187,25 @@ STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, 
     // Past the window: the next entries (one coalesced load) usually hold the key.
     window_load<G>(rv, c.base + G - 1, g, c);
     if (window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par)) return true;
+  }
+  if constexpr (INDEX) {
+    // Static index: a G-ary search of the directory for the last fence <= key.  The rank lies
+    // in [lo, top); each step every lane of the group probes one fence, and the range shrinks
+    // to one probe interval.  A range of at most G - 1 entries fits a window with the
+    // successor of its last entry.
+    uint32_t lo = 0, top = rv.L;
+    while (top - lo > (uint32_t)G - 1) {
+      const uint32_t step = (top - lo + G - 1) / G;
+      const uint32_t q = lo + (uint32_t)g * step;
+      const uint32_t below = group_sum<G>(q < top && rv.fences[q] <= key ? 1u : 0u);
+      if (below == 0) break;  // the key precedes the first fence: descend instead
+      lo += (below - 1) * step;
+      top = min(top, lo + step);
+    }
+    window_load<G>(rv, lo, g, c);
+    const bool ok = window_find<G>(c, key, gbase, lid, dirty, hi, cnt, par);
+    model = ok ? 1 : 2;
+    return ok;
   }
   // The model: root line -> segment -> MLP -> rank p; entries [p-2, p-2+G) are loaded and the
   // key must lie in [p-2, p+2] for a model hit.
@@ -221,7 +240,7 @@ STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, 
 // Stripe of the statistics counters a wavefront adds to.
 STM_DEV uint32_t wave_stripe() { return blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; }
 
-template <int F, int G, bool LEARNED>
+template <int F, int G, bool LEARNED, bool INDEX>
 __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __restrict__ ukey,
                                                 const uint8_t* __restrict__ uop,
                                                 const uint32_t* __restrict__ gstart,
@@ -314,7 +333,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
           const bool same_entry = leaf != NODE_NONE && key < cur_hi;
           uint64_t hi = cur_hi;
           routed = same_entry ||
-                   learned_route<F, G>(rv, key, g, gbase, cur, lid, dirty, hi, model, rpar);
+                   learned_route<F, G, INDEX>(rv, key, g, gbase, cur, lid, dirty, hi, model, rpar);
           if (routed) {
             par = rpar;  // the level-1 hint recorded for merged INSERTs
             cur_hi = hi;
@@ -1432,16 +1451,21 @@ static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipSt
   // Persistent grid: at most LOOKUP_GRID_CAP workgroups; each lane group walks one contiguous span.
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((threads + 255) / 256, LOOKUP_GRID_CAP);
   if (blocks == 0) return hipSuccess;
-  if (a.route_hdr)
-    k_lookup<F, G, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
-                                                 a.counters, a.chunk, a.out_vals, a.out_found,
-                                                 a.route_hdr, a.fences, a.lids, a.segs, a.desc,
-                                                 a.uleaf, a.uver, a.upar);
+  if (a.route_hdr && a.index)
+    k_lookup<F, G, true, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
+                                                       a.counters, a.chunk, a.out_vals, a.out_found,
+                                                       a.route_hdr, a.fences, a.lids, a.segs, a.desc,
+                                                       a.uleaf, a.uver, a.upar);
+  else if (a.route_hdr)
+    k_lookup<F, G, true, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
+                                                        a.counters, a.chunk, a.out_vals, a.out_found,
+                                                        a.route_hdr, a.fences, a.lids, a.segs, a.desc,
+                                                        a.uleaf, a.uver, a.upar);
   else
-    k_lookup<F, G, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
-                                                  a.counters, a.chunk, a.out_vals, a.out_found,
-                                                  nullptr, nullptr, nullptr, nullptr, a.desc,
-                                                  a.uleaf, a.uver, a.upar);
+    k_lookup<F, G, false, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
+                                                         a.counters, a.chunk, a.out_vals, a.out_found,
+                                                         nullptr, nullptr, nullptr, nullptr, a.desc,
+                                                         a.uleaf, a.uver, a.upar);
   return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ struct LookupArgs {
   const uint64_t* fences;
   const uint32_t* lids;
   const SegModel* segs;
+  uint32_t index;         // learned routing: 1 = search the directory instead of the model
   const BatchDesc* desc;  // optional: out_vals / out_found are taken from here
   uint32_t* uleaf;        // optional: leaf of every merged INSERT (the upsert kernels' hint),
   uint64_t* uver;         //   that leaf's lock word when the lookup kernel ran,

@@ -51,6 +51,9 @@ extern "C" {
 /* Lookup routing methods (stm_config.route). */
 #define STM_ROUTE_DESCENT 0 /* root-to-leaf descent (the plain method) */
 #define STM_ROUTE_LEARNED 1 /* learned leaf-window prediction, verified, with descent fallback */
+#define STM_ROUTE_INDEX 2   /* static leaf index: the same leaf directory searched without a model
+                               (the non-learned comparison; reference analogue: the rebuilt index
+                               of src/gpu-stm-tree-v35/bpt-gpustm-kernel.cu:17-231) */
 
 typedef struct stm_config {
     uint32_t fanout;      /* slots per node: 32, 64 or 128 (default 64) */

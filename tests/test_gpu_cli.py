@@ -35,7 +35,7 @@ def write_ops(path, rng, n, domain, warehouse):
             np.array(ops, dtype=np.uint8))
 
 
-@pytest.mark.parametrize("method", ["38", "learned"])
+@pytest.mark.parametrize("method", ["38", "learned", "index"])
 def test_cli_matches_oracle(tmp_path, oracle_mod, method):
     assert CLI.exists(), "build the package first (python -m eirene_amd.build)"
     rng = np.random.default_rng(11)

@@ -1,6 +1,7 @@
 """GPU tests of the learned leaf router: device gradients against the CPU restatement,
-bit-exact batch results with learned routing, and what the trained network gains over
-the untrained router (the linear root with identity segment models)."""
+bit-exact batch results with learned routing and with the static index over the same
+directory, the warm directory rewrite, and what the trained network gains over the untrained
+router (the linear root with identity segment models)."""
 import numpy as np
 import pytest
 
@@ -52,6 +53,33 @@ def test_learned_routing_results_match_oracle(oracle_mod):
     st = t.stats()
     assert st["route_hits"] > 0
     assert st["route_hits"] > 4 * st["route_misses"], st
+    k, v = t.dump()
+    ek, evv = o.dump()
+    np.testing.assert_array_equal(k, ek)
+    np.testing.assert_array_equal(v, evv)
+    t.close()
+
+
+def test_index_routing_results_match_oracle(oracle_mod):
+    """The static index (the directory searched without the model) routes every op exactly:
+    results and the final dump equal the oracle across directory rewrites."""
+    import eirene_amd as ea
+    from eirene_amd import workloads as w
+    t, load, vals = gamma_tree(ea, w, 1 << 18, 11, retrain_every=2, route=ea.ROUTE_INDEX)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, vals)
+    srt = w.sorted_copy(load)
+    for b in range(5):
+        lookup = 0.95 if b % 2 == 0 else 0.5
+        keys, bv, ops = w.mixed_batch(srt, 1 << 18, 110 + b, lookup, 0.9, w.DIST_GAMMA, 2.0, 2.0 ** 56)
+        gv, gf = t.run_batch(keys, bv, ops)
+        ev, ef = o.run_batch(keys, bv, ops)
+        np.testing.assert_array_equal(gf, ef)
+        np.testing.assert_array_equal(gv, ev)
+        assert t.check() == 0
+    st = t.stats()
+    assert st["route_hits"] > 0
+    assert st["route_misses"] == 0, st  # the search always lands on the key's entry
     k, v = t.dump()
     ek, evv = o.dump()
     np.testing.assert_array_equal(k, ek)
