@@ -403,7 +403,7 @@ def main():
                                if "learned" in res and "index" in res else None),
         "learned_and_plain_results_identical": same,
         "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
-                                   "root_splits", "deferred", "n_keys", "height")},
+                                   "root_splits", "deferred", "n_keys", "height", "node_reads")},
         "roofline": roofline, "cpu_baseline": cpu, "cpu_gpu_first_batch_agree": agree,
         "uneven": uneven,
         "serving": serving,

@@ -62,7 +62,8 @@ class _Stats(C.Structure):
         "n_keys", "n_nodes", "capacity_nodes", "height", "batches", "ops", "unique_ops",
         "upserts", "commits", "aborts", "leaf_splits", "inner_splits", "root_splits",
         "deferred", "route_hits", "route_misses")] + [
-        ("phase_ms", C.c_double * 8), ("phase_launches", C.c_uint64 * 8)]
+        ("phase_ms", C.c_double * 8), ("phase_launches", C.c_uint64 * 8),
+        ("node_reads", C.c_uint64)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -367,6 +368,7 @@ class StmTree:
         d = {name: getattr(s, name) for name, _ in _Stats._fields_[:16]}
         d["phase_ms"] = {p: s.phase_ms[i] for i, p in enumerate(PHASES)}
         d["phase_launches"] = {p: s.phase_launches[i] for i, p in enumerate(PHASES)}
+        d["node_reads"] = s.node_reads
         return d
 
     def reset_stats(self) -> None:

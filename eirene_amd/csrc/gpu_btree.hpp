@@ -144,6 +144,11 @@ class GPU_STM_Tree : public GPU_BTree {
     std::fprintf(out, "bucket size\t%llu\n", (unsigned long long)cfg.max_batch);
     std::fprintf(out, "total size\t%zu\n", dc.keys.size());
     std::fprintf(out, "total gpu exec time: %g\n", ms / 1000.0);
+    // The reference's ABORT_COUNT totals (src/gpu-stm-tree-v38/bpt-gpustm.cpp:142-160), per batch.
+    std::fprintf(out, "total roll back times: %llu\n",
+                 (unsigned long long)(after.aborts - before.aborts));
+    std::fprintf(out, "total traversal_steps: %llu\n",
+                 (unsigned long long)(after.node_reads - before.node_reads));
   }
 
   void test() override { dc.test(ans, found, out); }

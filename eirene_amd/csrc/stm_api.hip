@@ -1224,6 +1224,7 @@ int stm_get_stats(stm_tree* t, stm_stats* out) {
   out->deferred = c[ST_DEFERRED];
   out->route_hits = c[ST_ROUTE_HITS];
   out->route_misses = c[ST_ROUTE_MISSES];
+  out->node_reads = c[ST_NODE_READS];
   return STM_OK;
 }
 

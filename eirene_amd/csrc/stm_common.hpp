@@ -68,6 +68,7 @@ enum StatSlot : int {
   ST_UPSERTS,
   ST_ROUTE_HITS,
   ST_ROUTE_MISSES,
+  ST_NODE_READS,  // tree nodes read by lookups and transactions (the reference's traversal steps)
   ST_KEYS,  // keys added since the last bulk build (kept by stm_reset_stats); must stay last
   ST_NUM
 };

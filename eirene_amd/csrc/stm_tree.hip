@@ -269,7 +269,6 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   const uint64_t span = max<uint64_t>(chunk, (U + ngroups - 1) / ngroups);
   // Groups past the end skip the walk but stay for the wave's statistics reduction.
   const bool active = grp0 * span < U;
-  if (!LEARNED && !active) return;
   if (desc) {
     out_vals = desc->out_vals;
     out_found = desc->out_found;
@@ -288,6 +287,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
   RouteCursor cur;
   uint64_t cur_hi = 0;  // upper fence of the current directory entry
   uint32_t hits = 0, misses = 0;
+  uint32_t reads = 0;  // tree nodes this lane group loaded
   if constexpr (LEARNED) rv = route_view(rh, fences, lids, segs, t.dir);
 
   if (active) {
@@ -341,6 +341,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
             uint64_t mh = KEY_EMPTY;
             if (dirty) mh = t.meta[leaf].high;
             load_slots<F, G>(t.keys, leaf, g, lk);
+            reads += 1;
             if (dirty && key >= mh) {  // the key lives in a right sibling made by a split
               routed = false;
               for (int step = 0; step < 64; ++step) {
@@ -351,7 +352,10 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
                   break;
                 }
               }
-              if (routed) load_slots<F, G>(t.keys, leaf, g, lk);
+              if (routed) {
+                load_slots<F, G>(t.keys, leaf, g, lk);
+                reads += 1;
+              }
               else cur_hi = 0;
             }
             leaf_hi = min(cur_hi, mh);
@@ -375,6 +379,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
                 uint64_t nk[K], nv[K];
                 load_slots<F, G>(t.keys, node, g, nk);
                 load_slots<F, G>(t.vals, node, g, nv);
+                reads += 1;
                 const int s = max(group_count_le<F, G>(nk, key) - 1, 0);
                 const uint64_t child = group_at<F, G>(nv, s, gbase);
                 const uint64_t nx = (s + 1 < F) ? group_at<F, G>(nk, s + 1, gbase) : KEY_EMPTY;
@@ -383,6 +388,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
               }
               load_slots<F, G>(t.keys, node, g, pk);
               load_slots<F, G>(t.vals, node, g, pv);
+              reads += 1;
               par = node;
               par_hi = hi;
             }
@@ -392,6 +398,7 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
             leaf_hi = min(par_hi, nx);
           }
           load_slots<F, G>(t.keys, leaf, g, lk);
+          reads += 1;
         }
       }
       const int s = group_count_le<F, G>(lk, key) - 1;
@@ -425,18 +432,20 @@ __global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __re
     }
   }
   }  // span
-  if constexpr (LEARNED) {
-    // One atomic per wavefront and counter: the group leaders' counts summed by readlane.
-    uint32_t h = 0, m = 0;
+  // One atomic per wavefront and counter: the group leaders' counts summed by readlane.
+  uint32_t h = 0, m = 0, nr = 0;
 #pragma unroll
-    for (int q = 0; q < WAVE; q += G) {
+  for (int q = 0; q < WAVE; q += G) {
+    if constexpr (LEARNED) {
       h += (uint32_t)__builtin_amdgcn_readlane((int)hits, q);
       m += (uint32_t)__builtin_amdgcn_readlane((int)misses, q);
     }
-    if (lane == 0) {
-      if (h) stat_add(t.stats, ST_ROUTE_HITS, wave_stripe(), h);
-      if (m) stat_add(t.stats, ST_ROUTE_MISSES, wave_stripe(), m);
-    }
+    nr += (uint32_t)__builtin_amdgcn_readlane((int)reads, q);
+  }
+  if (lane == 0) {
+    if (h) stat_add(t.stats, ST_ROUTE_HITS, wave_stripe(), h);
+    if (m) stat_add(t.stats, ST_ROUTE_MISSES, wave_stripe(), m);
+    if (nr) stat_add(t.stats, ST_NODE_READS, wave_stripe(), nr);
   }
 }
 
@@ -599,6 +608,7 @@ enum TxResult { TX_DONE = 0, TX_ABORT = 1, TX_RESTART = 2, TX_FAIL = 3, TX_NEED_
 struct TxCounters {
   uint32_t commits = 0, aborts = 0, leaf_splits = 0, inner_splits = 0, root_splits = 0;
   uint32_t new_keys = 0;
+  uint32_t node_reads = 0;  // nodes read (inner nodes on the way, the leaf)
   bool split_happened = false;  // set by a committed leaf split (the parent changed)
 };
 
@@ -882,6 +892,7 @@ STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val
     // descent from the root.  Its fences and level check it; the leaf's fences check the route.
     pc.valid = tx_read<F>(t, par_hint, pc.n) && pc.n.level == 1 && key >= pc.n.low && key < pc.n.high;
     pc.fresh = pc.valid;
+    c.node_reads++;
   }
   if (pc.valid && key >= pc.n.low && key < pc.n.high) {
     // Route through the cached level-1 node; the tree never loses a level, so no header
@@ -906,6 +917,7 @@ STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val
     if (height >= 2) {
       uint32_t id = root;
       for (uint32_t lvl = height - 1; lvl >= 1; --lvl) {
+        c.node_reads++;
         if (!tx_read<F>(t, id, cur)) return TX_ABORT;
         if (cur.level != lvl || key < cur.low || key >= cur.high) return TX_ABORT;
         if (cur.count >= (uint32_t)F) {
@@ -926,6 +938,7 @@ STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val
       leaf_id = id;
     }
   }
+  c.node_reads++;
   if (!tx_lock_leaf<F>(t, leaf_id, cur)) return TX_ABORT;
   if (cur.level != 0 || cur.count > (uint32_t)F || key < cur.low || key >= cur.high) {
     leaf_release(&t.meta[leaf_id].lock, cur.ver);
@@ -936,6 +949,7 @@ STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val
   if (r == TX_NEED_PARENT) {
     // The leaf splits: the parent's current version and content are needed (the leaf
     // stays locked meanwhile).
+    c.node_reads++;
     if (!tx_read<F>(t, par.id, par) || par.level != 1 || key < par.low || key >= par.high) {
       leaf_release(&t.meta[leaf_id].lock, cur.ver);
       pc.valid = false;
@@ -997,6 +1011,7 @@ STM_DEV void flush_counters(const TreeView& t, const TxCounters& c) {
     stat_add(t.stats, ST_INNER_SPLITS, w, c.inner_splits);
     stat_add(t.stats, ST_ROOT_SPLITS, w, c.root_splits);
     stat_add(t.stats, ST_KEYS, w, c.new_keys);
+    stat_add(t.stats, ST_NODE_READS, w, c.node_reads);
   }
 }
 
@@ -1120,6 +1135,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   const uint64_t news = ballot(insert && g == 0);
   c.commits += (uint32_t)__popcll(leaders);
   c.new_keys += (uint32_t)__popcll(news);
+  c.node_reads += (uint32_t)__popcll(ballot(act && g == 0));
   uint64_t done = 0;
 #pragma unroll
   for (int q = 0; q < NG; ++q) {

@@ -109,6 +109,8 @@ typedef struct stm_stats {
     uint64_t route_misses;    /* learned routing: fell back to descent */
     double phase_ms[STM_NUM_PHASES];
     uint64_t phase_launches[STM_NUM_PHASES];
+    uint64_t node_reads;      /* tree nodes read by lookups and transactions (the reference's
+                                 ABORT_COUNT traversal steps, src/gpu-stm-tree-v38/bpt-gpustm.cpp:142-160) */
 } stm_stats;
 
 typedef struct stm_tree stm_tree;

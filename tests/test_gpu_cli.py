@@ -52,6 +52,7 @@ def test_cli_matches_oracle(tmp_path, oracle_mod, method):
     got = [LINE.match(l) for l in res.stdout.splitlines() if l.startswith("key: ")]
     assert all(got) and len(got) == 15000
     assert res.stdout.count("total gpu exec time") == 3
+    assert res.stdout.count("total traversal_steps: ") == 3  # the reference's ABORT_COUNT totals
     o = oracle_mod.Oracle()
     i = 0
     for keys, vals, ops in batches:

@@ -105,6 +105,11 @@ def test_empty_tree_growth_with_root_splits(oracle_mod, fanout):
             assert t.check() == 0
         st = t.stats()
         assert st["leaf_splits"] > 0 and st["root_splits"] > 0 and st["height"] >= 3
+        # Node reads (the reference's traversal steps): every merged op reads at least its leaf
+        # once between the lookup and upsert kernels, and at most a root-to-leaf path per
+        # attempt (lookup routing, transactions and their retries).
+        assert st["node_reads"] >= st["unique_ops"]
+        assert st["node_reads"] <= (st["height"] + 2) * (2 * st["unique_ops"] + st["aborts"])
         assert_dump(t, o)
 
 
