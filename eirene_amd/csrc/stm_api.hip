@@ -112,6 +112,7 @@ struct Scratch {
   uint32_t *gstart = nullptr, *npend = nullptr, *dlist = nullptr;
   uint32_t* uleaf = nullptr;  // leaf hint of every merged INSERT
   uint32_t* slist = nullptr;  // merged INSERTs the fast path hands back
+  uint32_t* scount = nullptr;  // per fast-path wavefront: how many it handed back
   uint32_t* ilist = nullptr;  // merged INSERTs of the batch
   uint32_t* upar = nullptr;   // level-1 node hint of every merged INSERT
   uint64_t* uver = nullptr;   // version of the hinted leaf
@@ -142,6 +143,7 @@ struct Scratch {
     A_(dlist, cap);
     A_(uleaf, cap);
     A_(slist, cap);
+    A_(scount, FAST_WAVES_MAX);
     A_(ilist, cap);
     A_(upar, cap);
     A_(uver, cap);
@@ -156,7 +158,7 @@ struct Scratch {
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
-                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, ilist, upar, uver, counters,
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, scount, ilist, upar, uver, counters,
                   desc_table, desc_live, cursor, acc, soff, lastins, scan_tiles};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -481,6 +483,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   ua.upar = s.upar;
   ua.ilist = s.ilist;
   ua.slist = s.slist;
+  ua.scount = s.scount;
   ua.lanes = t->cfg.upsert_lanes;
   if ((e = launch_upsert(t->view, ua, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_UPSERT);

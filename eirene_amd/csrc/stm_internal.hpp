@@ -60,7 +60,7 @@ struct BatchCounters {
   uint32_t n_bad;       // ops rejected (reserved key)
   uint32_t scan_bad;    // scan ops rejected (length or output area)
   uint64_t scan_total;  // sum of the scan lengths (pairs the output area must hold)
-  uint32_t n_slow;      // merged INSERTs the fast path handed to the transactional kernel
+  uint32_t pad1;
   uint32_t pad0;
   uint64_t hdr_error;   // the tree's sticky error bits after the batch (one readback per batch)
   uint32_t pad[4];

@@ -1146,8 +1146,10 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
 }
 
 // Fast-path kernel over the batch's merged INSERTs (ilist, n_upserts entries): each wavefront
-// owns a contiguous part and runs it NG ops at a time on the lane-group fast path; the ops it
-// hands back are appended to slist (one atomic per wavefront round that has any) for k_upsert.
+// owns a contiguous part and runs it NG ops at a time on the lane-group fast path.  The ops it
+// hands back go to the same part of slist, from its start, and their number to scount[wave]:
+// no counter is shared (one hot address took an atomic from every wavefront at once, at the
+// end of the launch).
 template <int F, int G>
 __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t* __restrict__ ukey,
                                                      const uint64_t* __restrict__ uval,
@@ -1158,6 +1160,7 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
                                                      const uint32_t* __restrict__ uleaf,
                                                      const uint64_t* __restrict__ uver,
                                                      BatchCounters* cnt, uint32_t* __restrict__ slist,
+                                                     uint32_t* __restrict__ scount,
                                                      uint64_t* __restrict__ out_vals,
                                                      uint8_t* __restrict__ out_found,
                                                      const BatchDesc* __restrict__ desc) {
@@ -1174,6 +1177,7 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
   const uint32_t e1 = min(e0 + per, N);
   TxCounters c;
   uint32_t upserts = 0;
+  uint32_t handed = 0;  // ops handed back so far (slist[e0 .. e0 + handed))
   for (uint32_t base = e0; base < e1; base += WAVE) {
     const uint32_t e = base + lane_id();
     const bool mine = e < e1;
@@ -1211,20 +1215,20 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
       upserts += (uint32_t)__popcll(done);
       todo &= ~done;
     }
-    if (todo) {  // handed back
-      uint32_t at = 0;
-      if (lane_id() == 0) at = atomicAdd(&cnt->n_slow, (uint32_t)__popcll(todo));
-      at = (uint32_t)__shfl((int)at, 0, WAVE);
-      if ((todo >> lane_id()) & 1ull) slist[at + (uint32_t)__popcll(todo & lanemask_lt())] = gi;
-    }
+    if ((todo >> lane_id()) & 1ull)  // handed back
+      slist[e0 + handed + (uint32_t)__popcll(todo & lanemask_lt())] = gi;
+    handed += (uint32_t)__popcll(todo);
   }
+  if (lane_id() == 0) scount[w] = handed;
   flush_counters(t, c);
   if (lane_id() == 0) stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
 }
 
-// Wave-wide transactions for the ops the fast path handed back (slist[0 .. n_slow)), or for
-// every merged INSERT of the batch when slist is null.  Each wavefront owns a contiguous part
-// of that sequence (key order within a part), so consecutive upserts mostly reuse the cached
+// Wave-wide transactions for the ops the fast path handed back, or for every merged INSERT
+// of the batch when slist is null.  The handed-back ops lie in the insert list's index range
+// [0, n_upserts): fast-path wavefront f owned [f * per_f, (f + 1) * per_f) and wrote its
+// scount[f] ops from the start of that range.  Each wavefront here owns a contiguous part of
+// the index range (key order within a part), so consecutive upserts mostly reuse the cached
 // parent.
 template <int F>
 __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __restrict__ ukey,
@@ -1234,14 +1238,17 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
                                                 const uint32_t* __restrict__ npend,
                                                 const uint32_t* __restrict__ sidx,
                                                 const uint32_t* __restrict__ slist,
+                                                const uint32_t* __restrict__ scount,
+                                                uint32_t fast_waves,
                                                 const uint32_t* __restrict__ upar,
                                                 BatchCounters* cnt, uint32_t* __restrict__ dlist,
                                                 uint32_t max_attempts,
                                                 uint64_t* __restrict__ out_vals,
                                                 uint8_t* __restrict__ out_found,
                                                 const BatchDesc* __restrict__ desc) {
-  const uint32_t U = slist ? cnt->n_slow : cnt->n_unique;
+  const uint32_t U = slist ? cnt->n_upserts : cnt->n_unique;
   if (U == 0) return;
+  const uint32_t per_f = slist ? (U + fast_waves - 1) / fast_waves : 1u;
   if (desc) {
     out_vals = desc->out_vals;
     out_found = desc->out_found;
@@ -1257,8 +1264,13 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
   for (uint32_t base = g0; base < g1; base += WAVE) {
     // Op data of the round's upserts, loaded together and read lane by lane.
     const uint32_t e = base + lane_id();
-    const uint32_t gi = e < g1 ? (slist ? slist[e] : e) : 0;
-    const bool mine = e < g1 && (slist || uop[gi] == OP_INSERT);
+    bool in = e < g1;
+    if (slist && in) {
+      const uint32_t f = e / per_f;
+      in = e - f * per_f < scount[f];  // within what fast-path wavefront f handed back
+    }
+    const uint32_t gi = in ? (slist ? slist[e] : e) : 0;
+    const bool mine = in && (slist || uop[gi] == OP_INSERT);
     uint64_t todo = ballot(mine);
     const uint64_t k_l = mine ? ukey[gi] : 0;
     const uint64_t v_l = mine ? uval[gi] : 0;
@@ -1501,7 +1513,8 @@ hipError_t launch_lookup(const TreeView& t, const LookupArgs& a, hipStream_t s) 
 }
 
 template <int F, int G>
-static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
+static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hipStream_t s,
+                                     uint32_t& waves) {
   // Each wavefront takes an equal contiguous share of the merged INSERTs and works through it
   // in rounds of WAVE / G: a grid of exactly the wavefronts the chip holds at once finishes
   // in one generation of rounds instead of two.
@@ -1515,29 +1528,34 @@ static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hip
     if (e != hipSuccess) return e;
     resident.store(std::max(1, cus * per_cu), std::memory_order_relaxed);
   }
-  const uint32_t blocks = std::min<uint32_t>(a.blocks, (uint32_t)resident.load(std::memory_order_relaxed));
+  const uint32_t blocks = std::min<uint32_t>(
+      std::min<uint32_t>(a.blocks, (uint32_t)resident.load(std::memory_order_relaxed)),
+      FAST_WAVES_MAX / 4);
+  waves = blocks * 4;  // one scount entry per wavefront
   k_upsert_fast<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx,
                                                 a.ilist, a.uleaf, a.uver, a.counters, a.slist,
-                                                a.out_vals, a.out_found, a.desc);
+                                                a.scount, a.out_vals, a.out_found, a.desc);
   return hipGetLastError();
 }
 
 template <int F>
 static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
   hipError_t e = hipSuccess;
-  const bool fast = a.uleaf && a.uver && a.ilist && a.slist;
+  const bool fast = a.uleaf && a.uver && a.ilist && a.slist && a.scount;
+  uint32_t fast_waves = 0;
   if (fast) {
     switch (a.lanes) {
-      case 8: e = launch_upsert_fast<F, (F >= 8 ? 8 : F)>(t, a, s); break;
-      case 16: e = launch_upsert_fast<F, (F >= 16 ? 16 : F)>(t, a, s); break;
-      case 32: e = launch_upsert_fast<F, (F >= 32 ? 32 : F)>(t, a, s); break;
+      case 8: e = launch_upsert_fast<F, (F >= 8 ? 8 : F)>(t, a, s, fast_waves); break;
+      case 16: e = launch_upsert_fast<F, (F >= 16 ? 16 : F)>(t, a, s, fast_waves); break;
+      case 32: e = launch_upsert_fast<F, (F >= 32 ? 32 : F)>(t, a, s, fast_waves); break;
       default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
   }
   k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
-                                        fast ? a.slist : nullptr, a.upar, a.counters, a.dlist,
-                                        a.max_attempts, a.out_vals, a.out_found, a.desc);
+                                        fast ? a.slist : nullptr, a.scount, fast_waves, a.upar,
+                                        a.counters, a.dlist, a.max_attempts, a.out_vals,
+                                        a.out_found, a.desc);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_upsert_serial<F><<<1, 64, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.dlist,

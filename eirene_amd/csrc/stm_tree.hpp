@@ -6,6 +6,10 @@
 
 namespace stm {
 
+// Wavefronts of the upsert fast path at most (2048 workgroups of 4); each records how many ops
+// it handed back.
+constexpr uint32_t FAST_WAVES_MAX = 8192;
+
 struct LookupArgs {
   const uint64_t* ukey;
   const uint8_t* uop;
@@ -48,7 +52,9 @@ struct UpsertArgs {
   const uint64_t* uver;   // the hinted leaf's lock word when the lookup kernel ran
   const uint32_t* upar;   // level-1 node hint for the wave-wide path
   const uint32_t* ilist;  // the merged INSERT groups (counters->n_upserts)
-  uint32_t* slist;        // ops the fast path hands back (capacity: groups)
+  uint32_t* slist;        // ops the fast path hands back (capacity: groups), in the handing
+                          //   wavefront's own range of the insert list
+  uint32_t* scount;       // per fast-path wavefront: ops it handed back (capacity: FAST_WAVES_MAX)
   uint32_t lanes;         // lanes per op of the fast path: 8, 16 or 32
 };
 
