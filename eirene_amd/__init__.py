@@ -217,7 +217,7 @@ class StmConfig:
     train_steps: int = 300
     sort_tile_items: int = 16
     use_graphs: bool = True
-    upsert_lanes: int = 16
+    upsert_lanes: int = 0  # 0 = automatic
 
     def to_c(self) -> _Config:
         c = _Config()

@@ -786,7 +786,7 @@ int stm_config_init(stm_config* cfg) {
   cfg->train_steps = 300;
   cfg->sort_tile_items = 16;
   cfg->use_graphs = 1;
-  cfg->upsert_lanes = 16;
+  cfg->upsert_lanes = 0;  // automatic
   return STM_OK;
 }
 
@@ -812,7 +812,9 @@ int stm_create(const stm_config* cfg_in, stm_tree** out) {
   if (cfg.sort_tile_items == 0) cfg.sort_tile_items = 16;
   if (cfg.sort_tile_items != 8 && cfg.sort_tile_items != 16)
     return fail(STM_ERR_INVALID, "sort_tile_items %u not in {8, 16}", cfg.sort_tile_items);
-  if (cfg.upsert_lanes == 0) cfg.upsert_lanes = std::min<uint32_t>(16, cfg.fanout);
+  // Automatic: 8 lanes (8 inserts per wavefront round) while that keeps the leaf in at most 8
+  // elements per lane (measured 0.186 vs 0.206 ms per batch at fanout 64), 16 above.
+  if (cfg.upsert_lanes == 0) cfg.upsert_lanes = cfg.fanout <= 64 ? 8u : 16u;
   if (!(cfg.upsert_lanes == 8 || cfg.upsert_lanes == 16 || cfg.upsert_lanes == 32) ||
       cfg.upsert_lanes > cfg.fanout)
     return fail(STM_ERR_INVALID, "upsert_lanes %u not in {8, 16, 32} or above fanout", cfg.upsert_lanes);

@@ -76,7 +76,8 @@ typedef struct stm_config {
                              size class (default 1), 0 = launch every kernel from the host */
     uint32_t upsert_lanes;/* lanes per insert on the upsert kernel's lane-group fast path:
                              8, 16 or 32, at most fanout (64 / this many inserts per
-                             wavefront at once; default 16) */
+                             wavefront at once); 0 = automatic (default): 8 up to fanout 64,
+                             16 above */
 } stm_config;
 
 /* Per-phase timing slots of stm_stats.phase_ms / phase_launches. */

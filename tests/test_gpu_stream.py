@@ -76,21 +76,21 @@ def check_same(got_v, got_f, want_v, want_f, mask=None):
     assert idx.size == 0, f"{idx.size} mismatches, first at {idx[:5].tolist()}"
 
 
-@pytest.mark.parametrize("fanout,tile,graphs,lanes,group,per_wave", [
-    (32, 8, True, 8, 8, 256),
-    (64, 16, True, 16, 16, 512),
-    (64, 8, True, 32, 32, 1024),
-    (128, 16, True, 16, 16, 512),
-    (64, 16, False, 16, 16, 512),
+@pytest.mark.parametrize("fanout,tile,graphs,lanes,group,per_wave,ulanes", [
+    (32, 8, True, 8, 8, 256, 8),
+    (64, 16, True, 16, 16, 512, 16),
+    (64, 8, True, 32, 32, 1024, 32),
+    (128, 16, True, 16, 16, 512, 0),  # automatic: 16 lanes per insert
+    (64, 16, False, 16, 16, 512, 0),  # automatic: 8
 ])
-def test_stream_matches_oracle(oracle_mod, fanout, tile, graphs, lanes, group, per_wave):
+def test_stream_matches_oracle(oracle_mod, fanout, tile, graphs, lanes, group, per_wave, ulanes):
     rng = np.random.default_rng(fanout + tile)
     load = rng.integers(0, 1 << 63, size=40000, dtype=np.uint64)
     lv = rng.integers(0, 1 << 63, size=load.size, dtype=np.uint64)
     keys, vals, ops, offsets = make_stream(fanout * 7 + tile, load, 40, 1 << 14)
     o, want_v, want_f = oracle_replay(oracle_mod, load, lv, keys, vals, ops, offsets)
     with tree(fanout=fanout, sort_tile_items=tile, use_graphs=graphs, lanes_per_op=lanes,
-              ops_per_group=group, upsert_ops_per_wave=per_wave) as t:
+              ops_per_group=group, upsert_ops_per_wave=per_wave, upsert_lanes=ulanes) as t:
         t.bulk_load(load, lv)
         got_v, got_f = run_stream(t, keys, vals, ops, offsets, calls=2)
         check_same(got_v, got_f, want_v, want_f)
