@@ -334,9 +334,9 @@ int train_router(stm_tree* t, uint32_t steps, bool periodic) {
   if (rc) return rc;
   const uint32_t nn = (uint32_t)std::min<uint64_t>(h.n_nodes, t->capacity);
   const bool index = t->cfg.route == STM_ROUTE_INDEX;
-  if (index) steps = 0;  // the static index only needs the directory
   const bool warm = periodic && t->route_warm;
   if (warm) steps = std::max<uint32_t>(steps / 8, 16);
+  if (index) steps = 0;  // the static index only needs the directory
   // Adam's first steps move every parameter by about the learning rate whatever the
   // gradient's size, so fine-tuning a converged model uses a tenth of it.
   const float lr = warm ? 0.002f : 0.02f;
