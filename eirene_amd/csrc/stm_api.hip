@@ -92,7 +92,12 @@ __global__ void k_epilogue(const BatchCounters* __restrict__ cnt, const BatchDes
   acc->batches += 1;
 }
 
-// Plain streaming copy: the known-size reference kernel for HBM traffic counters.
+// Plain streaming copy of n16 16-byte words: the known-size reference kernel for HBM traffic
+// counters and the measured bandwidth ceiling.  One word per thread over a grid that covers the
+// whole copy: on gfx950 this outruns both a resident grid-stride loop and several words in
+// flight per thread (6.2 TB/s read + write against 4.7-5.6 TB/s at 4 GiB).  The stride loop
+// only serves copies beyond COPY_MAX_BLOCKS blocks.
+constexpr uint32_t COPY_MAX_BLOCKS = 1u << 24;
 __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src,
                                                 uint64_t n16) {
   for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256ull)
@@ -1363,7 +1368,7 @@ int stm_device_copy(void* dst, const void* src, size_t bytes) {
   if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
     return fail(STM_ERR_INVALID, "device copy needs 16-byte aligned pointers and size");
   const uint64_t n16 = bytes / 16;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 8192);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, COPY_MAX_BLOCKS);
   k_copy16<<<blocks, 256, 0, nullptr>>>((uint4*)dst, (const uint4*)src, n16);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipDeviceSynchronize());

@@ -207,3 +207,20 @@ def test_device_api_matches_host_api(oracle_mod):
         t.run_batch_device(dk, dv, do, ov, of, n=n - half, offset=half)
         assert_same((ov.download(), of.download()), want)
         assert_dump(t, o)
+
+
+@pytest.mark.parametrize("n16", [1, 255, 1024, 4 * 1024 * 3 + 5, (1 << 20) + 17])
+def test_device_copy_ragged(n16):
+    """The streaming copy kernel moves every 16-byte word of ragged sizes, including sizes
+    that leave a partial unrolled step and a partial block."""
+    from eirene_amd import DeviceBuffer
+    rng = np.random.default_rng(n16)
+    a = rng.integers(0, 2**63, size=2 * n16, dtype=np.uint64)
+    src = DeviceBuffer.from_array(a)
+    dst = DeviceBuffer.from_array(np.zeros_like(a))
+    try:
+        dst.copy_from(src)
+        assert np.array_equal(dst.download(), a)
+    finally:
+        src.free()
+        dst.free()
