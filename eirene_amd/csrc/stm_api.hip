@@ -153,7 +153,8 @@ struct Scratch {
     A_(upar, cap);
     A_(uver, cap);
     A_(counters, 1);
-    if ((e = hipHostMalloc((void**)&h_counters, sizeof(BatchCounters), 0)) != hipSuccess) return e;
+    if ((e = hipHostMalloc((void**)&h_counters, sizeof(BatchCounters), hipHostMallocCoherent)) != hipSuccess)
+      return e;
     if (staging) {
       A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
       A_(soff, cap); A_(lastins, cap); A_(scan_tiles, scan_tile_words(cap));
@@ -183,6 +184,7 @@ struct stm_tree {
   stm_stats st{};
   hipEvent_t ev_a[STM_NUM_PHASES] = {};
   hipEvent_t ev_b[STM_NUM_PHASES] = {};
+  hipEvent_t ev_begin[STM_NUM_PHASES] = {};  // the event a phase's time starts from
   bool phase_used[STM_NUM_PHASES] = {};
   // learned routing
   RouteState route;
@@ -212,18 +214,27 @@ int set_device(stm_tree* t) {
 void phase_begin(stm_tree* t, int p) {
   if (!t->cfg.timing) return;
   (void)hipEventRecord(t->ev_a[p], t->stream);
+  t->ev_begin[p] = t->ev_a[p];
 }
 void phase_end(stm_tree* t, int p) {
   if (!t->cfg.timing) return;
   (void)hipEventRecord(t->ev_b[p], t->stream);
   t->phase_used[p] = true;
 }
+// Phase `from` ends where phase `to` begins: one event marks both.  Each recorded event sits
+// between two kernels and holds the second back by a few microseconds, so a boundary costs one
+// event instead of two.
+void phase_switch(stm_tree* t, int from, int to) {
+  if (!t->cfg.timing) return;
+  phase_end(t, from);
+  t->ev_begin[to] = t->ev_b[from];
+}
 void phase_collect(stm_tree* t) {
   if (!t->cfg.timing) return;
   for (int p = 0; p < STM_NUM_PHASES; ++p) {
     if (!t->phase_used[p]) continue;
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, t->ev_a[p], t->ev_b[p]) == hipSuccess) {
+    if (hipEventElapsedTime(&ms, t->ev_begin[p], t->ev_b[p]) == hipSuccess) {
       t->st.phase_ms[p] += ms;
       t->st.phase_launches[p] += 1;
     }
@@ -374,6 +385,8 @@ struct PipeIn {
   uint64_t* scan_keys;
   uint64_t* scan_vals;
   uint64_t scan_cap;
+  // optional: the last upsert kernel copies the batch counters here (pinned host memory)
+  BatchCounters* host_counters;
 };
 
 hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
@@ -384,9 +397,8 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   e = radix_sort_pairs_desc(in.keys, in.ops, in.desc, in.n, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, 0,
                             64, s.hist, (int)t->cfg.sort_tile_items, st);
   if (e != hipSuccess) return e;
-  if (timed) phase_end(t, STM_PHASE_SORT);
+  if (timed) phase_switch(t, STM_PHASE_SORT, STM_PHASE_MERGE);
 
-  if (timed) phase_begin(t, STM_PHASE_MERGE);
   FoldArgs fa{};
   fa.skeys = s.skeys;
   fa.sidx = s.sidx;
@@ -406,7 +418,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   fa.lastins = in.scans ? s.lastins : nullptr;
   fa.ilist = s.ilist;
   if ((e = fold_batch(fa, st)) != hipSuccess) return e;
-  if (timed) phase_end(t, STM_PHASE_MERGE);
+  if (timed) phase_switch(t, STM_PHASE_MERGE, STM_PHASE_LOOKUP);
 
   ScanArgs sa{};
   if (in.scans) {
@@ -431,7 +443,6 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
     sa.tile_sums = s.scan_tiles;
   }
 
-  if (timed) phase_begin(t, STM_PHASE_LOOKUP);
   LookupArgs la{};
   if (t->cfg.route != STM_ROUTE_DESCENT) {
     la.route_hdr = t->route.hdr;
@@ -456,16 +467,16 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   la.uver = s.uver;
   la.upar = s.upar;
   if ((e = launch_lookup(t->view, la, st)) != hipSuccess) return e;
-  if (timed) phase_end(t, STM_PHASE_LOOKUP);
 
   if (in.scans) {  // on the tree as it was before the batch: before the upsert kernel
-    if (timed) phase_begin(t, STM_PHASE_SCAN);
+    if (timed) phase_switch(t, STM_PHASE_LOOKUP, STM_PHASE_SCAN);
     if ((e = launch_scan_offsets(sa, st)) != hipSuccess) return e;
     if ((e = launch_range_scan(t->view, sa, st)) != hipSuccess) return e;
-    if (timed) phase_end(t, STM_PHASE_SCAN);
+    if (timed) phase_switch(t, STM_PHASE_SCAN, STM_PHASE_UPSERT);
+  } else if (timed) {
+    phase_switch(t, STM_PHASE_LOOKUP, STM_PHASE_UPSERT);
   }
 
-  if (timed) phase_begin(t, STM_PHASE_UPSERT);
   UpsertArgs ua{};
   ua.ukey = s.ukey;
   ua.uval = s.uval;
@@ -490,6 +501,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   ua.slist = s.slist;
   ua.scount = s.scount;
   ua.lanes = t->cfg.upsert_lanes;
+  ua.host_counters = in.host_counters;
   if ((e = launch_upsert(t->view, ua, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_UPSERT);
   if (in.scans && (e = launch_scan_answers(sa, st)) != hipSuccess) return e;
@@ -519,16 +531,19 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   hipStream_t st = t->stream;
   int rc = maybe_retrain(t);
   if (rc) return rc;
-  PipeIn in{keys, vals, ops, n, out_vals, out_found, nullptr, false, nullptr, nullptr, 0};
+  PipeIn in{keys, vals, ops, n, out_vals, out_found, nullptr, false, nullptr, nullptr, 0, nullptr};
   if (so) {
     in.scans = true;
     in.scan_keys = so->keys;
     in.scan_vals = so->vals;
     in.scan_cap = so->cap;
+  } else {
+    in.host_counters = s.h_counters;  // the scan answers kernel runs after the upsert kernels
   }
   HIP_TRY(enqueue_pipeline(t, in, true));
   // One readback per batch, into pinned memory: counters and the tree's error bits.
-  HIP_TRY(hipMemcpyAsync(s.h_counters, s.counters, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
+  if (so)
+    HIP_TRY(hipMemcpyAsync(s.h_counters, s.counters, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   const BatchCounters bc = *s.h_counters;
   phase_collect(t);

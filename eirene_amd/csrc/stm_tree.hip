@@ -1315,7 +1315,8 @@ __global__ __launch_bounds__(64) void k_upsert_serial(TreeView t, const uint64_t
                                                       const BatchCounters* cnt,
                                                       uint64_t* __restrict__ out_vals,
                                                       uint8_t* __restrict__ out_found,
-                                                      const BatchDesc* __restrict__ desc) {
+                                                      const BatchDesc* __restrict__ desc,
+                                                      BatchCounters* host_out) {
   const uint32_t n = cnt->n_deferred;
   if (desc) {
     out_vals = desc->out_vals;
@@ -1333,7 +1334,22 @@ __global__ __launch_bounds__(64) void k_upsert_serial(TreeView t, const uint64_t
   flush_counters(t, c);
   // The last kernel of a batch: the tree's error bits travel back with the batch counters.
   wave_order();
-  if (lane_id() == 0) const_cast<BatchCounters*>(cnt)->hdr_error = ld_relaxed(&t.hdr->error);
+  const uint64_t err = ld_relaxed(&t.hdr->error);
+  if (lane_id() == 0) const_cast<BatchCounters*>(cnt)->hdr_error = err;
+  if (host_out) {
+    // Lanes 0-7 store the 64-byte block into pinned host memory, which spares the batch a
+    // separate copy on the stream.  Agent-scope loads read the other kernels' counter updates
+    // past this CU's cache; the system fence makes the stores visible before the kernel ends.
+    static_assert(sizeof(BatchCounters) == 64 && offsetof(BatchCounters, hdr_error) == 40,
+                  "hdr_error is word 5");
+    const int l = lane_id();
+    if (l < 8) {
+      const uint64_t w = l == 5 ? err : ld_relaxed(reinterpret_cast<const uint64_t*>(cnt) + l);
+      __hip_atomic_store(reinterpret_cast<uint64_t*>(host_out) + l, w, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __threadfence_system();
+  }
 }
 
 // ================================================================== dump / check
@@ -1559,7 +1575,8 @@ static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStr
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_upsert_serial<F><<<1, 64, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx, a.dlist,
-                                      a.counters, a.out_vals, a.out_found, a.desc);
+                                      a.counters, a.out_vals, a.out_found, a.desc,
+                                      a.host_counters);
   return hipGetLastError();
 }
 

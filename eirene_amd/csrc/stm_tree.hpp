@@ -56,6 +56,8 @@ struct UpsertArgs {
                           //   wavefront's own range of the insert list
   uint32_t* scount;       // per fast-path wavefront: ops it handed back (capacity: FAST_WAVES_MAX)
   uint32_t lanes;         // lanes per op of the fast path: 8, 16 or 32
+  BatchCounters* host_counters;  // optional: pinned host memory the last kernel copies the
+                                 //   counters into (null: the caller copies them back)
 };
 
 hipError_t launch_build_leaves(const TreeView& t, const uint64_t* ukey, const uint64_t* uval,
