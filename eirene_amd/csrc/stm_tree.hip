@@ -240,8 +240,15 @@ STM_DEV bool learned_route(const RouteView& rv, uint64_t key, int g, int gbase, 
 // Stripe of the statistics counters a wavefront adds to.
 STM_DEV uint32_t wave_stripe() { return blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; }
 
+// Latency-bound: resident wavefronts hide the node loads.  For F <= 64 and G >= 16 the
+// register budget of seven wavefronts per SIMD (72 VGPRs) is requested; the learned variant
+// otherwise lands one register above it and loses a seventh of its wavefronts.  Narrower lane
+// groups hold more slots per lane and would spill.
+template <int F, int G>
+constexpr int lookup_min_waves() { return (F <= 64 && G >= 16) ? 7 : 1; }
+
 template <int F, int G, bool LEARNED, bool INDEX>
-__global__ __launch_bounds__(256) void k_lookup(TreeView t, const uint64_t* __restrict__ ukey,
+__global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(TreeView t, const uint64_t* __restrict__ ukey,
                                                 const uint8_t* __restrict__ uop,
                                                 const uint32_t* __restrict__ gstart,
                                                 const uint32_t* __restrict__ npend,
