@@ -92,6 +92,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def record_name(path):
+    """A tuning record's path relative to the repository when it lies inside it."""
+    p = Path(path).resolve()
+    try:
+        return str(p.relative_to(ROOT.resolve()))
+    except ValueError:
+        return str(p)
+
+
 def copy_ceiling_gbs(ea, mib=4096, reps=4):
     """Read + write GB/s of the library's streaming copy kernel over mib MiB: the achievable
     HBM rate beside the nominal peak."""
@@ -410,8 +419,7 @@ def main():
                    "defaults": {"fanout": args.fanout, "lanes_per_op": args.lanes,
                                 "ops_per_group": args.group},
                    "descent_tuned": tuned,
-                   "tuning_record": (str(Path(args.tuning_headline).relative_to(ROOT))
-                                     if tuned else None),
+                   "tuning_record": record_name(args.tuning_headline) if tuned else None,
                    "parallelism": "single GPU"},
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
