@@ -309,9 +309,12 @@ def main():
     routes = ["descent", "learned", "index"] if args.route == "both" else [args.route]
     if tuned and "descent" in routes:  # the record was tuned with root-to-leaf descent
         routes.append("descent_tuned")
+    if tuned and "learned" in routes:  # the same launch parameters under learned routing
+        routes.append("learned_tuned")
     res = {r: run_route(r) for r in routes}
-    # The workload is the learned-routing configuration: `value` is its learned route.
-    head_name = "learned" if "learned" in res else routes[0]
+    # The workload is the learned-routing configuration: `value` is its learned route, with the
+    # autotuned launch parameters when a tuning record is present.
+    head_name = next((r for r in ("learned_tuned", "learned") if r in res), routes[0])
     head = res[head_name]
     st, elapsed, lat = head["st"], head["elapsed"], head["lat"]
     mops = head["mops"]
@@ -418,14 +421,15 @@ def main():
                    "global_batch": n_batch, "value_route": head_name,
                    "defaults": {"fanout": args.fanout, "lanes_per_op": args.lanes,
                                 "ops_per_group": args.group},
-                   "descent_tuned": tuned,
+                   "tuned": tuned,  # the launch parameters of descent_tuned and learned_tuned
                    "tuning_record": record_name(args.tuning_headline) if tuned else None,
                    "parallelism": "single GPU"},
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "routes": {r: summary(r) for r in res},
-        "learned_over_plain": {("descent" + k): round(res["learned"]["mops"] / res["descent" + k]["mops"], 4)
-                               for k in ("", "_tuned") if "learned" in res and "descent" + k in res},
+        # learned over plain descent at the same launch parameters (defaults, tuned)
+        "learned_over_plain": {("descent" + k): round(res["learned" + k]["mops"] / res["descent" + k]["mops"], 4)
+                               for k in ("", "_tuned") if "learned" + k in res and "descent" + k in res},
         # the same leaf directory searched without the model (a rebuilt static index)
         "learned_over_index": (round(res["learned"]["mops"] / res["index"]["mops"], 4)
                                if "learned" in res and "index" in res else None),
