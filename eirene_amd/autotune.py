@@ -21,6 +21,7 @@ The result is a JSON tuning file:
 read back with `load_tuning()`.
 
     python -m eirene_amd.autotune --out tuning.json [--keys-log2 22] [--batches 48]
+        [--route learned --retrain-every 8]
 """
 from __future__ import annotations
 
@@ -33,7 +34,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from . import DeviceBuffer, ROUTE_DESCENT, StmConfig, StmTree
+from . import DeviceBuffer, ROUTE_DESCENT, ROUTE_LEARNED, StmConfig, StmTree
 
 SPACE = {
     "lanes_per_op": [8, 16, 32],
@@ -70,6 +71,7 @@ def _trial(sample: Sample, cfg: StmConfig, repeats: int = 2) -> float:
     """Mops/s of the sample stream under cfg: best of `repeats` timed replays, each on a
     fresh tree bulk-loaded from the same keys (so every trial starts from the same tree)."""
     best = 0.0
+    routed = cfg.route != ROUTE_DESCENT
     with StmTree(cfg) as t:
         t.bulk_load_device(sample.load_k, sample.load_v, sample.n_load)
         # untimed pass: graph capture and first-touch costs
@@ -78,6 +80,8 @@ def _trial(sample: Sample, cfg: StmConfig, repeats: int = 2) -> float:
     for _ in range(repeats):
         with StmTree(cfg) as t:
             t.bulk_load_device(sample.load_k, sample.load_v, sample.n_load)
+            if routed:  # the first router build trains from scratch: outside the timed stream
+                t.retrain()
             t.sync()
             t0 = time.perf_counter()
             t.run_batches_device(sample.k, sample.v, sample.o, sample.out_v, sample.out_f,
@@ -175,6 +179,10 @@ def main() -> None:
     ap.add_argument("--lo", type=float, default=10)
     ap.add_argument("--hi", type=float, default=22)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--route", choices=["descent", "learned"], default="descent",
+                    help="leaf routing the trials run with")
+    ap.add_argument("--retrain-every", type=int, default=8,
+                    help="batches between router retrains (learned route)")
     args = ap.parse_args()
     n = 1 << args.keys_log2
     if args.dist == "gamma":
@@ -188,12 +196,16 @@ def main() -> None:
                                              dist, p0, p1)
     sample = Sample(load, w.values_for(load), keys, vals, ops, offsets)
     base = StmConfig(max_keys=n + int(offsets[-1]) + (1 << 20), max_batch=1 << int(args.hi),
-                     route=ROUTE_DESCENT)
+                     route=ROUTE_LEARNED if args.route == "learned" else ROUTE_DESCENT,
+                     retrain_every=args.retrain_every)
     rec = tune(sample, base, log=print)
     sample.free()
     size = (f"of 2^{args.hi:g} ops" if args.lo == args.hi
             else f"log-uniform in [2^{args.lo:g}, 2^{args.hi:g}]")
     rec["workload"] = f"2^{args.keys_log2} {dname} keys, {args.batches} batches {size}, 95/5 lookup/insert"
+    rec["route"] = args.route
+    if args.route == "learned":
+        rec["retrain_every"] = args.retrain_every
     Path(args.out).write_text(json.dumps(rec, indent=1) + "\n")
     print(json.dumps({k: rec[k] for k in ("best", "best_mops", "default_mops")}))
 
