@@ -11,8 +11,9 @@ Search space (every point is a valid StmConfig):
 Method: for each fanout (the tree must be rebuilt for it), coordinate descent over the other
 parameters from the library defaults, two sweeps, keeping a change only when it is faster by
 more than `min_gain`.  A trial bulk-loads a fresh tree from device-resident keys and replays the
-sample stream once untimed (graph capture, first touch), then times the stream on fresh trees
-loaded from the same keys, so every trial starts from the same tree; throughput is ops / wall
+sample stream once untimed (first touch), then times the stream on fresh trees loaded from the
+same keys, so every trial starts from the same tree.  Each timed tree captures its graphs and
+(with learned routing) trains its router before the clock starts; throughput is ops / wall
 time of stm_run_batches_device, which returns after the stream has completed.
 
 The result is a JSON tuning file:
@@ -74,7 +75,7 @@ def _trial(sample: Sample, cfg: StmConfig, repeats: int = 2) -> float:
     routed = cfg.route != ROUTE_DESCENT
     with StmTree(cfg) as t:
         t.bulk_load_device(sample.load_k, sample.load_v, sample.n_load)
-        # untimed pass: graph capture and first-touch costs
+        # untimed pass: first-touch costs
         t.run_batches_device(sample.k, sample.v, sample.o, sample.out_v, sample.out_f,
                              sample.offsets)
     for _ in range(repeats):
@@ -82,6 +83,7 @@ def _trial(sample: Sample, cfg: StmConfig, repeats: int = 2) -> float:
             t.bulk_load_device(sample.load_k, sample.load_v, sample.n_load)
             if routed:  # the first router build trains from scratch: outside the timed stream
                 t.retrain()
+            t.capture_graphs()  # graphs belong to the tree: capture them before the clock starts
             t.sync()
             t0 = time.perf_counter()
             t.run_batches_device(sample.k, sample.v, sample.o, sample.out_v, sample.out_f,
