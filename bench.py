@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU-baseline budget (batches run until it is spent)")
+    ap.add_argument("--no-stream", dest="stream", action="store_false",
+                    help="time only one synchronous call per batch (no stream runs)")
     ap.add_argument("--no-headline", action="store_true",
                     help="skip the learned-vs-plain section (profiling the other sections alone)")
     ap.add_argument("--serve-requests", type=int, default=8000,
@@ -266,7 +268,7 @@ def main():
         from eirene_amd import autotune as at
         tuned = at.load_tuning(args.tuning_headline)["best"]
 
-    def run_route(route):
+    def route_config(route):
         cfg = ea.StmConfig(fanout=args.fanout, max_keys=n_load + n_batches * n_batch + (1 << 20),
                            max_batch=n_batch, lanes_per_op=args.lanes, ops_per_group=args.group,
                            route={"learned": ea.ROUTE_LEARNED, "index": ea.ROUTE_INDEX}.get(
@@ -275,6 +277,18 @@ def main():
         if route.endswith("_tuned"):
             from eirene_amd import autotune as at
             cfg = at.apply_tuning(cfg, tuned)
+        return cfg
+
+    tail_batches = min(8, args.steps)
+
+    def tail():
+        """Answers of the last timed batches, to check other runs against."""
+        o = (n_batches - tail_batches) * n_batch
+        return (out_v.download(tail_batches * n_batch, offset=o),
+                out_f.download(tail_batches * n_batch, offset=o))
+
+    def run_route(route):
+        cfg = route_config(route)
         tree = ea.StmTree(cfg)
         t0 = time.time()
         dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(load_vals)
@@ -301,7 +315,32 @@ def main():
         mops = args.steps * n_batch / elapsed / 1e6
         log(f"[{route}] {args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s; "
             f"phases {st['phase_ms']}")
-        return {"mops": mops, "elapsed": elapsed, "lat": lat, "st": st, "first": first, "info": info}
+        return {"mops": mops, "elapsed": elapsed, "lat": lat, "st": st, "first": first, "info": info,
+                "tail": tail() if args.stream else None}
+
+    def run_stream(route):
+        """The same batches through stm_run_batches_device: the warm-up batches as one stream,
+        then the timed batches as one device-resident stream (one captured graph per size class,
+        replayed; retrains between replays), on a fresh tree from the same bulk load."""
+        cfg = route_config(route)
+        cfg.timing = False
+        tree = ea.StmTree(cfg)
+        dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(load_vals)
+        tree.bulk_load_device(dk, dv)
+        dk.free()
+        dv.free()
+        tree.capture_graphs()
+        offs = np.arange(n_batches + 1, dtype=np.uint64) * np.uint64(n_batch)
+        if args.warmup:
+            tree.run_batches_device(bk, bv, bo, out_v, out_f, offs[:args.warmup + 1])
+        tree.sync()
+        t0 = time.perf_counter()
+        tree.run_batches_device(bk, bv, bo, out_v, out_f, offs[args.warmup:])
+        elapsed = time.perf_counter() - t0
+        tree.close()
+        mops = args.steps * n_batch / elapsed / 1e6
+        log(f"[{route} stream] {args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s")
+        return {"mops": mops, "elapsed": elapsed, "tail": tail()}
 
     bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
@@ -312,12 +351,26 @@ def main():
     if tuned and "learned" in routes:  # the same launch parameters under learned routing
         routes.append("learned_tuned")
     res = {r: run_route(r) for r in routes}
-    # The workload is the learned-routing configuration: `value` is its learned route at the
-    # library's default launch parameters (the tuned routes are reported beside it).
-    head_name = "learned" if "learned" in res else routes[0]
+    # The workload is the learned-routing configuration: `value` is its learned route, at the
+    # tuned launch parameters when a tuning record is present, with the timed batches submitted
+    # as one device-resident stream.  Plain descent runs the same way beside it.  Per-call
+    # latency and the lookup kernel's roofline come from the synchronous run of the same route.
+    head_name = next((r for r in ("learned_tuned", "learned") if r in res), routes[0])
     head = res[head_name]
-    st, elapsed, lat = head["st"], head["elapsed"], head["lat"]
-    mops = head["mops"]
+    st, lat = head["st"], head["lat"]
+    streams = {}
+    if args.stream and head_name.startswith("learned"):
+        plain = head_name.replace("learned", "descent")
+        for r in (head_name, plain):
+            if r in res:
+                streams[r] = run_stream(r)
+                x = streams[r]
+                x["identical_to_sync"] = bool(np.array_equal(x["tail"][0], res[r]["tail"][0]) and
+                                              np.array_equal(x["tail"][1], res[r]["tail"][1]))
+    if head_name in streams:
+        mops, elapsed = streams[head_name]["mops"], streams[head_name]["elapsed"]
+    else:
+        mops, elapsed = head["mops"], head["elapsed"]
     total_ops = args.steps * n_batch
     same = None
     firsts = [x["first"] for x in res.values() if x["first"] is not None]
@@ -419,14 +472,25 @@ def main():
                                f"keys, {args.steps} batches of 2^{args.batch_log2} ops, 95% lookup "
                                f"(90% hits) / 5% insert, router retrained every 8 batches",
                    "global_batch": n_batch, "value_route": head_name,
+                   "value_api": ("stm_run_batches_device: the timed batches as one device-resident "
+                                 "stream (graph replay)" if head_name in streams else
+                                 "stm_run_batch_device: one synchronous call per batch"),
                    "defaults": {"fanout": args.fanout, "lanes_per_op": args.lanes,
                                 "ops_per_group": args.group},
                    "tuned": tuned,  # the launch parameters of descent_tuned and learned_tuned
                    "tuning_record": record_name(args.tuning_headline) if tuned else None,
                    "parallelism": "single GPU"},
+        # per synchronous call of the value route (stm_run_batch_device, one batch per call)
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
-        "routes": {r: summary(r) for r in res},
+        "routes": {r: summary(r) for r in res},  # one synchronous call per batch
+        "streams": {r: {"mops": round(x["mops"], 2),
+                        "ms_per_batch": round(x["elapsed"] / args.steps * 1e3, 3),
+                        "last_batches_identical_to_sync": x["identical_to_sync"]}
+                    for r, x in streams.items()},
+        "learned_over_plain_stream": (round(streams[head_name]["mops"] /
+                                            streams[head_name.replace("learned", "descent")]["mops"], 4)
+                                      if len(streams) == 2 else None),
         # learned over plain descent at the same launch parameters (defaults, tuned)
         "learned_over_plain": {("descent" + k): round(res["learned" + k]["mops"] / res["descent" + k]["mops"], 4)
                                for k in ("", "_tuned") if "learned" + k in res and "descent" + k in res},

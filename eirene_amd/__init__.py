@@ -175,10 +175,15 @@ class DeviceBuffer:
             raise ValueError("array larger than the buffer")
         _check(load_library().stm_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes))
 
-    def download(self, n: Optional[int] = None) -> np.ndarray:
-        n = self.n if n is None else int(n)
+    def download(self, n: Optional[int] = None, offset: int = 0) -> np.ndarray:
+        """Elements [offset, offset + n) (default: to the end) copied to the host."""
+        offset = int(offset)
+        n = self.n - offset if n is None else int(n)
+        if offset < 0 or n < 0 or offset + n > self.n:
+            raise ValueError("range outside the buffer")
         out = np.empty(n, dtype=self.dtype)
-        _check(load_library().stm_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        _check(load_library().stm_memcpy_d2h(out.ctypes.data, self.ptr + offset * self.dtype.itemsize,
+                                             out.nbytes))
         return out
 
     def copy_from(self, other: "DeviceBuffer") -> None:
