@@ -69,6 +69,9 @@ def parse():
                     help="skip the learned-vs-plain section (profiling the other sections alone)")
     ap.add_argument("--serve-requests", type=int, default=8000,
                     help="serving loop: requests offered (0 = skip)")
+    ap.add_argument("--serve-warmup", type=int, default=2000,
+                    help="serving requests run before the measured ones (device clocks and host "
+                         "threads reach steady state)")
     ap.add_argument("--serve-mops", type=float, default=200.0,
                     help="serving loop: offered load in Mops/s (Poisson arrivals)")
     ap.add_argument("--uneven-batches", type=int, default=256,
@@ -207,7 +210,7 @@ def run_serving(args, ea, w):
         log(f"[serve] loaded 2^25 keys in {time.time() - t0:.1f}s")
         rep = open_loop(t, srt, requests=args.serve_requests, offered_mops=args.serve_mops,
                         ops_log2=(0, 16), lookup_frac=0.90, insert_frac=0.08, scan_len=(16, 256),
-                        seed=args.seed + 7)
+                        seed=args.seed + 7, warmup=args.serve_warmup)
         bad = t.check()
     log(f"[serve] {rep['requests']} requests, {rep['mops']:.1f} Mops/s, p50 {rep['p50_ms']:.2f} ms, "
         f"p99 {rep['p99_ms']:.2f} ms, {rep['batches']} batches")
@@ -222,6 +225,8 @@ def run_serving(args, ea, w):
         "max_pending": rep["max_pending"], "busy_frac": round(rep["busy_frac"], 3),
         "scans": rep["scans"], "scan_pairs": rep["scan_pairs"],
         "hbm_budget_bytes": rep["hbm_budget"], "slot_ops": rep["slot_ops"],
+        # requests run first on the same arrival process, outside the figures above
+        "warmup_requests": rep["warmup_requests"], "warmup_p99_ms": round(rep["warmup_p99_ms"], 3),
         "tree_check_violations": bad,
     }
 
@@ -346,9 +351,10 @@ def main():
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
     out_f = ea.DeviceBuffer(keys.size, np.uint8)
     routes = ["descent", "learned", "index"] if args.route == "both" else [args.route]
-    if tuned and "descent" in routes:  # the record was tuned with root-to-leaf descent
+    # The tuning record's launch parameters under both routings.
+    if tuned and "descent" in routes:
         routes.append("descent_tuned")
-    if tuned and "learned" in routes:  # the same launch parameters under learned routing
+    if tuned and "learned" in routes:
         routes.append("learned_tuned")
     res = {r: run_route(r) for r in routes}
     # The workload is the learned-routing configuration: `value` is its learned route, at the

@@ -35,6 +35,10 @@ constexpr int OFF_BLOCK = 256;
 constexpr int OFF_ITEMS = 16;
 constexpr int OFF_TILE = OFF_BLOCK * OFF_ITEMS;
 constexpr uint32_t SCAN_GRID_CAP = 2048;
+// Wavefronts that share the scans of one 64-op window.  A scan is a chain of dependent leaf
+// reads, so a window holding several scans set the kernel's time when one wavefront ran them
+// all.
+constexpr uint32_t SCAN_SPLIT = 4;
 constexpr uint32_t SCAN_ROUND_CAP = 1u << 24;
 
 STM_DEV uint64_t scan_len(uint8_t op, uint64_t v) {
@@ -319,12 +323,20 @@ __global__ __launch_bounds__(256) void k_range_scan(TreeView t, ScanArgs a) {
   const uint64_t wave0 = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / WAVE;
   const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / WAVE;
   const uint64_t nwin = ((uint64_t)in.n + WAVE - 1) / WAVE;
-  for (uint64_t w = wave0; w < nwin; w += nwaves) {
+  const uint64_t lt = lanemask_lt();
+  // Work item p = (window p / SCAN_SPLIT, share p % SCAN_SPLIT): the wavefront runs the scans
+  // of the window whose rank among the window's scans is p % SCAN_SPLIT modulo SCAN_SPLIT, so
+  // a window with several scans spreads them over SCAN_SPLIT wavefronts.
+  for (uint64_t p = wave0; p < nwin * SCAN_SPLIT; p += nwaves) {
+    const uint64_t w = p / SCAN_SPLIT;
+    const uint32_t share = (uint32_t)(p % SCAN_SPLIT);
     const uint64_t i_l = w * WAVE + lane;
     const bool is_scan = i_l < in.n && in.ops[i_l] == OP_RANGE;
-    uint64_t todo = ballot(is_scan);
+    const uint32_t rank = (uint32_t)__popcll(ballot(is_scan) & lt);
+    const bool mine = is_scan && rank % SCAN_SPLIT == share;
+    uint64_t todo = ballot(mine);
     uint64_t k_l = 0, v_l = 0, off_l = 0;
-    if (is_scan) {
+    if (mine) {
       k_l = in.keys[i_l];
       v_l = in.vals[i_l];
       off_l = a.soff[i_l];
@@ -377,7 +389,7 @@ hipError_t launch_scan_offsets(const ScanArgs& a, hipStream_t s) {
 
 template <int F>
 static hipError_t launch_range_scan_f(const TreeView& t, const ScanArgs& a, hipStream_t s) {
-  const uint64_t waves = ((uint64_t)a.n + WAVE - 1) / WAVE;
+  const uint64_t waves = ((uint64_t)a.n + WAVE - 1) / WAVE * SCAN_SPLIT;
   const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, SCAN_GRID_CAP));
   k_range_scan<F><<<blocks, 256, 0, s>>>(t, a);
   return hipGetLastError();

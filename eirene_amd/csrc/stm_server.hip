@@ -66,6 +66,12 @@ constexpr uint64_t BYTES_PER_OP = 8 + 8 + 1 + 8 + 1;  // key, value, op in; valu
 constexpr uint64_t BYTES_PER_PAIR = 16;
 constexpr unsigned POOL_THREADS = 8;        // helpers for large copy-ins and answer scatters
 constexpr uint64_t PARALLEL_BYTES = 4u << 20;  // a copy this large is split over the pool
+// Without a max_batch_ops of the caller's, a forming batch takes no request that would carry it
+// past this many ops (a larger request still runs alone).  Unbounded batches let a burst grow
+// the answer scatter of one batch past what the completer keeps up with: at 200 Mops/s offered
+// (2^25 keys, requests of 1..2^16 ops with 2 % scans) unbounded batches gave a p99 of 25-40 ms
+// and batches of at most 2^18 ops 7 ms.
+constexpr uint64_t SOFT_BATCH_OPS = 1u << 18;
 
 // A fixed set of helper threads that run the parts of one job together with the calling
 // thread.  Jobs from different threads run one after another.
@@ -192,6 +198,7 @@ struct stm_server {
   int device = 0;
   stm_server_config cfg{};
   uint64_t budget = 0;
+  uint64_t soft_ops = SOFT_BATCH_OPS;  // a batch closes to requests that would pass this
   hipStream_t s_in = nullptr, s_out = nullptr, s_tree = nullptr;
   std::vector<std::unique_ptr<Slot>> slots;
 
@@ -228,6 +235,7 @@ struct stm_server {
       Req* r = pending.front();
       Slot* s = forming;
       if (s->n + r->n > s->cap_ops || s->pairs + r->pairs > s->cap_pairs) break;
+      if (s->n && s->n + r->n > soft_ops) break;
       pending.pop_front();
       r->off = s->n;
       r->poff = s->pairs;
@@ -456,6 +464,7 @@ int stm_server_create(stm_tree* tree, const stm_server_config* cfg_in, stm_serve
   s->device = dev;
   s->cfg = cfg;
   s->budget = budget;
+  s->soft_ops = cfg.max_batch_ops ? ~0ull : SOFT_BATCH_OPS;
   s->s_tree = (hipStream_t)stream;
   if (hipStreamCreate(&s->s_in) != hipSuccess || hipStreamCreate(&s->s_out) != hipSuccess) {
     s->free_all();
@@ -489,6 +498,15 @@ int stm_server_create(stm_tree* tree, const stm_server_config* cfg_in, stm_serve
                    (unsigned long long)cap_ops, (unsigned long long)cap_pairs);
     }
     s->free_slots.push_back(p);
+  }
+  // The first command on a stream sets up its hardware queue, which takes milliseconds: one
+  // byte each way here keeps that off the first requests' latency.
+  Slot* p0 = s->slots[0].get();
+  if (hipMemcpyAsync(p0->d_ops, p0->h_ops, 1, hipMemcpyHostToDevice, s->s_in) != hipSuccess ||
+      hipMemcpyAsync(p0->h_ofound, p0->d_ofound, 1, hipMemcpyDeviceToHost, s->s_out) != hipSuccess ||
+      hipStreamSynchronize(s->s_in) != hipSuccess || hipStreamSynchronize(s->s_out) != hipSuccess) {
+    s->free_all();
+    return sfail(STM_ERR_DEVICE, "server stream set-up");
   }
   s->pool = std::make_unique<Pool>(POOL_THREADS);
   s->runner = std::thread([s] { s->run_loop(); });
@@ -619,25 +637,29 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
                         const uint64_t* loaded_sorted, size_t n_loaded, stm_serve_report* out,
                         double* latencies_ms) {
   if (!tree || !w || !out || !loaded_sorted || !n_loaded) return sfail(STM_ERR_INVALID, "null argument");
+  // The first W requests warm the loop up (device clocks, host threads) on the same arrival
+  // process; the latency figures cover the R requests after them.
+  const size_t W = (size_t)w->warmup_requests;
   const size_t R = (size_t)w->requests;
   if (R == 0 || !(w->offered_mops > 0)) return sfail(STM_ERR_INVALID, "no load to offer");
+  const size_t RT = R + W;
   std::memset(out, 0, sizeof(*out));
   // Request sizes, arrival times (Poisson at offered_mops), then the request data.
-  std::vector<uint64_t> sizes(R);
-  stm_gen_batch_sizes(R, w->seed, w->ops_lo_log2, w->ops_hi_log2, sizes.data());
-  std::vector<uint64_t> off(R + 1, 0);
-  for (size_t r = 0; r < R; ++r) off[r + 1] = off[r] + std::max<uint64_t>(sizes[r], 1);
-  const double mean_ops = (double)off[R] / (double)R;
+  std::vector<uint64_t> sizes(RT);
+  stm_gen_batch_sizes(RT, w->seed, w->ops_lo_log2, w->ops_hi_log2, sizes.data());
+  std::vector<uint64_t> off(RT + 1, 0);
+  for (size_t r = 0; r < RT; ++r) off[r + 1] = off[r] + std::max<uint64_t>(sizes[r], 1);
+  const double mean_ops = (double)off[RT] / (double)RT;
   const double lambda = w->offered_mops * 1e6 / mean_ops;  // requests per second
-  std::vector<double> arrival(R);
+  std::vector<double> arrival(RT);
   double t = 0;
-  for (size_t r = 0; r < R; ++r) {
+  for (size_t r = 0; r < RT; ++r) {
     const uint64_t h = stm_splitmix64(w->seed ^ stm_splitmix64(0xa441 + r));
     const double u = ((h >> 11) + 1) * (1.0 / 9007199254740992.0);
     t += -std::log(u) / lambda;
     arrival[r] = t;
   }
-  const uint64_t N = off[R];
+  const uint64_t N = off[RT];
   std::vector<uint64_t> keys(N), vals(N), ovals(N);
   std::vector<uint8_t> ops(N), ofound(N);
   {
@@ -646,7 +668,7 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
     std::atomic<int> bad{0};
     for (unsigned k = 0; k < T; ++k)
       th.emplace_back([&, k] {
-        for (size_t r = k; r < R; r += T)
+        for (size_t r = k; r < RT; r += T)
           if (stm_gen_serving_request(loaded_sorted, n_loaded, w->seed, r, off[r + 1] - off[r],
                                       w->lookup_frac, w->insert_frac, w->scan_lo, w->scan_hi,
                                       keys.data() + off[r], vals.data() + off[r], ops.data() + off[r]))
@@ -655,14 +677,14 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
     for (auto& x : th) x.join();
     if (bad) return sfail(STM_ERR_INVALID, "serving workload parameters");
   }
-  std::vector<uint64_t> poff(R + 1, 0);
-  for (size_t r = 0; r < R; ++r) {
+  std::vector<uint64_t> poff(RT + 1, 0);
+  for (size_t r = 0; r < RT; ++r) {
     uint64_t p = 0;
     for (uint64_t i = off[r]; i < off[r + 1]; ++i)
       if (ops[i] == STM_OP_RANGE) p += vals[i];
     poff[r + 1] = poff[r] + p;
   }
-  std::vector<uint64_t> skeys(std::max<uint64_t>(poff[R], 1)), svals(std::max<uint64_t>(poff[R], 1));
+  std::vector<uint64_t> skeys(std::max<uint64_t>(poff[RT], 1)), svals(std::max<uint64_t>(poff[RT], 1));
 
   stm_server* srv = nullptr;
   int rc = stm_server_create(tree, cfg, &srv);
@@ -670,14 +692,14 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
   // CLIENTS threads submit the requests (client c takes every CLIENTS-th), each at its
   // arrival time, so copying requests in does not delay later arrivals.
   constexpr unsigned CLIENTS = 4;
-  std::vector<uint64_t> tickets(R, 0);
+  std::vector<uint64_t> tickets(RT, 0);
   std::atomic<int> submit_rc{STM_OK};
   const double t0 = now_s() + 0.005;
   {
     std::vector<std::thread> clients;
     for (unsigned cl = 0; cl < CLIENTS; ++cl)
       clients.emplace_back([&, cl] {
-        for (size_t r = cl; r < R && submit_rc.load() == STM_OK; r += CLIENTS) {
+        for (size_t r = cl; r < RT && submit_rc.load() == STM_OK; r += CLIENTS) {
           const double due = t0 + arrival[r];
           for (double n = now_s(); n < due; n = now_s())
             if (due - n > 300e-6) std::this_thread::sleep_for(std::chrono::microseconds(100));
@@ -692,15 +714,20 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
     for (auto& x : clients) x.join();
   }
   rc = submit_rc.load();
-  std::vector<double> lat;
+  std::vector<double> lat, warm;
   lat.reserve(R);
+  warm.reserve(W);
   uint64_t failed = 0;
-  for (size_t r = 0; r < R; ++r) {
+  for (size_t r = 0; r < RT; ++r) {
     if (!tickets[r]) continue;
     double l = 0;
     if (stm_server_wait(srv, tickets[r], &l) != STM_OK) failed += 1;
+    if (r < W) {
+      warm.push_back(l * 1e3);
+      continue;
+    }
     lat.push_back(l * 1e3);
-    if (latencies_ms) latencies_ms[r] = l * 1e3;
+    if (latencies_ms) latencies_ms[r - W] = l * 1e3;
   }
   const double t_end = now_s();
   stm_server_stats st{};
@@ -709,30 +736,36 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
   if (rc) return rc;
   std::vector<double> sorted_lat = lat;
   std::sort(sorted_lat.begin(), sorted_lat.end());
-  auto pct = [&](double q) {
-    if (sorted_lat.empty()) return 0.0;
-    const size_t i = std::min(sorted_lat.size() - 1, (size_t)std::floor(q * (sorted_lat.size() - 1) + 0.5));
-    return sorted_lat[i];
+  auto pct_of = [](const std::vector<double>& v, double q) {
+    if (v.empty()) return 0.0;
+    const size_t i = std::min(v.size() - 1, (size_t)std::floor(q * (v.size() - 1) + 0.5));
+    return v[i];
   };
+  auto pct = [&](double q) { return pct_of(sorted_lat, q); };
+  std::sort(warm.begin(), warm.end());
   double sum = 0;
   for (double x : lat) sum += x;
+  // Measured window: from the first measured arrival to the last completion.
+  const double t_first = t0 + arrival[W];
   out->requests = R;
-  out->ops = N;
+  out->ops = N - off[W];
   out->batches = st.batches;
   out->scans = st.scans;
   out->scan_pairs = st.scan_pairs;
   out->max_pending = st.max_pending;
   out->failed = failed;
-  out->wall_s = t_end - t0;
-  out->mops = N / out->wall_s / 1e6;
-  out->offered_mops = R > 1 ? N / (arrival[R - 1] - arrival[0]) / 1e6 : 0;
+  out->wall_s = t_end - t_first;
+  out->mops = out->ops / out->wall_s / 1e6;
+  out->offered_mops = R > 1 ? out->ops / (arrival[RT - 1] - arrival[W]) / 1e6 : 0;
   out->p50_ms = pct(0.50);
   out->p90_ms = pct(0.90);
   out->p99_ms = pct(0.99);
   out->p999_ms = pct(0.999);
   out->max_ms = sorted_lat.empty() ? 0 : sorted_lat.back();
   out->mean_ms = lat.empty() ? 0 : sum / lat.size();
-  out->busy_frac = st.busy_s / out->wall_s;
+  out->busy_frac = st.busy_s / (t_end - t0);  // over the whole run, warm-up included
+  out->warmup_requests = W;
+  out->warmup_p99_ms = pct_of(warm, 0.99);
   out->mean_batch_ops = st.batches ? (double)st.ops / st.batches : 0;
   out->hbm_budget = st.hbm_budget;
   out->slot_ops = st.slot_ops;

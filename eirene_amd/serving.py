@@ -37,7 +37,7 @@ class _Workload(C.Structure):
                 ("ops_lo_log2", C.c_uint32), ("ops_hi_log2", C.c_uint32),
                 ("lookup_frac", C.c_double), ("insert_frac", C.c_double),
                 ("scan_lo", C.c_uint32), ("scan_hi", C.c_uint32), ("seed", C.c_uint64),
-                ("reserved", C.c_uint32 * 4)]
+                ("warmup_requests", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class _Report(C.Structure):
@@ -45,7 +45,8 @@ class _Report(C.Structure):
                                           "max_pending", "failed")] + [
         (n, C.c_double) for n in ("wall_s", "mops", "offered_mops", "p50_ms", "p90_ms", "p99_ms",
                                   "p999_ms", "max_ms", "mean_ms", "busy_frac", "mean_batch_ops")] + [
-        ("hbm_budget", C.c_uint64), ("slot_ops", C.c_uint64)]
+        ("hbm_budget", C.c_uint64), ("slot_ops", C.c_uint64), ("warmup_requests", C.c_uint64),
+        ("warmup_p99_ms", C.c_double)]
 
 
 _P = C.c_void_p
@@ -166,13 +167,17 @@ def now() -> float:
 def open_loop(tree: StmTree, loaded_sorted, requests: int, offered_mops: float,
               ops_log2=(0, 16), lookup_frac=0.90, insert_frac=0.08, scan_len=(16, 256),
               seed: int = 1, hbm_budget: int = 0, slots: int = 3, max_batch_ops: int = 0,
-              scan_fraction: float = 0.5, return_latencies: bool = False) -> Dict:
-    """Native open-loop run of the serving loop; returns its report (latencies in ms)."""
+              scan_fraction: float = 0.5, return_latencies: bool = False,
+              warmup: int = 0) -> Dict:
+    """Native open-loop run of the serving loop; returns its report (latencies in ms).  The
+    first `warmup` requests run on the same arrival process before the measured ones and are
+    left out of the report's figures (their p99 is `warmup_p99_ms`)."""
     ls = np.ascontiguousarray(loaded_sorted, dtype=np.uint64)
     w = _Workload(requests=int(requests), offered_mops=float(offered_mops),
                   ops_lo_log2=int(ops_log2[0]), ops_hi_log2=int(ops_log2[1]),
                   lookup_frac=float(lookup_frac), insert_frac=float(insert_frac),
-                  scan_lo=int(scan_len[0]), scan_hi=int(scan_len[1]), seed=int(seed))
+                  scan_lo=int(scan_len[0]), scan_hi=int(scan_len[1]), seed=int(seed),
+                  warmup_requests=int(warmup))
     c = _config(hbm_budget, slots, max_batch_ops, scan_fraction)
     rep = _Report()
     lat = np.zeros(int(requests), dtype=np.float64)

@@ -242,7 +242,9 @@ typedef struct stm_server_config {
                               at most 8 GiB; never above 90 % of the free HBM */
     uint32_t slots;        /* batch slots the budget is split into: forming, running, draining
                               (2..8, default 3) */
-    uint32_t max_batch_ops;/* ops per batch at most (0 = what a slot holds) */
+    uint32_t max_batch_ops;/* ops per batch at most; 0 = a batch takes no request that would
+                              carry it past 2^18 ops, and a larger request runs alone (up to
+                              what a slot holds) */
     double scan_fraction;  /* share of a slot kept for scan pairs (default 0.5) */
     uint32_t reserved[4];
 } stm_server_config;
@@ -283,7 +285,9 @@ typedef struct stm_serve_workload {
     double lookup_frac, insert_frac;
     uint32_t scan_lo, scan_hi;
     uint64_t seed;
-    uint32_t reserved[4];
+    uint32_t warmup_requests; /* requests run first on the same arrival process and left out of
+                                 the report's figures (0 = none) */
+    uint32_t reserved[3];
 } stm_serve_workload;
 
 typedef struct stm_serve_report {
@@ -292,9 +296,11 @@ typedef struct stm_serve_report {
     double mops;            /* ops / wall_s */
     double offered_mops;    /* ops / (last arrival - first arrival) */
     double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms;  /* request latency */
-    double busy_frac;       /* share of wall_s the tree was running a batch */
+    double busy_frac;       /* share of the whole run (warm-up included) the tree ran a batch */
     double mean_batch_ops;
     uint64_t hbm_budget, slot_ops;
+    uint64_t warmup_requests;  /* requests run before the measured ones */
+    double warmup_p99_ms;      /* p99 latency of those */
 } stm_serve_report;
 
 int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_serve_workload* w,

@@ -224,3 +224,22 @@ def test_device_copy_ragged(n16):
     finally:
         src.free()
         dst.free()
+
+
+def test_device_buffer_ranges():
+    """download(n, offset) returns exactly elements [offset, offset + n) and rejects ranges
+    outside the buffer."""
+    from eirene_amd import DeviceBuffer
+    a = np.arange(1000, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    b = DeviceBuffer.from_array(a)
+    try:
+        assert np.array_equal(b.download(), a)
+        assert np.array_equal(b.download(10, offset=990), a[990:])
+        assert np.array_equal(b.download(offset=500), a[500:])
+        assert b.download(0, offset=1000).size == 0
+        with pytest.raises(ValueError):
+            b.download(11, offset=990)
+        with pytest.raises(ValueError):
+            b.download(offset=1001)
+    finally:
+        b.free()
