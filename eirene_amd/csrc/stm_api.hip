@@ -59,7 +59,8 @@ struct StreamAcc {
   unsigned long long bad;      // ops rejected for the reserved key
   unsigned long long batches;  // batches that ran
   unsigned long long skipped;  // batches skipped after an error
-  unsigned long long pad[4];
+  unsigned long long sort_stalls;  // batches whose sort gave up a look-back wait
+  unsigned long long pad[3];
 };
 
 constexpr uint32_t DESC_TABLE = 4096;  // descriptors uploaded per window of a stream
@@ -74,7 +75,7 @@ __global__ void k_advance(const BatchDesc* __restrict__ table, uint32_t* __restr
   if (threadIdx.x != 0) return;
   const uint32_t c = cursor[0];
   BatchDesc d = table[c];
-  if (acc->bad || hdr->error) d.n = 0;
+  if (acc->bad || acc->sort_stalls || hdr->error) d.n = 0;
   *live = d;
   cursor[0] = c + 1;
 }
@@ -89,6 +90,7 @@ __global__ void k_epilogue(const BatchCounters* __restrict__ cnt, const BatchDes
   }
   acc->unique += cnt->n_unique;
   acc->bad += cnt->n_bad;
+  acc->sort_stalls += cnt->sort_stall;
   acc->batches += 1;
 }
 
@@ -141,6 +143,7 @@ struct Scratch {
     }
     A_(skeys_tmp, cap); A_(skeys, cap); A_(sidx_tmp, cap); A_(sidx, cap);
     A_(hist, sort_hist_words(cap));
+    if ((e = sort_scratch_init(hist)) != hipSuccess) return e;
     uint8_t* ft = nullptr;
     A_(ft, fold_scratch_bytes(cap));
     fold_tiles = ft;
@@ -318,6 +321,7 @@ int alloc_router(stm_tree* t) {
       dalloc(&r.order, cap) || dalloc(&r.order_tmp, cap) || dalloc(&r.hist, sort_hist_words(cap)) ||
       dalloc(&r.dir_alt, cap + 1))
     return fail(STM_ERR_NOMEM, "learned-routing buffers for %llu nodes", (unsigned long long)cap);
+  if (sort_scratch_init(r.hist) != hipSuccess) return fail(STM_ERR_DEVICE, "learned-routing sort scratch");
   t->route_alloc = true;
   t->view.rank_of = r.rank_of;
   t->view.dir = r.dir;
@@ -422,6 +426,7 @@ hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
   fa.tile_scratch = s.fold_tiles;
   fa.lastins = in.scans ? s.lastins : nullptr;
   fa.ilist = s.ilist;
+  fa.sort_stall = sort_stall_word(s.hist);
   if ((e = fold_batch(fa, st)) != hipSuccess) return e;
   if (timed) phase_switch(t, STM_PHASE_MERGE, STM_PHASE_LOOKUP);
 
@@ -552,6 +557,7 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   HIP_TRY(hipStreamSynchronize(st));
   const BatchCounters bc = *s.h_counters;
   phase_collect(t);
+  if (bc.sort_stall) return fail(STM_ERR_DEVICE, "batch sort: a look-back wait gave up");
   t->st.batches += 1;
   t->st.ops += n;
   t->st.unique_ops += bc.n_unique;
@@ -649,6 +655,9 @@ int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const ui
   HIP_TRY(hipMemcpyAsync(&acc, s.acc, sizeof(acc), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   t->st.unique_ops += acc.unique;
+  if (acc.sort_stalls)
+    return fail(STM_ERR_DEVICE, "a batch sort gave up a look-back wait (%llu later batches skipped)",
+                acc.skipped);
   if (acc.bad)
     return fail(STM_ERR_INVALID, "a batch holds the reserved key 0xffffffffffffffff (%llu later batches skipped)",
                 acc.skipped);
@@ -718,6 +727,7 @@ int build_from_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, s
       fa.uop = tmp.uop;
       fa.counters = tmp.counters;
       fa.tile_scratch = tmp.fold_tiles;
+      fa.sort_stall = sort_stall_word(tmp.hist);
       r1 = fold_batch(fa, st);
       if (r1 != hipSuccess) { rc = fail(STM_ERR_DEVICE, "fold: %s", hipGetErrorString(r1)); break; }
       BatchCounters bc;
@@ -726,6 +736,7 @@ int build_from_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, s
         rc = fail(STM_ERR_DEVICE, "bulk-load counters");
         break;
       }
+      if (bc.sort_stall) { rc = fail(STM_ERR_DEVICE, "bulk-load sort: a look-back wait gave up"); break; }
       if (bc.n_bad) { rc = fail(STM_ERR_INVALID, "bulk load holds the reserved key"); break; }
       U = bc.n_unique;
     }
@@ -1169,6 +1180,7 @@ int stm_dump(stm_tree* t, uint64_t* keys, uint64_t* vals, size_t cap, size_t* n)
       dalloc(&counts, nn) || dalloc(&counter, 1) || dalloc(&offs, (size_t)nn + 1) ||
       dalloc(&hist, sort_hist_words(nn)))
     return err(STM_ERR_NOMEM, "scratch");
+  if (sort_scratch_init(hist) != hipSuccess) return err(STM_ERR_DEVICE, "sort scratch");
   if (hipMemsetAsync(counter, 0, 4, st) != hipSuccess) return err(STM_ERR_DEVICE, "memset");
   if (launch_collect_leaves(t->view, nn, lows, ids, counter, st) != hipSuccess)
     return err(STM_ERR_DEVICE, "collect");
@@ -1179,6 +1191,11 @@ int stm_dump(stm_tree* t, uint64_t* keys, uint64_t* vals, size_t cap, size_t* n)
   if (radix_sort_pairs(lows, slows_tmp, slows, order_tmp, order, n_leaves, 0, 64, hist, st) != hipSuccess ||
       launch_leaf_counts(t->view, order, ids, n_leaves, sorted_ids, counts, st) != hipSuccess)
     return err(STM_ERR_DEVICE, "leaf order");
+  uint32_t stalled = 0;
+  if (hipMemcpyAsync(&stalled, sort_stall_word(hist), 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return err(STM_ERR_DEVICE, "leaf order");
+  if (stalled) return err(STM_ERR_DEVICE, "leaf order: a sort look-back wait gave up");
   std::vector<uint32_t> hc(n_leaves);
   if (n_leaves && (hipMemcpyAsync(hc.data(), counts, 4ull * n_leaves, hipMemcpyDeviceToHost, st) != hipSuccess ||
                    hipStreamSynchronize(st) != hipSuccess))

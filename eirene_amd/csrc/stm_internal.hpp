@@ -34,6 +34,16 @@ __host__ __device__ inline uint32_t op_of(uint32_t payload) { return payload >> 
 // ------------------------------------------------------------------ radix sort (stm_sort.hip)
 // Words of scratch (u32) the sort needs for n elements.
 size_t sort_hist_words(uint64_t n);
+// Prepares freshly allocated sort scratch (synchronous): the single-sweep look-back status
+// must start with no valid tag.
+hipError_t sort_scratch_init(uint32_t* words);
+// Non-zero after a sort whose look-back gave up waiting (its output is then wrong); cleared
+// when the next sort on the same scratch starts.
+const uint32_t* sort_stall_word(const uint32_t* words);
+// The three-kernel region of the scratch (at least n/8 words for scratch sized for n), which
+// single-sweep sorts leave alone and three-kernel sorts write before they read: free for other
+// kernels between sorts.
+uint32_t* sort_spare_words(uint32_t* words);
 // Stable ascending sort of keys[begin_bit, end_bit) with the original index as payload.
 // keys_in is not modified; results land in keys_out / idx_out.
 hipError_t radix_sort_pairs(const uint64_t* keys_in, uint64_t* keys_tmp, uint64_t* keys_out,
@@ -60,7 +70,7 @@ struct BatchCounters {
   uint32_t n_bad;       // ops rejected (reserved key)
   uint32_t scan_bad;    // scan ops rejected (length or output area)
   uint64_t scan_total;  // sum of the scan lengths (pairs the output area must hold)
-  uint32_t pad1;
+  uint32_t sort_stall;  // the batch's sort gave up a look-back wait
   uint32_t pad0;
   uint64_t hdr_error;   // the tree's sticky error bits after the batch (one readback per batch)
   uint32_t pad[4];
@@ -88,6 +98,7 @@ struct FoldArgs {
                           // before it (sorted index), -1 if none; null otherwise
   uint32_t* ilist;        // optional: the groups whose merged op is INSERT (n_upserts of them,
                           // key order within each fold tile)
+  const uint32_t* sort_stall;  // optional: the sort's stall word, copied into the counters
 };
 constexpr uint8_t UOP_SKIP = 0xff;
 constexpr uint8_t OP_RANGE = 3;

@@ -499,7 +499,7 @@ hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint
     const uint32_t gt = (n_nodes + DS_TILE - 1) / DS_TILE;
     uint32_t* counts = rs.order;
     uint32_t* offs = rs.order_tmp;
-    uint32_t* tiles = rs.hist;
+    uint32_t* tiles = sort_spare_words(rs.hist);  // one word per scan tile: n_nodes / DS_TILE
     k_dir_count<<<g, 256, 0, s>>>(t, rs.hdr, rs.dir, counts);
     k_dir_scan_reduce<<<gt, DS_BLOCK, 0, s>>>(rs.hdr, counts, tiles);
     k_dir_scan_tiles<<<1, DS_BLOCK, 0, s>>>(rs.hdr, tiles);
@@ -516,6 +516,12 @@ hipError_t route_build(const TreeView& t, RouteState& rs, uint32_t n_nodes, uint
     e = radix_sort_pairs(rs.lows, rs.lows_tmp, rs.sorted_lows, rs.order_tmp, rs.order, n_nodes, 0, 64,
                          rs.hist, s);
     if (e != hipSuccess) return e;
+    // Cold builds are rare: check the sort's stall word before its order is used.
+    uint32_t stalled = 0;
+    if ((e = hipMemcpyAsync(&stalled, sort_stall_word(rs.hist), 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return e;
+    if (stalled) return hipErrorLaunchTimeOut;
     k_route_directory<<<(n_nodes + 255) / 256, 256, 0, s>>>(rs.sorted_lows, rs.order, n_nodes, rs.hdr,
                                                             rs.fences, rs.lids, rs.dir, rs.rank_of);
     k_route_parents<<<(n_nodes + 255) / 256, 256, 0, s>>>(t, n_nodes, rs.dir);

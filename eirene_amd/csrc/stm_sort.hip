@@ -5,18 +5,42 @@
 // Stability is what makes the per-key fold reproduce input order
 // (src/gpu-stm-tree-v38/bpt-gpustm-kernel-aux.cuh:16-44), so every pass is stable.
 //
-// One pass = 8 key bits, three kernels:
+// One pass = 8 key bits.  Two forms, chosen by the largest batch a launch sequence serves:
+//
+// Up to ONESWEEP_MAX_N keys, single sweep (10 kernels per sort):
+//   k_digit_hist  one read of the keys builds the histograms of every digit position
+//                 (per-workgroup partials)
+//   k_digit_scan  sums the partials and scans them into each pass's global digit offsets
+//   k_onesweep    one launch per pass: ranking, the tile's offset among earlier tiles by
+//                 decoupled look-back, staging and coalesced writes (as k_scatter below)
+// Larger, three kernels per pass (24 per sort):
 //   k_hist      tile digit histograms           -> hist[digit][tile]
 //   k_scan_hist exclusive scan per digit over tiles, digit totals
 //   k_scatter   stable in-tile ranking with 64-lane ballots (a wavefront owns a contiguous
 //               segment, so (wave, round, lane) order is input order), LDS staging into
 //               digit order (keys, then payloads, through one buffer), then coalesced runs
 //               to global memory.
-// The element count and the input keys can come from a device-side batch descriptor, and
-// the tile kernels loop over tiles with a fixed grid, so one launch sequence (and one
-// captured hipGraph) serves any batch size up to the one the scratch was sized for.  The
-// tile is SORT_BLOCK * ITEMS elements (ITEMS = 8 or 16, chosen by the autotuner).
-// No kernel waits on another workgroup, so no launch order can deadlock.
+// Small batches pay mostly fixed per-kernel costs, which the single sweep cuts by more than
+// half.  At 2^22 keys about 1024 tiles of a pass are resident at once and the look-back chain,
+// through agent-scope status loads across the XCD caches, costs more per pass (43.6 µs against
+// 26.7) than the histogram and scan kernels it replaces.
+//
+// Look-back: a tile publishes its digit counts as soon as it has ranked (an aggregate), then
+// sums the aggregates of the tiles before it until it meets an inclusive prefix, and publishes
+// its own inclusive prefix.  Each status is one 64-bit word (tag, inclusive bit, count), so a
+// single agent-scope load reads it whole; the tag carries a per-sort epoch and the pass, so
+// the status array is never cleared between sorts.
+// Progress: workgroups take tiles from per-class tickets (class = blockIdx.x & 7, tile =
+// ticket * 8 + class), so within a class tiles are taken in order by running workgroups.  The
+// smallest untaken tile then belongs to a class whose running workgroups only wait on smaller,
+// already taken tiles, whose aggregates need no waiting: every wait ends.  A wait that still
+// exceeds LOOKBACK_SPIN_LIMIT polls sets the stall word and proceeds, so a fault in this
+// reasoning reports an error instead of hanging the device.
+//
+// The element count and the input keys can come from a device-side batch descriptor, and the
+// grids are sized for the largest batch, so one launch sequence (and one captured hipGraph)
+// serves any batch size up to the one it was sized for.  The tile is SORT_BLOCK * ITEMS
+// elements (ITEMS = 8 or 16, chosen by the autotuner).
 #include <algorithm>
 
 #include "stm_common.hpp"
@@ -29,6 +53,32 @@ constexpr int SORT_WAVES = SORT_BLOCK / WAVE;  // 4
 constexpr int RADIX = 256;
 constexpr uint32_t SORT_GRID_CAP = 2048;       // workgroups of the tile kernels
 constexpr uint32_t SCAN_REG = 8;               // tiles per thread the histogram scan keeps in registers
+
+// Single sweep.
+constexpr uint64_t ONESWEEP_MAX_N = 1u << 18;  // largest batch sorted in a single sweep
+constexpr uint32_t OS_MAX_TILES = (uint32_t)(ONESWEEP_MAX_N / (SORT_BLOCK * 8));  // 2048-key tiles
+constexpr uint32_t TICKET_CLASSES = 8;         // ticket counters per pass
+constexpr int MAX_PASSES = 8;
+constexpr int HIST_ITEMS = 8;                  // keys in flight per thread in k_digit_hist
+constexpr uint32_t HIST_GRID_CAP = (uint32_t)(ONESWEEP_MAX_N / (SORT_BLOCK * HIST_ITEMS));
+constexpr uint32_t LOOKBACK_SPIN_LIMIT = 1u << 18;
+constexpr uint32_t EPOCH_LIMIT = 1u << 29;     // epoch << 3 | pass fits the 32-bit tag
+constexpr uint64_t STATUS_INCL = 1ull << 31;
+constexpr uint32_t STATUS_COUNT = (1u << 31) - 1;
+
+// Sort scratch (u32 words).  First a fixed single-sweep region: control words (ticket counters
+// 128 bytes apart), the global digit offsets of every pass, the per-workgroup partial
+// histograms and the u64 look-back status of every tile; then the three-kernel region
+// (per-tile histograms and digit totals).
+constexpr uint32_t CTL_TICKETS = 0;  // [MAX_PASSES][TICKET_CLASSES] * 32
+constexpr uint32_t CTL_EPOCH = MAX_PASSES * TICKET_CLASSES * 32;
+constexpr uint32_t CTL_WRAPPED = CTL_EPOCH + 32;
+constexpr uint32_t CTL_STALL = CTL_WRAPPED + 32;
+constexpr uint32_t GOFF_OFF = (CTL_STALL + 32 + 63) / 64 * 64;      // goff[MAX_PASSES][RADIX]
+constexpr uint32_t PART_OFF = GOFF_OFF + MAX_PASSES * RADIX;        // [HIST_GRID_CAP][MAX_PASSES][RADIX]
+constexpr uint32_t STATUS_OFF = PART_OFF + HIST_GRID_CAP * MAX_PASSES * RADIX;  // u64 [OS_MAX_TILES][RADIX]
+constexpr uint32_t ONESWEEP_WORDS = STATUS_OFF + 2 * OS_MAX_TILES * RADIX;
+static_assert(STATUS_OFF % 2 == 0 && ONESWEEP_WORDS % 64 == 0, "aligned regions");
 
 // Exclusive scan of one u32 per thread over a 256-thread block; returns the block total.
 STM_DEV uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t& total) {
@@ -81,9 +131,11 @@ template <int ITEMS>
 __global__ __launch_bounds__(SORT_BLOCK) void k_hist(const uint64_t* __restrict__ keys,
                                                      const BatchDesc* __restrict__ desc,
                                                      uint32_t n_host, int first_pass, int shift,
-                                                     uint32_t* __restrict__ hist, uint32_t stride) {
+                                                     uint32_t* __restrict__ hist, uint32_t stride,
+                                                     uint32_t* __restrict__ stall) {
   constexpr int TILE = SORT_BLOCK * ITEMS;
   __shared__ uint32_t s_h[SORT_WAVES][RADIX];
+  if (first_pass && blockIdx.x == 0 && threadIdx.x == 0) *stall = 0;  // no look-back in this form
   const SortRange rg = sort_range(desc, n_host, keys, first_pass);
   const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
   const int wave = threadIdx.x / WAVE;
@@ -301,6 +353,303 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
   }
 }
 
+// Histograms of every digit position in one read of the keys.  Workgroup w counts a
+// contiguous share of the keys into LDS and writes its partials to part[w][pass][digit].
+// Workgroup 0 also opens the sort: it advances the epoch (wrapping before the tag would
+// overflow), clears the stall word and resets the tickets of every pass.
+__global__ __launch_bounds__(SORT_BLOCK) void k_digit_hist(const uint64_t* __restrict__ keys,
+                                                           const BatchDesc* __restrict__ desc,
+                                                           uint32_t n_host, int begin_bit,
+                                                           int passes, uint32_t* __restrict__ words) {
+  __shared__ uint32_t s_h[SORT_WAVES][MAX_PASSES][RADIX];
+  const SortRange rg = sort_range(desc, n_host, keys, 1);
+  const int wave = threadIdx.x / WAVE;
+  for (int i = threadIdx.x; i < SORT_WAVES * MAX_PASSES * RADIX; i += SORT_BLOCK)
+    (&s_h[0][0][0])[i] = 0;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      const uint32_t e = words[CTL_EPOCH] + 1;
+      const bool wrap = e >= EPOCH_LIMIT;
+      words[CTL_WRAPPED] = wrap ? 1u : 0u;
+      words[CTL_EPOCH] = wrap ? 1u : e;
+      words[CTL_STALL] = 0;
+    }
+    if (threadIdx.x < MAX_PASSES * TICKET_CLASSES) words[CTL_TICKETS + threadIdx.x * 32] = 0;
+  }
+  __syncthreads();
+  constexpr uint32_t ROUND = SORT_BLOCK * HIST_ITEMS;
+  const uint32_t share = ((rg.n + gridDim.x - 1) / gridDim.x + ROUND - 1) / ROUND * ROUND;
+  const uint64_t b0 = (uint64_t)blockIdx.x * share;
+  const uint64_t b1 = min<uint64_t>(b0 + share, rg.n);
+  const uint64_t* kp = rg.keys;
+  for (uint64_t base = b0; base < b1; base += ROUND) {
+    uint64_t k[HIST_ITEMS];
+#pragma unroll
+    for (int r = 0; r < HIST_ITEMS; ++r) {
+      const uint64_t e = base + (uint64_t)r * SORT_BLOCK + threadIdx.x;
+      k[r] = gload(kp + min(e, b1 - 1));
+    }
+#pragma unroll
+    for (int r = 0; r < HIST_ITEMS; ++r) {
+      if (base + (uint64_t)r * SORT_BLOCK + threadIdx.x < b1) {
+#pragma unroll
+        for (int p = 0; p < MAX_PASSES; ++p)
+          if (p < passes) atomicAdd(&s_h[wave][p][digit_of(k[r], begin_bit + 8 * p)], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t* part = words + PART_OFF + (uint64_t)blockIdx.x * MAX_PASSES * RADIX;
+  const int d = threadIdx.x;  // SORT_BLOCK == RADIX
+  for (int p = 0; p < passes; ++p)
+    part[p * RADIX + d] = s_h[0][p][d] + s_h[1][p][d] + s_h[2][p][d] + s_h[3][p][d];
+}
+
+// Workgroup p sums the partials of pass p over the hist_blocks histogram workgroups and scans
+// them into goff[p][digit]: the sorted position of each digit's first element.  After an
+// epoch wrap the workgroups also clear the look-back status, so no stale status can carry a
+// tag of the new epochs.
+__global__ __launch_bounds__(SORT_BLOCK) void k_digit_scan(uint32_t* __restrict__ words,
+                                                           uint32_t hist_blocks) {
+  __shared__ uint32_t s_tmp[SORT_WAVES];
+  const int p = blockIdx.x;
+  const int d = threadIdx.x;
+  const uint32_t* part = words + PART_OFF + p * RADIX + d;
+  uint32_t sum = 0;
+#pragma unroll 8
+  for (uint32_t b = 0; b < hist_blocks; ++b) sum += part[(uint64_t)b * MAX_PASSES * RADIX];
+  uint32_t total;
+  words[GOFF_OFF + p * RADIX + d] = block_exclusive_scan_256(sum, s_tmp, total);
+  if (words[CTL_WRAPPED]) {
+    uint64_t* status = reinterpret_cast<uint64_t*>(words + STATUS_OFF);
+    const uint64_t n = (uint64_t)OS_MAX_TILES * RADIX;
+    for (uint64_t i = (uint64_t)blockIdx.x * SORT_BLOCK + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * SORT_BLOCK)
+      status[i] = 0;
+  }
+}
+
+// Sum of digit d's counts over the tiles before tile b: aggregates back to the first
+// inclusive prefix.  LOOKBACK_WINDOW statuses are read at once, so the walk back over tiles
+// that have only published aggregates advances a window per load latency.  A status whose tag
+// is not this pass's has not been published yet: the walk resumes there after a short sleep.
+constexpr int LOOKBACK_WINDOW = 16;
+STM_DEV uint32_t look_back(const uint64_t* status, uint32_t b, uint32_t d, uint32_t tag,
+                           uint32_t* stall) {
+  uint32_t acc = 0, spins = 0;
+  int64_t j = (int64_t)b - 1;  // next tile to account for
+  while (j >= 0) {
+    uint64_t st[LOOKBACK_WINDOW];
+#pragma unroll
+    for (int w = 0; w < LOOKBACK_WINDOW; ++w) {
+      const int64_t t = j - w;
+      st[w] = t >= 0 ? ld_relaxed(status + (uint64_t)t * RADIX + d) : 0ull;
+    }
+    int used = 0;
+    bool done = false, blocked = false;
+#pragma unroll
+    for (int w = 0; w < LOOKBACK_WINDOW; ++w) {
+      if (!done && !blocked) {
+        if (j - w < 0) {
+          done = true;  // walked past tile 0 (only reached if no inclusive was met)
+        } else if ((uint32_t)(st[w] >> 32) != tag) {
+          blocked = true;
+        } else {
+          acc += (uint32_t)st[w] & STATUS_COUNT;
+          ++used;
+          if (st[w] & STATUS_INCL) done = true;
+        }
+      }
+    }
+    if (done) break;
+    j -= used;
+    if (blocked) {
+      if (++spins >= LOOKBACK_SPIN_LIMIT) {
+        __hip_atomic_store(stall, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return acc;
+}
+
+STM_DEV uint64_t make_status(uint32_t tag, bool inclusive, uint32_t count) {
+  return ((uint64_t)tag << 32) | (inclusive ? STATUS_INCL : 0ull) | count;
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
+    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
+    const uint8_t* __restrict__ ops_in,
+    const BatchDesc* __restrict__ desc, uint32_t n_host, int first_pass, int pass, int shift,
+    uint32_t* __restrict__ words, uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out) {
+  constexpr int TILE = SORT_BLOCK * ITEMS;
+  constexpr int SEG = TILE / SORT_WAVES;  // contiguous elements owned by one wavefront
+  __shared__ uint32_t s_wcnt[SORT_WAVES][RADIX];  // running per-wave digit counts, then wave bases
+  __shared__ uint32_t s_dbase[RADIX];             // first tile position of each digit
+  __shared__ uint32_t s_gbase[RADIX];             // global position of this tile's run of each digit
+  __shared__ uint32_t s_tmp[SORT_WAVES];
+  __shared__ uint32_t s_ticket;
+  // Keys, then payloads, staged in digit order through one buffer: 8 bytes per element of
+  // LDS instead of 12 lets four 4096-element tiles share a CU.
+  __shared__ uint64_t s_stage[TILE];
+  uint32_t* s_idx = reinterpret_cast<uint32_t*>(s_stage);
+
+  const SortRange rg = sort_range(desc, n_host, keys_in, first_pass);
+  const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
+  const uint64_t* kin = rg.keys;
+  const uint8_t* ops = (first_pass && desc) ? desc->ops : ops_in;  // read on the first pass only
+  const int lane = lane_id();
+  const int wave = threadIdx.x / WAVE;
+  const uint32_t cls = blockIdx.x & (TICKET_CLASSES - 1);
+  uint32_t* ticket = words + CTL_TICKETS + (pass * TICKET_CLASSES + cls) * 32;
+  uint32_t* stall = words + CTL_STALL;
+  uint64_t* status = reinterpret_cast<uint64_t*>(words + STATUS_OFF);
+  const uint32_t tag = (words[CTL_EPOCH] << 3) | (uint32_t)pass;
+  const uint32_t goff = words[GOFF_OFF + pass * RADIX + threadIdx.x];  // digit threadIdx.x
+  const uint64_t lt = lanemask_lt();
+  for (;;) {
+    for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wcnt[0][0])[i] = 0;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t b = s_ticket * TICKET_CLASSES + cls;
+    if (b >= ntiles) break;
+    const uint64_t tile0 = (uint64_t)b * TILE;
+    const uint32_t tile_n = (uint32_t)min<uint64_t>(TILE, rg.n - tile0);
+
+    // Registers bound the tile: keys and ranks live through the ranking, payloads are loaded
+    // only once the keys are staged.
+    uint64_t k[ITEMS];
+    uint32_t rk[ITEMS];
+    // Per-tile copies of the lane's indices: opaque to the compiler, so it recomputes the
+    // per-item offsets in each tile instead of hoisting 32 of them into registers.
+    uint32_t lane0 = (uint32_t)wave * SEG + lane;  // tile-local index of item 0
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(lane0), "+v"(tid));
+    // Item r of a lane is tile element lane0 + r * WAVE.  Loads of a partial tile clamp the
+    // index into the tile instead of branching; the clamped values are never used.
+    const uint32_t last = tile_n - 1;
+    const uint64_t* kt = kin + tile0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t le = lane0 + (uint32_t)r * WAVE;
+      const uint64_t v = gload(kt + min(le, last));
+      k[r] = le < tile_n ? v : KEY_EMPTY;
+    }
+    asm volatile("" : "+v"(lane0));  // each phase recomputes its offsets
+    uint32_t dk[(ITEMS + 3) / 4];  // the items' digits, four to a register
+#pragma unroll
+    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dk[q] = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
+      const uint32_t d = digit_of(k[r], shift);
+      dk[r / 4] |= d << (8 * (r % 4));
+      // Lanes holding the same digit: intersect 8 bit-ballots.
+      uint64_t peers = ballot(valid);
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        const bool on = (d >> bit) & 1u;
+        const uint64_t bb = ballot(on);
+        peers &= on ? bb : ~bb;
+      }
+      uint32_t base = valid ? s_wcnt[wave][d] : 0u;
+      rk[r] = base + (uint32_t)__popcll(peers & lt);
+      // Materialise the rank here: left to itself the compiler keeps the 64-bit peer mask of
+      // every item alive across the barrier below, which costs occupancy.
+      asm volatile("" : "+v"(rk[r]));
+      const bool leader = valid && (lane == 63 - __clzll(peers));
+      if (leader) s_wcnt[wave][d] = base + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    uint64_t* row = status + (uint64_t)b * RADIX;
+    uint32_t count;
+    {
+      const int d = threadIdx.x;
+      uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
+      count = c0 + c1 + c2 + c3;
+      // Publish the aggregate first (tile 0's is already its inclusive prefix).
+      st_relaxed(row + d, make_status(tag, b == 0, count));
+      uint32_t tot;
+      s_dbase[d] = block_exclusive_scan_256(count, s_tmp, tot);
+      s_wcnt[0][d] = 0;
+      s_wcnt[1][d] = c0;
+      s_wcnt[2][d] = c0 + c1;
+      s_wcnt[3][d] = c0 + c1 + c2;
+    }
+    __syncthreads();
+    asm volatile("" : "+v"(lane0));
+    // rk becomes the element's position in the tile's digit order
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      if (lane0 + (uint32_t)r * WAVE < tile_n) {
+        const uint32_t d = (dk[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
+        rk[r] += s_dbase[d] + s_wcnt[wave][d];
+        s_stage[rk[r]] = k[r];
+      }
+    }
+    asm volatile("" : "+v"(lane0));
+    // Payloads: the input's, or on the first pass the position with the op code above it.
+    // Their loads are in flight during the look-back below.
+    uint32_t id[ITEMS];
+    if (idx_in) {
+      const uint32_t* it = idx_in + tile0;
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) id[r] = gload(it + min(lane0 + (uint32_t)r * WAVE, last));
+    } else if (ops) {
+      const uint8_t* ot = ops + tile0;
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) {
+        const uint32_t le = lane0 + (uint32_t)r * WAVE;
+        const uint32_t o = gload(ot + min(le, last));
+        id[r] = ((uint32_t)tile0 + le) | ((o > 3u ? 1u : o) << POS_BITS);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) id[r] = (uint32_t)tile0 + lane0 + (uint32_t)r * WAVE;
+    }
+    {
+      const int d = threadIdx.x;
+      uint32_t before = 0;
+      if (b > 0) {
+        before = look_back(status, b, (uint32_t)d, tag, stall);
+        st_relaxed(row + d, make_status(tag, true, before + count));
+      }
+      s_gbase[d] = goff + before;
+    }
+    __syncthreads();
+    // The digit of every position a thread writes, four to a register, for the payload phase.
+    uint32_t dg[(ITEMS + 3) / 4];
+#pragma unroll
+    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dg[q] = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
+      if (p < tile_n) {
+        const uint64_t kk = s_stage[p];
+        const uint32_t d = digit_of(kk, shift);
+        dg[r / 4] |= d << (8 * (r % 4));
+        keys_out[s_gbase[d] + (p - s_dbase[d])] = kk;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      if (lane0 + (uint32_t)r * WAVE < tile_n) s_idx[rk[r]] = id[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
+      if (p < tile_n) {
+        const uint32_t d = (dg[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
+        idx_out[s_gbase[d] + (p - s_dbase[d])] = s_idx[p];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_iota_copy(const uint64_t* __restrict__ keys_in, const BatchDesc* __restrict__ desc,
                             uint32_t n_host, uint64_t* __restrict__ keys_out,
                             uint32_t* __restrict__ idx_out) {
@@ -314,37 +663,75 @@ __global__ void k_iota_copy(const uint64_t* __restrict__ keys_in, const BatchDes
 static uint32_t sort_tile(int items) { return (uint32_t)(SORT_BLOCK * items); }
 
 size_t sort_hist_words(uint64_t n) {
-  // sized for the smallest tile (the most tiles)
+  // the three-kernel region, sized for the smallest tile (the most tiles)
   uint64_t ntiles = (n + sort_tile(8) - 1) / sort_tile(8);
   if (ntiles == 0) ntiles = 1;
-  return (size_t)ntiles * RADIX + RADIX;
+  return (size_t)ONESWEEP_WORDS + (size_t)ntiles * RADIX + RADIX;
 }
+
+hipError_t sort_scratch_init(uint32_t* words) {
+  return hipMemset(words, 0, (size_t)ONESWEEP_WORDS * sizeof(uint32_t));
+}
+
+const uint32_t* sort_stall_word(const uint32_t* words) { return words + CTL_STALL; }
+
+uint32_t* sort_spare_words(uint32_t* words) { return words + ONESWEEP_WORDS; }
 
 template <int ITEMS>
 static hipError_t sort_passes(const uint64_t* keys_in, const uint8_t* ops_in, const BatchDesc* desc,
                               uint32_t n_host,
                               uint32_t n_max, uint64_t* keys_tmp, uint64_t* keys_out,
                               uint32_t* idx_tmp, uint32_t* idx_out, int begin_bit, int end_bit,
-                              uint32_t* hist_words, hipStream_t stream) {
+                              uint32_t* words, hipStream_t stream) {
   const uint32_t tile = sort_tile(ITEMS);
   const uint32_t stride = (n_max + tile - 1) / tile;  // tiles of the largest batch
-  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(stride, SORT_GRID_CAP));
-  uint32_t* hist = hist_words;
-  uint32_t* totals = hist_words + (size_t)stride * RADIX;
   const int passes = (end_bit - begin_bit + 7) / 8;
   if (passes <= 0) {
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(stride, SORT_GRID_CAP));
     k_iota_copy<<<grid, 256, 0, stream>>>(keys_in, desc, n_host, keys_out, idx_out);
     return hipGetLastError();
   }
+  if (passes > MAX_PASSES) return hipErrorInvalidValue;
   const uint64_t* kin = keys_in;
   const uint32_t* iin = nullptr;
+  if (n_max <= ONESWEEP_MAX_N) {
+    // Every ticket class needs workgroups: the grid is a multiple of TICKET_CLASSES.
+    const uint32_t grid = (std::max<uint32_t>(stride, 1) + TICKET_CLASSES - 1) / TICKET_CLASSES *
+                          TICKET_CLASSES;
+    const uint32_t hist_grid = std::max<uint32_t>(
+        1, (n_max + SORT_BLOCK * HIST_ITEMS - 1) / (SORT_BLOCK * HIST_ITEMS));
+    k_digit_hist<<<hist_grid, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, begin_bit, passes,
+                                                       words);
+    k_digit_scan<<<passes, SORT_BLOCK, 0, stream>>>(words, hist_grid);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int p = 0; p < passes; ++p) {
+      const int shift = begin_bit + 8 * p;
+      const bool to_out = ((passes - 1 - p) % 2) == 0;
+      uint64_t* kout = to_out ? keys_out : keys_tmp;
+      uint32_t* iout = to_out ? idx_out : idx_tmp;
+      const int first = p == 0 ? 1 : 0;
+      k_onesweep<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, first ? ops_in : nullptr, desc,
+                                                         n_host, first, p, shift, words, kout, iout);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      kin = kout;
+      iin = iout;
+    }
+    return hipSuccess;
+  }
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(stride, SORT_GRID_CAP));
+  uint32_t* hist = words + ONESWEEP_WORDS;
+  uint32_t* totals = hist + (size_t)stride * RADIX;
+  uint32_t* stall = words + CTL_STALL;
   for (int p = 0; p < passes; ++p) {
     const int shift = begin_bit + 8 * p;
     const bool to_out = ((passes - 1 - p) % 2) == 0;
     uint64_t* kout = to_out ? keys_out : keys_tmp;
     uint32_t* iout = to_out ? idx_out : idx_tmp;
     const int first = p == 0 ? 1 : 0;
-    k_hist<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, desc, n_host, first, shift, hist, stride);
+    k_hist<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, desc, n_host, first, shift, hist, stride,
+                                                   stall);
     k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
     k_scatter<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, first ? ops_in : nullptr, desc,
                                                       n_host, first, shift, hist, stride,
