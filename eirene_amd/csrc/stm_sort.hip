@@ -55,7 +55,10 @@ constexpr uint32_t SORT_GRID_CAP = 2048;       // workgroups of the tile kernels
 constexpr uint32_t SCAN_REG = 8;               // tiles per thread the histogram scan keeps in registers
 
 // Single sweep.
-constexpr uint64_t ONESWEEP_MAX_N = 1u << 18;  // largest batch sorted in a single sweep
+// Largest batch sorted in a single sweep.  Batch of 2^k ops (2^25 keys, 90/10): at k = 17 and 18
+// the single sweep took 0.276 and 0.313 ms against 0.313 and 0.327; at 19 and 20 0.412 and
+// 0.563 against 0.394 and 0.490.
+constexpr uint64_t ONESWEEP_MAX_N = 1u << 18;
 constexpr uint32_t OS_MAX_TILES = (uint32_t)(ONESWEEP_MAX_N / (SORT_BLOCK * 8));  // 2048-key tiles
 constexpr uint32_t TICKET_CLASSES = 8;         // ticket counters per pass
 constexpr int MAX_PASSES = 8;
