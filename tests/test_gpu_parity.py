@@ -243,3 +243,21 @@ def test_device_buffer_ranges():
             b.download(offset=1001)
     finally:
         b.free()
+
+
+@pytest.mark.parametrize("n", [1 << 18, (1 << 18) + 1])
+def test_sort_forms_at_their_boundary(oracle_mod, n):
+    """Batches of at most 2^18 ops sort in a single sweep, larger ones in three kernels per pass:
+    both sides of the boundary, with heavy key duplication (stability decides the answers)."""
+    rng = np.random.default_rng(n)
+    load = rng.integers(0, 1 << 16, size=30000).astype(np.uint64)
+    lv = rng.integers(0, 1 << 62, size=load.size).astype(np.uint64)
+    keys = rng.integers(0, 1 << 16, size=n).astype(np.uint64)
+    vals = rng.integers(0, 1 << 62, size=n).astype(np.uint64)
+    ops = (rng.random(n) < 0.3).astype(np.uint8) ^ 1  # 30 % inserts
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, lv)
+    with tree(max_keys=1 << 20, max_batch=(1 << 18) + 1) as t:
+        t.bulk_load(load, lv)
+        assert_same(t.run_batch(keys, vals, ops), o.run_batch(keys, vals, ops), f"n={n}")
+        assert_dump(t, o)
