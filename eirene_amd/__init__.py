@@ -114,6 +114,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         # workload generators (host only)
         "stm_splitmix64": (C.c_uint64, [C.c_uint64]),
         "stm_gen_values": (None, [_P, C.c_size_t, _P]),
+        "stm_gen_batch_values": (None, [_P, C.c_size_t, C.c_uint64, _P]),
         "stm_gen_keys": (C.c_int, [C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double, _P]),
         "stm_gen_mixed_batch": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.c_uint64, C.c_double,
                                           C.c_double, C.c_int, C.c_double, C.c_double, _P, _P, _P]),

@@ -114,9 +114,17 @@ struct TreeView {
 // completion: wave_order() waits until every earlier load has returned and every earlier
 // store has been acknowledged at device scope before any later access issues.  The
 // agent-scope fences of the LLVM model would add an L2 write-back (buffer_wbl2 sc1) and
-// invalidate (buffer_inv sc1), which only matter for non-atomic accesses; none exist on
-// this path.  The read-only lookup kernel uses plain loads and runs only when no writer
-// is active; kernel boundaries order it against the transactional kernels.
+// invalidate (buffer_inv sc1), which only matter for non-atomic accesses.  The read-only
+// lookup kernel uses plain loads and runs only when no writer is active; kernel boundaries
+// order it against the transactional kernels.
+// One exception inside the transactional kernels: the lane-group fast path of k_upsert_fast
+// (fast_leaf_upserts in stm_tree.hip) reads and writes a leaf's fences, keys and values with
+// plain loads and stores.  That is safe only because a group uses the content solely when its
+// test-and-set of the leaf's lock word returns exactly the version the lookup kernel recorded:
+// then no group of the same launch has committed to that leaf, so the plain loads return what
+// earlier kernels left, and no group of the launch reads what another group of it wrote (the
+// launch boundary publishes the plain stores).  A change that lets a group of that kernel re-read
+// a leaf it did not lock with the recorded version must use the agent-scope accesses instead.
 // Plain load through a pointer known to address device global memory.  Pointers read from
 // memory (a batch descriptor's) are generic to the compiler, which then emits flat loads; this
 // lets it use global loads with a scalar base and immediate offsets.

@@ -190,12 +190,16 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Acc* tile_agg, F
     run = acc_combine(run, t);
   }
   if (threadIdx.x == 0) {
-    counters->n_unique = total.c;
+    // A sort that gave up a look-back wait may have mis-ordered the batch: the batch then has
+    // no groups, so the fold, lookup, scan and upsert kernels do nothing and the tree keeps
+    // its state from before the batch.
+    const uint32_t stalled = a.sort_stall ? *a.sort_stall : 0u;
+    counters->n_unique = stalled ? 0u : total.c;
     counters->n_upserts = 0;
     counters->n_deferred = 0;
     counters->queue_head = 0;
     counters->n_bad = 0;
-    counters->sort_stall = a.sort_stall ? *a.sort_stall : 0u;
+    counters->sort_stall = stalled;
   }
 }
 
@@ -204,6 +208,7 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Acc
   __shared__ uint32_t s_ins[FOLD_TILE];  // the tile's merged INSERT groups
   __shared__ uint32_t s_nins, s_base;
   fold_resolve(a);
+  if (a.counters->sort_stall) return;  // the batch runs empty (k_fold_scan_tiles)
   const uint32_t ntiles = (a.n + FOLD_TILE - 1) / FOLD_TILE;
   const uint32_t wave = threadIdx.x / WAVE;
   const uint64_t lt = lanemask_lt();

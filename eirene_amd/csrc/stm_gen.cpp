@@ -2,7 +2,10 @@
 //
 // All randomness is counter based: draw i of stream s under seed S is
 // splitmix64(S ^ (s * GOLDEN) + i * GOLDEN2), so results do not depend on the number of
-// threads.  Values are always splitmix64(key).  Distributions:
+// threads.  Bulk-load values are splitmix64(key); the values a batch inserts are
+// splitmix64(key ^ salt(seed)), so an insert of a key that is already stored carries a value
+// different from the stored one and an answer that returns the new value instead of the old
+// one is visible.  Distributions:
 //   uniform  keys in [0, 2^63)
 //   normal   N(p0, p1) by Box-Muller, clamped to [0, 2^63)
 //   gamma    Gamma(k = p0 (integer), theta = p1) as a sum of k exponentials, clamped
@@ -138,6 +141,16 @@ extern "C" {
 
 uint64_t stm_splitmix64(uint64_t x) { return splitmix64(x); }
 
+// Insert values of a generated batch with seed `seed`: splitmix64(key ^ salt), salt != 0.
+uint64_t stm_batch_value_salt(uint64_t seed) { return draw(seed, 13, 0) | 1; }
+
+void stm_gen_batch_values(const uint64_t* keys, size_t n, uint64_t seed, uint64_t* out) {
+  const uint64_t salt = stm_batch_value_salt(seed);
+  parallel_for(n, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) out[i] = splitmix64(keys[i] ^ salt);
+  });
+}
+
 void stm_gen_values(const uint64_t* keys, size_t n, uint64_t* out) {
   parallel_for(n, [&](size_t b, size_t e) {
     for (size_t i = b; i < e; ++i) out[i] = splitmix64(keys[i]);
@@ -179,7 +192,7 @@ int stm_gen_keys(int dist, size_t n, uint64_t seed, double p0, double p1, uint64
 //   lookups  = floor(n_ops * lookup_frac); hits = round(lookups * hit_frac) sampled uniformly
 //   (with replacement) from the loaded keys, the other lookups are fresh keys of the
 //   distribution not in the loaded set; the rest of the ops are inserts of fresh keys.
-// Op order is a seeded Fisher-Yates shuffle.  vals = splitmix64(key).
+// Op order is a seeded Fisher-Yates shuffle.  vals = splitmix64(key ^ salt(seed)).
 int stm_gen_mixed_batch(const uint64_t* loaded_sorted, size_t n_loaded, size_t n_ops, uint64_t seed,
                         double lookup_frac, double hit_frac, int dist, double p0, double p1,
                         uint64_t* keys, uint64_t* vals, uint8_t* ops) {
@@ -207,7 +220,7 @@ int stm_gen_mixed_batch(const uint64_t* loaded_sorted, size_t n_loaded, size_t n
     std::swap(keys[i - 1], keys[j]);
     std::swap(ops[i - 1], ops[j]);
   }
-  stm_gen_values(keys, n_ops, vals);
+  stm_gen_batch_values(keys, n_ops, seed, vals);
   return 0;
 }
 
@@ -263,7 +276,7 @@ int stm_gen_zipf_batch(const uint64_t* loaded_sorted, size_t n_loaded, size_t n_
       ops[i] = look ? 1 : 0;
     }
   });
-  stm_gen_values(keys, n_ops, vals);
+  stm_gen_batch_values(keys, n_ops, seed, vals);
   return 0;
 }
 
@@ -277,6 +290,7 @@ int stm_gen_serving_request(const uint64_t* loaded_sorted, size_t n_loaded, uint
                             uint8_t* ops) {
   if (n_loaded == 0 || scan_lo == 0 || scan_hi < scan_lo) return 1;
   const uint64_t rs = splitmix64(seed ^ (req * GOLDEN));
+  const uint64_t salt = stm_batch_value_salt(rs);
   for (size_t i = 0; i < n_ops; ++i) {
     const double u = unit(draw(rs, 40, i));
     const uint64_t r = draw(rs, 41, i);
@@ -286,7 +300,7 @@ int stm_gen_serving_request(const uint64_t* loaded_sorted, size_t n_loaded, uint
       ops[i] = 1;
     } else if (u <= lookup_frac + insert_frac) {
       keys[i] = r & (KEY_LIMIT - 1);
-      vals[i] = splitmix64(keys[i]);
+      vals[i] = splitmix64(keys[i] ^ salt);
       ops[i] = 0;
     } else {
       keys[i] = r & (KEY_LIMIT - 1);

@@ -633,9 +633,22 @@ int stm_gen_serving_request(const uint64_t* loaded_sorted, size_t n_loaded, uint
 void stm_gen_batch_sizes(size_t count, uint64_t seed, double lo_exp, double hi_exp, uint64_t* out);
 uint64_t stm_splitmix64(uint64_t x);
 
-int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_serve_workload* w,
-                        const uint64_t* loaded_sorted, size_t n_loaded, stm_serve_report* out,
-                        double* latencies_ms) {
+}  // extern "C"
+
+namespace {
+
+// What a recorded open-loop run keeps: the generated requests, their answers and their
+// admission tickets.  The C view points into the vectors.
+struct RecordHolder {
+  stm_serve_record c{};
+  std::vector<uint64_t> off, keys, vals, ovals, poff, skeys, svals, tickets;
+  std::vector<uint8_t> ops, ofound;
+};
+
+int open_loop_impl(stm_tree* tree, const stm_server_config* cfg, const stm_serve_workload* w,
+                   const uint64_t* loaded_sorted, size_t n_loaded, stm_serve_report* out,
+                   double* latencies_ms, stm_serve_record** record) {
+  if (record) *record = nullptr;
   if (!tree || !w || !out || !loaded_sorted || !n_loaded) return sfail(STM_ERR_INVALID, "null argument");
   // The first W requests warm the loop up (device clocks, host threads) on the same arrival
   // process; the latency figures cover the R requests after them.
@@ -769,7 +782,60 @@ int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_
   out->mean_batch_ops = st.batches ? (double)st.ops / st.batches : 0;
   out->hbm_budget = st.hbm_budget;
   out->slot_ops = st.slot_ops;
-  return failed ? sfail(STM_ERR_DEVICE, "%llu requests failed", (unsigned long long)failed) : STM_OK;
+  if (failed) return sfail(STM_ERR_DEVICE, "%llu requests failed", (unsigned long long)failed);
+  if (record) {
+    auto h = std::make_unique<RecordHolder>();
+    h->off = std::move(off);
+    h->keys = std::move(keys);
+    h->vals = std::move(vals);
+    h->ops = std::move(ops);
+    h->ovals = std::move(ovals);
+    h->ofound = std::move(ofound);
+    h->poff = std::move(poff);
+    h->skeys = std::move(skeys);
+    h->svals = std::move(svals);
+    h->tickets = std::move(tickets);
+    stm_serve_record& c = h->c;
+    c.requests = RT;
+    c.ops = N;
+    c.pairs = h->poff[RT];
+    c.req_offsets = h->off.data();
+    c.keys = h->keys.data();
+    c.vals = h->vals.data();
+    c.ops_codes = h->ops.data();
+    c.out_vals = h->ovals.data();
+    c.out_found = h->ofound.data();
+    c.pair_offsets = h->poff.data();
+    c.scan_keys = h->skeys.data();
+    c.scan_vals = h->svals.data();
+    c.tickets = h->tickets.data();
+    c.opaque = h.get();
+    *record = &h.release()->c;
+  }
+  return STM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_serve_workload* w,
+                        const uint64_t* loaded_sorted, size_t n_loaded, stm_serve_report* out,
+                        double* latencies_ms) {
+  return open_loop_impl(tree, cfg, w, loaded_sorted, n_loaded, out, latencies_ms, nullptr);
+}
+
+int stm_serve_open_loop_record(stm_tree* tree, const stm_server_config* cfg,
+                               const stm_serve_workload* w, const uint64_t* loaded_sorted,
+                               size_t n_loaded, stm_serve_report* out, double* latencies_ms,
+                               stm_serve_record** record) {
+  if (!record) return sfail(STM_ERR_INVALID, "null record");
+  return open_loop_impl(tree, cfg, w, loaded_sorted, n_loaded, out, latencies_ms, record);
+}
+
+int stm_serve_record_free(stm_serve_record* record) {
+  if (record) delete static_cast<RecordHolder*>(record->opaque);
+  return STM_OK;
 }
 
 }  // extern "C"

@@ -49,6 +49,16 @@ class _Report(C.Structure):
         ("warmup_p99_ms", C.c_double)]
 
 
+class _Record(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("ops", C.c_uint64), ("pairs", C.c_uint64),
+                ("req_offsets", C.POINTER(C.c_uint64)), ("keys", C.POINTER(C.c_uint64)),
+                ("vals", C.POINTER(C.c_uint64)), ("ops_codes", C.POINTER(C.c_uint8)),
+                ("out_vals", C.POINTER(C.c_uint64)), ("out_found", C.POINTER(C.c_uint8)),
+                ("pair_offsets", C.POINTER(C.c_uint64)), ("scan_keys", C.POINTER(C.c_uint64)),
+                ("scan_vals", C.POINTER(C.c_uint64)), ("tickets", C.POINTER(C.c_uint64)),
+                ("opaque", C.c_void_p)]
+
+
 _P = C.c_void_p
 _bound = False
 
@@ -70,6 +80,10 @@ def _lib():
             "stm_server_last_error": (C.c_char_p, []),
             "stm_serve_open_loop": (C.c_int, [_P, C.POINTER(_SrvConfig), C.POINTER(_Workload), _P,
                                               C.c_size_t, C.POINTER(_Report), _P]),
+            "stm_serve_open_loop_record": (C.c_int, [_P, C.POINTER(_SrvConfig), C.POINTER(_Workload),
+                                                     _P, C.c_size_t, C.POINTER(_Report), _P,
+                                                     C.POINTER(C.POINTER(_Record))]),
+            "stm_serve_record_free": (C.c_int, [C.POINTER(_Record)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -168,10 +182,13 @@ def open_loop(tree: StmTree, loaded_sorted, requests: int, offered_mops: float,
               ops_log2=(0, 16), lookup_frac=0.90, insert_frac=0.08, scan_len=(16, 256),
               seed: int = 1, hbm_budget: int = 0, slots: int = 3, max_batch_ops: int = 0,
               scan_fraction: float = 0.5, return_latencies: bool = False,
-              warmup: int = 0) -> Dict:
+              warmup: int = 0, record: bool = False) -> Dict:
     """Native open-loop run of the serving loop; returns its report (latencies in ms).  The
     first `warmup` requests run on the same arrival process before the measured ones and are
-    left out of the report's figures (their p99 is `warmup_p99_ms`)."""
+    left out of the report's figures (their p99 is `warmup_p99_ms`).  With `record` the report
+    also holds "record": every request (warm-up ones first) with its answers and admission
+    ticket, as numpy arrays (keys, vals, ops, req_offsets, out_vals, out_found, pair_offsets,
+    scan_keys, scan_vals, tickets)."""
     ls = np.ascontiguousarray(loaded_sorted, dtype=np.uint64)
     w = _Workload(requests=int(requests), offered_mops=float(offered_mops),
                   ops_lo_log2=int(ops_log2[0]), ops_hi_log2=int(ops_log2[1]),
@@ -181,9 +198,33 @@ def open_loop(tree: StmTree, loaded_sorted, requests: int, offered_mops: float,
     c = _config(hbm_budget, slots, max_batch_ops, scan_fraction)
     rep = _Report()
     lat = np.zeros(int(requests), dtype=np.float64)
-    _check(_lib().stm_serve_open_loop(tree._h, C.byref(c), C.byref(w), ls.ctypes.data, ls.size,
-                                      C.byref(rep), lat.ctypes.data))
+    rec = None
+    if record:
+        rp = C.POINTER(_Record)()
+        _check(_lib().stm_serve_open_loop_record(tree._h, C.byref(c), C.byref(w), ls.ctypes.data,
+                                                 ls.size, C.byref(rep), lat.ctypes.data, C.byref(rp)))
+        r = rp.contents
+        n_req, n_ops, n_pairs = int(r.requests), int(r.ops), int(r.pairs)
+
+        def copy(ptr, n, dt):
+            return np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n].astype(dt, copy=True)
+
+        rec = {"req_offsets": copy(r.req_offsets, n_req + 1, np.uint64),
+               "keys": copy(r.keys, n_ops, np.uint64), "vals": copy(r.vals, n_ops, np.uint64),
+               "ops": copy(r.ops_codes, n_ops, np.uint8),
+               "out_vals": copy(r.out_vals, n_ops, np.uint64),
+               "out_found": copy(r.out_found, n_ops, np.uint8),
+               "pair_offsets": copy(r.pair_offsets, n_req + 1, np.uint64),
+               "scan_keys": copy(r.scan_keys, n_pairs, np.uint64),
+               "scan_vals": copy(r.scan_vals, n_pairs, np.uint64),
+               "tickets": copy(r.tickets, n_req, np.uint64)}
+        _lib().stm_serve_record_free(rp)
+    else:
+        _check(_lib().stm_serve_open_loop(tree._h, C.byref(c), C.byref(w), ls.ctypes.data,
+                                          ls.size, C.byref(rep), lat.ctypes.data))
     out = {n: getattr(rep, n) for n, _ in _Report._fields_}
+    if rec is not None:
+        out["record"] = rec
     if return_latencies:
         out["latencies_ms"] = lat
     return out

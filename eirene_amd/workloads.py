@@ -1,7 +1,9 @@
 """Seeded synthetic workloads of the five benchmark configurations (host side).
 
 Generators run in libstmtree.so's host code (counter-based splitmix64 streams, so the
-output does not depend on thread count).  Values are always splitmix64(key).
+output does not depend on thread count).  Bulk-load values are splitmix64(key); the insert
+values of a generated batch are splitmix64(key ^ salt(seed)), so they differ from the stored
+ones (an update that answered the new value instead of the old would show).
 
     uniform_keys(n, seed)                     keys uniform in [0, 2^63)
     normal_keys(n, seed, mean, sd)            N(mean, sd) clamped to [0, 2^63), unique
@@ -27,6 +29,15 @@ def values_for(keys: np.ndarray) -> np.ndarray:
     k = np.ascontiguousarray(keys, dtype=np.uint64)
     out = np.empty_like(k)
     load_library().stm_gen_values(k.ctypes.data, k.size, out.ctypes.data)
+    return out
+
+
+def batch_values(keys: np.ndarray, seed: int) -> np.ndarray:
+    """Insert values a batch generated with `seed` carries: splitmix64(key ^ salt(seed))."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    out = np.empty_like(k)
+    load_library().stm_gen_batch_values(k.ctypes.data, k.size, seed & 0xFFFFFFFFFFFFFFFF,
+                                        out.ctypes.data)
     return out
 
 

@@ -142,7 +142,10 @@ int stm_run_batch(stm_tree* tree, const uint64_t* keys, const uint64_t* vals, co
                   size_t n, uint64_t* out_vals, uint8_t* out_found);
 
 /* Same with device-resident inputs and outputs (d_vals is required).  Work is queued on
- * the tree's stream (stm_get_stream); the call returns after the batch has completed. */
+ * the tree's stream (stm_get_stream); the call returns after the batch has completed.
+ * Batches of at most 2^18 ops are sorted in a single sweep whose waits between workgroups are
+ * bounded; a sort that gives up such a wait fails the call with STM_ERR_DEVICE, and that batch
+ * then leaves the tree unchanged (its lookups, scans and inserts do not run). */
 int stm_run_batch_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
                          const uint8_t* d_ops, size_t n, uint64_t* d_out_vals, uint8_t* d_out_found);
 
@@ -306,6 +309,32 @@ typedef struct stm_serve_report {
 int stm_serve_open_loop(stm_tree* tree, const stm_server_config* cfg, const stm_serve_workload* w,
                         const uint64_t* loaded_sorted, size_t n_loaded, stm_serve_report* out,
                         double* latencies_ms);
+
+/* The same open-loop run with every request and answer kept, for checking the answers
+ * against a sequential replay in admission order.  On success *record receives the generated
+ * requests (warm-up requests included), their answers and their admission tickets; free it
+ * with stm_serve_record_free.  No reference counterpart (test and measurement harness). */
+typedef struct stm_serve_record {
+    uint64_t requests;        /* every request of the run, warm-up ones first */
+    uint64_t ops, pairs;      /* ops and scan pairs of all requests */
+    uint64_t* req_offsets;    /* [requests + 1]: request r holds ops [req_offsets[r], req_offsets[r+1]) */
+    uint64_t* keys;           /* [ops] */
+    uint64_t* vals;           /* [ops] insert values; scan lengths for op 3 */
+    uint8_t* ops_codes;       /* [ops] */
+    uint64_t* out_vals;       /* [ops] answers, as stm_run_batch_scan gives them */
+    uint8_t* out_found;       /* [ops] */
+    uint64_t* pair_offsets;   /* [requests + 1]: request r's scan pairs */
+    uint64_t* scan_keys;      /* [pairs] */
+    uint64_t* scan_vals;      /* [pairs] */
+    uint64_t* tickets;        /* [requests] admission ticket: requests ran in ticket order */
+    void* opaque;
+} stm_serve_record;
+
+int stm_serve_open_loop_record(stm_tree* tree, const stm_server_config* cfg,
+                               const stm_serve_workload* w, const uint64_t* loaded_sorted,
+                               size_t n_loaded, stm_serve_report* out, double* latencies_ms,
+                               stm_serve_record** record);
+int stm_serve_record_free(stm_serve_record* record);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* stm_last_error(void);

@@ -7,6 +7,8 @@ eirene_amd never touches it.
     Oracle          sequential replay on std::map (oracle.cpp) and the restated v38
                     pipeline (stable sort, RLE, scan, mergeMission, writeback)
     CpuExecutor     T-thread partitioned CPU B+tree (cpu_baseline.cpp), the timed baseline
+                    and the checker of the full-size GPU tests (pinned to Oracle by
+                    tests/test_oracle.py: every answer and the in-order dump)
     build()         compiles liboracle.so with g++ (no GPU needed)
 
 Parity status of each function is recorded in the header of oracle.cpp and in the tests
@@ -58,6 +60,8 @@ def _load():
             ("cpu_exec_create", P, [P, P, C.c_size_t, C.c_int]),
             ("cpu_exec_destroy", None, [P]),
             ("cpu_exec_run_batch", C.c_double, [P, P, P, P, C.c_size_t, P, P]),
+            ("cpu_exec_size", C.c_uint64, [P]),
+            ("cpu_exec_dump", None, [P, P, P]),
         ]:
             fn = getattr(lib, name)
             fn.restype = res
@@ -161,6 +165,31 @@ class CpuExecutor:
         sec = self._lib.cpu_exec_run_batch(self._h, k.ctypes.data, v.ctypes.data, o.ctypes.data,
                                            k.size, out_v.ctypes.data, out_f.ctypes.data)
         return out_v, out_f, sec
+
+    def run_batches(self, keys, vals, ops, offsets):
+        """Batches [offsets[b], offsets[b+1]) one after another; returns (values, found,
+        seconds summed over the batches)."""
+        k, v, o = _a(keys, np.uint64), _a(vals, np.uint64), _a(ops, np.uint8)
+        out_v = np.zeros(k.size, dtype=np.uint64)
+        out_f = np.zeros(k.size, dtype=np.uint8)
+        sec = 0.0
+        for b in range(len(offsets) - 1):
+            a, z = int(offsets[b]), int(offsets[b + 1])
+            if z > a:
+                sec += self._lib.cpu_exec_run_batch(self._h, k[a:].ctypes.data, v[a:].ctypes.data,
+                                                    o[a:].ctypes.data, z - a,
+                                                    out_v[a:].ctypes.data, out_f[a:].ctypes.data)
+        return out_v, out_f, sec
+
+    def size(self) -> int:
+        return int(self._lib.cpu_exec_size(self._h))
+
+    def dump(self):
+        n = self.size()
+        k = np.empty(n, dtype=np.uint64)
+        v = np.empty(n, dtype=np.uint64)
+        self._lib.cpu_exec_dump(self._h, k.ctypes.data, v.ctypes.data)
+        return k, v
 
     def close(self):
         if self._h:

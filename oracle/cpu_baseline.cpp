@@ -13,8 +13,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -70,6 +73,9 @@ class CpuBTree {
     }
     root_ = level[0];
   }
+
+  // Appends every (key, value) in key order.
+  void dump(std::vector<uint64_t>& ks, std::vector<uint64_t>& vs) const { dump_node(root_, ks, vs); }
 
   bool get(uint64_t key, uint64_t* val) const {
     const Node* x = root_;
@@ -151,6 +157,15 @@ class CpuBTree {
     p->c[s + 1] = b;
     p->n++;
   }
+  static void dump_node(const Node* x, std::vector<uint64_t>& ks, std::vector<uint64_t>& vs) {
+    if (!x) return;
+    if (x->leaf) {
+      ks.insert(ks.end(), x->k, x->k + x->n);
+      vs.insert(vs.end(), x->v, x->v + x->n);
+      return;
+    }
+    for (int i = 0; i < x->n; ++i) dump_node(x->c[i], ks, vs);
+  }
   static void free_all(Node* x) {
     if (!x) return;
     if (!x->leaf)
@@ -159,23 +174,71 @@ class CpuBTree {
   }
 };
 
+// T persistent worker threads; run(fn) calls fn(t) on every worker t and returns when all
+// have finished, so a batch pays no thread start-up.
+class Pool {
+ public:
+  explicit Pool(int T) : T_(T) {
+    for (int t = 0; t < T_; ++t) th_.emplace_back([this, t] { loop(t); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& x : th_) x.join();
+  }
+  void run(const std::function<void(int)>& fn) {
+    std::unique_lock<std::mutex> g(m_);
+    fn_ = &fn;
+    busy_ = T_;
+    ++gen_;
+    cv_.notify_all();
+    done_.wait(g, [this] { return busy_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void loop(int t) {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(int)>* fn;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+        fn = fn_;
+      }
+      (*fn)(t);
+      std::lock_guard<std::mutex> g(m_);
+      if (--busy_ == 0) done_.notify_one();
+    }
+  }
+  int T_;
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* fn_ = nullptr;
+  uint64_t gen_ = 0;
+  int busy_ = 0;
+  bool stop_ = false;
+};
+
 struct Executor {
   int T = 1;
   std::vector<uint64_t> splitters;  // T-1 ascending keys
   std::vector<std::unique_ptr<CpuBTree>> parts;
   std::vector<std::vector<uint32_t>> lists;
+  std::vector<size_t> cnt, off;
+  std::unique_ptr<Pool> pool;
 
   int part_of(uint64_t key) const {
     return (int)(std::upper_bound(splitters.begin(), splitters.end(), key) - splitters.begin());
   }
 };
-
-template <typename Fn>
-void run_threads(int T, Fn fn) {
-  std::vector<std::thread> th;
-  for (int t = 0; t < T; ++t) th.emplace_back([=] { fn(t); });
-  for (auto& x : th) x.join();
-}
 
 }  // namespace
 
@@ -189,9 +252,12 @@ void* cpu_exec_create(const uint64_t* sorted_keys, const uint64_t* vals, size_t 
   for (int t = 1; t < T; ++t) e->splitters.push_back(n ? sorted_keys[(size_t)n * t / T] : 0);
   e->parts.resize(T);
   e->lists.resize(T);
+  e->cnt.assign((size_t)T * T, 0);
+  e->off.assign((size_t)T * T, 0);
+  e->pool.reset(new Pool(T));
   std::vector<size_t> lo(T + 1);
   for (int t = 0; t <= T; ++t) lo[t] = (size_t)n * t / T;
-  run_threads(T, [&](int t) {
+  e->pool->run([&](int t) {
     e->parts[t].reset(new CpuBTree());
     e->parts[t]->bulk_load(sorted_keys + lo[t], vals + lo[t], lo[t + 1] - lo[t], NF * 3 / 4);
   });
@@ -207,33 +273,31 @@ double cpu_exec_run_batch(void* h, const uint64_t* keys, const uint64_t* vals, c
   const int T = e->T;
   const auto t0 = std::chrono::steady_clock::now();
   // Per-slice partition counts, then an order-preserving scatter of op indices.
-  std::vector<size_t> cnt((size_t)T * T, 0);
+  std::vector<size_t>& cnt = e->cnt;
+  std::vector<size_t>& off = e->off;
+  std::fill(cnt.begin(), cnt.end(), 0);
   const size_t per = (n + T - 1) / T;
-  run_threads(T, [&](int s) {
-    const size_t b = s * per, en = std::min(n, b + per);
+  e->pool->run([&](int s) {
+    const size_t b = std::min(n, s * per), en = std::min(n, b + per);
     for (size_t i = b; i < en; ++i) cnt[(size_t)s * T + e->part_of(keys[i])]++;
   });
-  std::vector<size_t> off((size_t)T * T);
-  std::vector<size_t> tot(T, 0);
   for (int p = 0; p < T; ++p) {
     size_t run = 0;
     for (int s = 0; s < T; ++s) {
       off[(size_t)s * T + p] = run;
       run += cnt[(size_t)s * T + p];
     }
-    tot[p] = run;
     e->lists[p].resize(run);
   }
-  run_threads(T, [&](int s) {
-    const size_t b = s * per, en = std::min(n, b + per);
-    std::vector<size_t> o(T);
-    for (int p = 0; p < T; ++p) o[p] = off[(size_t)s * T + p];
+  e->pool->run([&](int s) {
+    const size_t b = std::min(n, s * per), en = std::min(n, b + per);
+    size_t* o = &off[(size_t)s * T];
     for (size_t i = b; i < en; ++i) {
       const int p = e->part_of(keys[i]);
       e->lists[p][o[p]++] = (uint32_t)i;
     }
   });
-  run_threads(T, [&](int p) {
+  e->pool->run([&](int p) {
     CpuBTree& tree = *e->parts[p];
     for (uint32_t i : e->lists[p]) {
       uint64_t v = 0;
@@ -246,6 +310,35 @@ double cpu_exec_run_batch(void* h, const uint64_t* keys, const uint64_t* vals, c
   });
   const auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// Number of keys held (the sum over the partitions).
+uint64_t cpu_exec_size(void* h) {
+  Executor* e = static_cast<Executor*>(h);
+  std::vector<uint64_t> counts(e->T, 0);
+  e->pool->run([&](int p) {
+    std::vector<uint64_t> ks, vs;
+    e->parts[p]->dump(ks, vs);
+    counts[p] = ks.size();
+  });
+  uint64_t total = 0;
+  for (uint64_t c : counts) total += c;
+  return total;
+}
+
+// In-order dump: the partitions cover ascending key ranges, so their dumps concatenate.
+// keys / vals must hold cpu_exec_size() entries.
+void cpu_exec_dump(void* h, uint64_t* keys, uint64_t* vals) {
+  Executor* e = static_cast<Executor*>(h);
+  const int T = e->T;
+  std::vector<std::vector<uint64_t>> ks(T), vs(T);
+  e->pool->run([&](int p) { e->parts[p]->dump(ks[p], vs[p]); });
+  size_t at = 0;
+  for (int p = 0; p < T; ++p) {
+    std::memcpy(keys + at, ks[p].data(), ks[p].size() * 8);
+    std::memcpy(vals + at, vs[p].data(), vs[p].size() * 8);
+    at += ks[p].size();
+  }
 }
 
 }  // extern "C"

@@ -50,7 +50,12 @@ uint64_t oracle_size(void* h) { return static_cast<Oracle*>(h)->m.size(); }
 // Sequential inserts; the last value of a repeated key wins.
 void oracle_bulk_load(void* h, const uint64_t* keys, const uint64_t* vals, size_t n) {
   auto& m = static_cast<Oracle*>(h)->m;
-  for (size_t i = 0; i < n; ++i) m[keys[i]] = vals[i];
+  for (size_t i = 0; i < n; ++i) {
+    // A key above every stored key goes to the end in constant time (ascending loads);
+    // otherwise a plain assignment (a later duplicate overwrites, as a sequential insert).
+    if (m.empty() || m.rbegin()->first < keys[i]) m.emplace_hint(m.end(), keys[i], vals[i]);
+    else m[keys[i]] = vals[i];
+  }
 }
 
 void oracle_run_batch(void* h, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,

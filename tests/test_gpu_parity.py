@@ -148,6 +148,40 @@ def test_contended_zipf(oracle_mod):
             assert t.check() == 0
         st = t.stats()
         assert st["upserts"] > 0 and st["commits"] >= st["upserts"]
+        assert st["aborts"] > 0  # Zipf-hot leaves make transactions collide
+        assert_dump(t, o)
+
+
+@pytest.mark.parametrize("fanout", [32, 64])
+def test_forced_aborts_run_the_serial_pass(oracle_mod, fanout):
+    """Thousands of distinct new keys packed into the range of a few leaves, one upsert per
+    wavefront (upsert_ops_per_wave = 1) and a single attempt per transaction: nearly every
+    handed-back insert collides on a leaf lock or a parent split, gives up after its first
+    abort and runs in the serial pass (k_upsert_serial, the only forward-progress guarantee).
+    Every answer, the dump and the tree's invariants must still equal the sequential replay
+    (reference retry loop: src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:107-237)."""
+    rng = np.random.default_rng(fanout + 7)
+    load = (np.arange(4096, dtype=np.uint64) << np.uint64(40)) + np.uint64(12345)
+    lv = rng.integers(0, 1 << 63, size=load.size, dtype=np.uint64)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, lv)
+    with tree(fanout=fanout, max_keys=1 << 16, max_batch=1 << 14, max_attempts=1,
+              upsert_ops_per_wave=1) as t:
+        t.bulk_load(load, lv)
+        for b in range(3):
+            n = 6000 if b < 2 else 3000
+            lo, hi = int(load[100 + 40 * b]), int(load[103 + 40 * b])
+            fresh = rng.integers(lo, hi, size=n, dtype=np.uint64)
+            keys = np.concatenate([fresh, rng.choice(fresh, n // 4), rng.choice(load, n // 4)])
+            rng.shuffle(keys)
+            vals = rng.integers(0, 1 << 63, size=keys.size, dtype=np.uint64)
+            ops = (rng.random(keys.size) < 0.2).astype(np.uint8)  # 80 % inserts
+            assert_same(t.run_batch(keys, vals, ops), o.run_batch(keys, vals, ops), f"batch {b}")
+            assert t.check() == 0
+        st = t.stats()
+        assert st["aborts"] > 0, st
+        assert st["deferred"] > 0, st
+        assert st["leaf_splits"] > 0
         assert_dump(t, o)
 
 

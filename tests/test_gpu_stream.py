@@ -104,6 +104,38 @@ def test_stream_matches_oracle(oracle_mod, fanout, tile, graphs, lanes, group, p
         assert st["upserts"] > 0
 
 
+def test_stream_large_size_classes(oracle_mod):
+    """max_batch = 2^20: graph size classes 2^16..2^20, batches on both sides of the 2^18-key
+    single-sweep boundary (so the three-kernel sort reads its length from the device-side
+    descriptor inside a replayed graph), a batch longer than max_batch (chunked) and small
+    batches between them; every op and the dump against the oracle."""
+    rng = np.random.default_rng(41)
+    load = rng.integers(0, 1 << 63, size=200000, dtype=np.uint64)
+    lv = rng.integers(0, 1 << 63, size=load.size, dtype=np.uint64)
+    sizes = np.array([(1 << 18) - 3, (1 << 18) + 1, 3000, (1 << 20), (1 << 19) + 77, 70000,
+                      (1 << 18), 1, (1 << 20) + 4097, (1 << 16) + 1], dtype=np.int64)
+    n = int(sizes.sum())
+    hit = rng.random(n) < 0.6
+    keys = np.where(hit, rng.choice(load, n), rng.integers(0, 1 << 63, size=n, dtype=np.uint64))
+    keys = keys.astype(np.uint64)
+    rep = rng.random(n) < 0.1
+    keys[1:][rep[1:]] = keys[:-1][rep[1:]]
+    vals = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+    ops = np.where(rng.random(n) < 0.2, 0, 1).astype(np.uint8)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    o, want_v, want_f = oracle_replay(oracle_mod, load, lv, keys, vals, ops, offsets)
+    with tree(max_keys=1 << 22, max_batch=1 << 20) as t:
+        t.bulk_load(load, lv)
+        t.capture_graphs()
+        got_v, got_f = run_stream(t, keys, vals, ops, offsets)
+        check_same(got_v, got_f, want_v, want_f)
+        k, v = t.dump()
+        ek, ev = o.dump()
+        np.testing.assert_array_equal(k, ek)
+        np.testing.assert_array_equal(v, ev)
+        assert t.check() == 0
+
+
 def test_stream_equals_single_batches(oracle_mod):
     """The graph replay and per-batch host launches leave identical trees and outputs."""
     rng = np.random.default_rng(3)

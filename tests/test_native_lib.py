@@ -94,7 +94,9 @@ def test_mixed_batch_shape():
     hit = np.isin(keys, load)
     assert hit[ops == 0].sum() == 0  # inserts are fresh keys
     assert abs(hit[ops == 1].mean() - 0.9) < 0.001
-    np.testing.assert_array_equal(vals, w.values_for(keys))
+    np.testing.assert_array_equal(vals, w.batch_values(keys, 9))
+    # insert values differ from the stored values splitmix64(key)
+    assert not np.any(vals == w.values_for(keys))
     k2, _, o2 = w.mixed_batch(load, 1 << 16, 9, 0.95, 0.9)
     np.testing.assert_array_equal(keys, k2)
     np.testing.assert_array_equal(ops, o2)

@@ -131,6 +131,12 @@ def test_cpu_executor_matches_oracle(oracle_mod, threads):
         np.testing.assert_array_equal(f, ef)
         np.testing.assert_array_equal(v, ev)
         assert sec > 0
+    # the in-order dump of the partitions equals the sequential replay's
+    k, v = ex.dump()
+    ek, ev = o.dump()
+    np.testing.assert_array_equal(k, ek)
+    np.testing.assert_array_equal(v, ev)
+    ex.close()
 
 
 def py_replay_scan(state, keys, vals, ops):
