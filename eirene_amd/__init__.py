@@ -35,6 +35,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libstmtree.so"
 OP_INSERT, OP_GET, OP_DELETE, OP_RANGE = 0, 1, 2, 3
 KEY_RESERVED = 0xFFFFFFFFFFFFFFFF
 ROUTE_DESCENT, ROUTE_LEARNED, ROUTE_INDEX = 0, 1, 2
+NOALLOC_MAX_OPS = 65536  # largest batch of stm_run_batch_device_noalloc
 PHASES = ("sort", "merge", "lookup", "upsert", "deferred", "route", "train", "scan")
 
 STATUS = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "STATE"}
@@ -89,6 +90,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_bulk_load_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
         "stm_run_batch": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
         "stm_run_batch_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
+        "stm_run_batch_device_noalloc": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P]),
         "stm_run_batches_device": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, _P, _P]),
         "stm_run_batch_scan": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P, _P, _P, C.c_size_t]),
         "stm_run_batch_scan_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P, _P, _P,
@@ -326,6 +328,17 @@ class StmTree:
         """Device-resident batch of n ops starting at element `offset` of each buffer."""
         n = keys.n - offset if n is None else int(n)
         _check(self._lib.stm_run_batch_device(
+            self._h, keys.ptr + 8 * offset, vals.ptr + 8 * offset, ops.ptr + offset, n,
+            out_vals.ptr + 8 * offset, out_found.ptr + offset))
+
+    def run_batch_device_noalloc(self, keys: "DeviceBuffer", vals: "DeviceBuffer",
+                                 ops: "DeviceBuffer", out_vals: "DeviceBuffer",
+                                 out_found: "DeviceBuffer", n: Optional[int] = None,
+                                 offset: int = 0) -> None:
+        """run_batch_device through the entry point that allocates nothing (its scratch is a
+        fixed device array per GPU); n <= NOALLOC_MAX_OPS."""
+        n = keys.n - offset if n is None else int(n)
+        _check(self._lib.stm_run_batch_device_noalloc(
             self._h, keys.ptr + 8 * offset, vals.ptr + 8 * offset, ops.ptr + offset, n,
             out_vals.ptr + 8 * offset, out_found.ptr + offset))
 

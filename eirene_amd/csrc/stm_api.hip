@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -134,16 +135,19 @@ struct Scratch {
   uint32_t* cursor = nullptr;       // next desc_table entry
   StreamAcc* acc = nullptr;
 
-  hipError_t alloc(uint64_t cap, bool staging) {
+  // Every device array of the scratch for batches of up to cap ops, in one fixed order:
+  // take(&pointer, bytes) provides each one (allocation, or a place in a fixed region).
+  template <typename Take>
+  hipError_t layout(uint64_t cap, bool staging, Take&& take) {
     n = cap;
     hipError_t e;
-#define A_(p, c) if ((e = dalloc(&p, c)) != hipSuccess) return e
+#define A_(p, c) \
+    if ((e = take((void**)&p, sizeof(*p) * std::max<uint64_t>((uint64_t)(c), 1))) != hipSuccess) return e
     if (staging) {
       A_(in_keys, cap); A_(in_vals, cap); A_(in_ops, cap); A_(out_vals, cap); A_(out_found, cap);
     }
     A_(skeys_tmp, cap); A_(skeys, cap); A_(sidx_tmp, cap); A_(sidx, cap);
     A_(hist, sort_hist_words(cap));
-    if ((e = sort_scratch_init(hist)) != hipSuccess) return e;
     uint8_t* ft = nullptr;
     A_(ft, fold_scratch_bytes(cap));
     fold_tiles = ft;
@@ -156,14 +160,32 @@ struct Scratch {
     A_(upar, cap);
     A_(uver, cap);
     A_(counters, 1);
-    if ((e = hipHostMalloc((void**)&h_counters, sizeof(BatchCounters), hipHostMallocCoherent)) != hipSuccess)
-      return e;
     if (staging) {
       A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
       A_(soff, cap); A_(lastins, cap); A_(scan_tiles, scan_tile_words(cap));
     }
 #undef A_
     return hipSuccess;
+  }
+
+  hipError_t alloc(uint64_t cap, bool staging) {
+    hipError_t e = layout(cap, staging, [](void** p, size_t bytes) { return hipMalloc(p, bytes); });
+    if (e != hipSuccess) return e;
+    if ((e = sort_scratch_init(hist)) != hipSuccess) return e;
+    return hipHostMalloc((void**)&h_counters, sizeof(BatchCounters), hipHostMallocCoherent);
+  }
+
+  // Lays the arrays out back to back (256-byte aligned) from base; with base null only
+  // counts.  Returns the bytes the layout spans.
+  size_t carve(uint64_t cap, uint8_t* base) {
+    size_t at = 0;
+    (void)layout(cap, false, [&](void** p, size_t bytes) {
+      at = (at + 255) / 256 * 256;
+      *p = base ? base + at : nullptr;
+      at += bytes;
+      return hipSuccess;
+    });
+    return at;
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
@@ -398,8 +420,7 @@ struct PipeIn {
   BatchCounters* host_counters;
 };
 
-hipError_t enqueue_pipeline(stm_tree* t, const PipeIn& in, bool timed) {
-  Scratch& s = t->scr;
+hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool timed) {
   hipStream_t st = t->stream;
   hipError_t e;
   if (timed) phase_begin(t, STM_PHASE_SORT);
@@ -535,9 +556,11 @@ struct ScanOut {
 };
 
 // One chunk of at most max_batch ops, all pointers on the device; waits for its completion.
+// The batch scratch is the tree's own unless `scr` points at another layout of it.
 int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
-              uint32_t n, uint64_t* out_vals, uint8_t* out_found, ScanOut* so = nullptr) {
-  Scratch& s = t->scr;
+              uint32_t n, uint64_t* out_vals, uint8_t* out_found, ScanOut* so = nullptr,
+              Scratch* scr = nullptr) {
+  Scratch& s = scr ? *scr : t->scr;
   hipStream_t st = t->stream;
   int rc = maybe_retrain(t);
   if (rc) return rc;
@@ -550,7 +573,7 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   } else {
     in.host_counters = s.h_counters;  // the scan answers kernel runs after the upsert kernels
   }
-  HIP_TRY(enqueue_pipeline(t, in, true));
+  HIP_TRY(enqueue_pipeline(t, s, in, true));
   // One readback per batch, into pinned memory: counters and the tree's error bits.
   if (so)
     HIP_TRY(hipMemcpyAsync(s.h_counters, s.counters, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
@@ -595,7 +618,7 @@ int capture_class(stm_tree* t, int c) {
   hipError_t e = hipGetLastError();
   PipeIn in{nullptr, nullptr, nullptr, class_nmax(t, c), nullptr, nullptr, s.desc_live,
             false, nullptr, nullptr, 0};
-  if (e == hipSuccess) e = enqueue_pipeline(t, in, false);
+  if (e == hipSuccess) e = enqueue_pipeline(t, s, in, false);
   if (e == hipSuccess) {
     k_epilogue<<<1, 64, 0, st>>>(s.counters, s.desc_live, s.acc);
     e = hipGetLastError();
@@ -798,6 +821,64 @@ int build_from_device(stm_tree* t, const uint64_t* keys, const uint64_t* vals, s
   return rc;
 }
 
+// ------------------------------------------------------------------ batches without allocation
+// stm_run_batch_device_noalloc keeps the scratch of its batch (sorted keys and positions, the
+// sort's control words and histograms, fold tiles, per-group arrays, insert lists, counters)
+// in this fixed device array instead of the tree's allocations.  Each GPU the library runs on
+// holds its own copy until the process ends.  Host code never passes the array's name to a
+// kernel or copy: it stands for a host-side shadow; the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and is kept per device.
+constexpr size_t NOALLOC_BYTES = 8u << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
+struct NoAllocSlot {
+  std::once_flag once;
+  uint8_t* ptr = nullptr;  // device address of `scratch` on this device
+  int rc = STM_OK;
+  char err[192] = {};
+};
+constexpr int NOALLOC_DEVICES = 64;
+NoAllocSlot g_noalloc[NOALLOC_DEVICES];
+
+// Lays the batch scratch out in the device's copy of `scratch` (set up on the first call on
+// the device: address lookup, then the sort's control words cleared so its look-back status
+// starts untagged).  Nothing is allocated.  The caller has made the tree's device current.
+int noalloc_scratch(stm_tree* t, Scratch& s) {
+  const int dev = t->cfg.device;
+  if (dev < 0 || dev >= NOALLOC_DEVICES)
+    return fail(STM_ERR_INVALID, "device %d outside the no-allocation table", dev);
+  NoAllocSlot& d = g_noalloc[dev];
+  std::call_once(d.once, [&] {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      d.rc = STM_ERR_DEVICE;
+      snprintf(d.err, sizeof(d.err), "hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return;
+    }
+    Scratch probe;
+    const size_t need = probe.carve(STM_NOALLOC_MAX_OPS, nullptr);
+    if (need > NOALLOC_BYTES) {
+      d.rc = STM_ERR_STATE;
+      snprintf(d.err, sizeof(d.err), "layout of %zu bytes exceeds the %zu-byte array", need,
+               NOALLOC_BYTES);
+      return;
+    }
+    probe.carve(STM_NOALLOC_MAX_OPS, static_cast<uint8_t*>(p));
+    e = sort_scratch_init(probe.hist);
+    if (e != hipSuccess) {
+      d.rc = STM_ERR_DEVICE;
+      snprintf(d.err, sizeof(d.err), "sort control words: %s", hipGetErrorString(e));
+      return;
+    }
+    d.ptr = static_cast<uint8_t*>(p);
+  });
+  if (d.rc != STM_OK) return fail(d.rc, "no-allocation scratch on device %d: %s", dev, d.err);
+  s.carve(STM_NOALLOC_MAX_OPS, d.ptr);
+  s.h_counters = t->scr.h_counters;  // pinned at stm_create
+  return STM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -991,6 +1072,27 @@ int stm_run_batch_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_
   int rc = set_device(t);
   if (rc) return rc;
   return run_device(t, d_keys, d_vals, d_ops, n, d_out_vals, d_out_found);
+}
+
+int stm_run_batch_device_noalloc(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_vals,
+                                 const uint8_t* d_ops, size_t n, uint64_t* d_out_vals,
+                                 uint8_t* d_out_found) {
+  if (!t) return fail(STM_ERR_INVALID, "null tree");
+  if (n == 0) return STM_OK;
+  if (!d_keys || !d_vals || !d_ops || !d_out_vals || !d_out_found)
+    return fail(STM_ERR_INVALID, "null device array (device batches always carry values)");
+  if (n > STM_NOALLOC_MAX_OPS)
+    return fail(STM_ERR_INVALID, "batch of %zu ops exceeds the %u ops the no-allocation scratch "
+                "holds", n, (unsigned)STM_NOALLOC_MAX_OPS);
+  int rc = set_device(t);
+  if (rc) return rc;
+  if (t->cfg.route != STM_ROUTE_DESCENT && !t->route_alloc)
+    return fail(STM_ERR_STATE, "routing buffers do not exist yet: they are allocated by the first "
+                "training (stm_retrain or one stm_run_batch_device call)");
+  Scratch s;
+  rc = noalloc_scratch(t, s);
+  if (rc) return rc;
+  return run_chunk(t, d_keys, d_vals, d_ops, (uint32_t)n, d_out_vals, d_out_found, nullptr, &s);
 }
 
 int stm_run_batches_device(stm_tree* t, const uint64_t* d_keys, const uint64_t* d_vals,

@@ -149,6 +149,25 @@ int stm_run_batch(stm_tree* tree, const uint64_t* keys, const uint64_t* vals, co
 int stm_run_batch_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
                          const uint8_t* d_ops, size_t n, uint64_t* d_out_vals, uint8_t* d_out_found);
 
+/* Largest batch of one stm_run_batch_device_noalloc call. */
+#define STM_NOALLOC_MAX_OPS 65536
+
+/* stm_run_batch_device for callers that cannot allocate memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  No counterpart in the reference.
+ * Same arguments, results and errors as stm_run_batch_device for n <= STM_NOALLOC_MAX_OPS
+ * (larger n: STM_ERR_INVALID).  The batch's scratch (sorted keys and positions, sort control
+ * words and histograms, fold tiles, per-key groups, insert lists, counters: about 7 MiB) lives
+ * in one fixed device array per GPU, which the library holds until the process ends and sets
+ * up on the first call on each device; the call allocates nothing (no hipMalloc, hipHostMalloc
+ * or hipFree).  With learned or index routing the router's buffers must exist already (an
+ * earlier stm_run_batch_device or stm_retrain allocates them), else STM_ERR_STATE.  Every call
+ * on one GPU uses that GPU's one array, which keeps the last call's contents: calls that
+ * overlap in time on one GPU (two threads, or two trees on one device) race and must not be
+ * made. */
+int stm_run_batch_device_noalloc(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
+                                 const uint8_t* d_ops, size_t n, uint64_t* d_out_vals,
+                                 uint8_t* d_out_found);
+
 /* Longest range scan (pairs). */
 #define STM_SCAN_MAX_LEN 65536
 
