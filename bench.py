@@ -80,6 +80,12 @@ def parse():
                     help="tuning record to use for the uneven stream instead of autotuning")
     ap.add_argument("--tune-batches", type=int, default=40,
                     help="batches of the stream the autotuner measures")
+    ap.add_argument("--plain-trials", type=int, default=3,
+                    help="plain-uniform section: fresh-tree trials (0 = skip the section)")
+    ap.add_argument("--contended-batches", type=int, default=8,
+                    help="contended-Zipf section: timed batches (0 = skip the section)")
+    ap.add_argument("--only", choices=["plain", "contended", "uneven", "serving"], default=None,
+                    help="run one section alone and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's copy kernel 4 times over MIB MiB (the known-size "
                          "kernel HBM traffic counters are scaled by) and exit")
@@ -95,6 +101,26 @@ def copy_calibration(mib):
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """The host CPU: model, logical CPUs, the CPUs this process may run on, and the worker
+    threads the CPU baseline uses (every CPU granted to the process, at most 16: a GPU job on
+    these machines is given 16 host threads)."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        granted = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        granted = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "granted": granted,
+            "threads_used": max(1, min(granted, 16))}
 
 
 def record_name(path):
@@ -191,6 +217,125 @@ def run_uneven(args, ea, w):
     }
 
 
+def phase_summary(st, batches):
+    return {k: round(v / batches, 4) for k, v in st["phase_ms"].items() if st["phase_launches"][k]}
+
+
+def run_plain(args, ea, w):
+    """Plain method, small: 2^23 uniform keys (values splitmix64(key)), one batch of 2^22 ops,
+    95% lookups (90% of them hits) / 5% inserts of fresh keys.  Timed on a fresh tree per trial
+    after one untimed batch of the same shape: the device-resident call (stm_run_batch_device),
+    then the host-buffer call (stm_run_batch: pageable host arrays in, PCIe both ways, the
+    reference's transferData / transferAns, bpt-gpustm.cpp:313-335).  Median of the trials."""
+    n_load, n = 1 << 23, 1 << 22
+    t0 = time.time()
+    load = w.uniform_keys(n_load, args.seed + 11)
+    lv = w.values_for(load)
+    srt = w.sorted_copy(load)
+    keys, vals, ops = w.mixed_batch(srt, n, args.seed + 12, 0.95, 0.9)
+    wk, wv, wo = w.mixed_batch(srt, n, args.seed + 13, 0.95, 0.9)
+    log(f"[plain] generated in {time.time() - t0:.1f}s")
+    cfg = ea.StmConfig(max_keys=n_load + 2 * n + (1 << 20), max_batch=n, timing=True)
+    dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(lv)
+    bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
+    xk, xv, xo = (ea.DeviceBuffer.from_array(a) for a in (wk, wv, wo))
+    ov, of = ea.DeviceBuffer(n, np.uint64), ea.DeviceBuffer(n, np.uint8)
+    dev, host, phases = [], [], []
+    same = True
+    for _ in range(args.plain_trials):
+        with ea.StmTree(cfg) as t:
+            t.bulk_load_device(dk, dv)
+            t.run_batch_device(xk, xv, xo, ov, of)
+            t.reset_stats()
+            t.sync()
+            t1 = time.perf_counter()
+            t.run_batch_device(bk, bv, bo, ov, of)
+            dev.append(time.perf_counter() - t1)
+            phases.append(phase_summary(t.stats(), 1))
+            dres = (ov.download(), of.download())
+        with ea.StmTree(cfg) as t:
+            t.bulk_load_device(dk, dv)
+            t.run_batch(wk, wv, wo)
+            t.sync()
+            t1 = time.perf_counter()
+            hres = t.run_batch(keys, vals, ops)
+            host.append(time.perf_counter() - t1)
+        same = same and bool(np.array_equal(dres[0], hres[0]) and np.array_equal(dres[1], hres[1]))
+    for b in (dk, dv, bk, bv, bo, xk, xv, xo, ov, of):
+        b.free()
+    md, mh = float(np.median(dev)), float(np.median(host))
+    mid = phases[int(np.argsort(dev)[len(dev) // 2])]
+    log(f"[plain] device {n / md / 1e6:.1f} Mops/s, host buffers {n / mh / 1e6:.1f} Mops/s")
+    return {
+        "workload": "bulk-load 2^23 uniform keys; one batch of 2^22 ops, 95% lookup (90% hits) / "
+                    "5% insert of fresh keys",
+        "mops_device": round(n / md / 1e6, 2), "ms_device": round(md * 1e3, 3),
+        "mops_host_buffers": round(n / mh / 1e6, 2), "ms_host_buffers": round(mh * 1e3, 3),
+        "host_buffers_note": "stm_run_batch from pageable host arrays: 17 B/op in and 9 B/op out "
+                             "over PCIe inside the timed call",
+        "phase_ms": mid, "trials": args.plain_trials, "identical_device_vs_host_api": same,
+    }
+
+
+def run_contended(args, ea, w):
+    """Contended STM: 2^24 uniform keys; 8 batches of 2^22 ops, 50% lookups / 50% inserts, op
+    keys at Zipf(0.99) ranks of the loaded keys, inserts next to the chosen key so they hammer
+    the hot leaves.  One synchronous stm_run_batch_device call per batch after one untimed
+    batch of the same shape; abort / deferral counts from the library's counters."""
+    n_load, n, nb = 1 << 24, 1 << 22, args.contended_batches
+    t0 = time.time()
+    load = w.uniform_keys(n_load, args.seed + 21)
+    lv = w.values_for(load)
+    srt = w.sorted_copy(load)
+    keys = np.empty((nb + 1) * n, dtype=np.uint64)
+    vals = np.empty_like(keys)
+    ops = np.empty((nb + 1) * n, dtype=np.uint8)
+    for b in range(nb + 1):
+        s_ = slice(b * n, (b + 1) * n)
+        keys[s_], vals[s_], ops[s_] = w.zipf_batch(srt, n, args.seed * 100 + 30 + b, 0.99, 0.5)
+    log(f"[contended] generated in {time.time() - t0:.1f}s")
+    cfg = ea.StmConfig(max_keys=n_load + (nb + 1) * n + (1 << 20), max_batch=n, timing=True)
+    bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
+    ov, of = ea.DeviceBuffer(keys.size, np.uint64), ea.DeviceBuffer(keys.size, np.uint8)
+    with ea.StmTree(cfg) as t:
+        dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(lv)
+        t.bulk_load_device(dk, dv)
+        dk.free()
+        dv.free()
+        t.run_batch_device(bk, bv, bo, ov, of, n=n, offset=0)  # untimed
+        t.sync()
+        t.reset_stats()
+        lat = []
+        t1 = time.perf_counter()
+        for b in range(1, nb + 1):
+            tb = time.perf_counter()
+            t.run_batch_device(bk, bv, bo, ov, of, n=n, offset=b * n)
+            lat.append(time.perf_counter() - tb)
+        elapsed = time.perf_counter() - t1
+        st = t.stats()
+        bad = t.check()
+    for b in (bk, bv, bo, ov, of):
+        b.free()
+    mops = nb * n / elapsed / 1e6
+    mix = 0.5 * BYTES_LOOKUP + 0.5 * BYTES_INSERT
+    log(f"[contended] {mops:.1f} Mops/s, aborts {st['aborts']}, deferred {st['deferred']}, "
+        f"phases {phase_summary(st, nb)}")
+    return {
+        "workload": "bulk-load 2^24 uniform keys; 8 batches of 2^22 ops, 50% lookup / 50% insert, "
+                    "keys at Zipf(0.99) ranks, inserts next to the chosen key",
+        "mops": round(mops, 2), "ms_per_batch": round(elapsed / nb * 1e3, 3),
+        "p50_batch_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_batch_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "phase_ms_per_batch": phase_summary(st, nb),
+        "unique_ops": st["unique_ops"], "upserts": st["upserts"], "commits": st["commits"],
+        "aborts": st["aborts"], "aborts_per_upsert": round(st["aborts"] / max(1, st["upserts"]), 4),
+        "deferred": st["deferred"], "deferred_per_upsert": round(st["deferred"] / max(1, st["upserts"]), 5),
+        "leaf_splits": st["leaf_splits"], "inner_splits": st["inner_splits"],
+        "whole_batch_gbs": round(nb * n * mix / elapsed / 1e9, 1), "bytes_per_op": mix,
+        "tree_check_violations": bad,
+    }
+
+
 def run_serving(args, ea, w):
     """Serving loop: 2^25 uniform keys resident in HBM; open-loop Poisson arrivals at
     serve_mops; requests of floor(2^(16u)) ops (1 .. 2^16), 90% lookups, 8% inserts, 2% range
@@ -244,6 +389,11 @@ def main():
     import eirene_amd as ea
     from eirene_amd import workloads as w
 
+    if args.only:
+        fn = {"plain": run_plain, "contended": run_contended, "uneven": run_uneven,
+              "serving": run_serving}[args.only]
+        print(json.dumps({args.only: fn(args, ea, w)}), flush=True)
+        return
     if args.no_headline:
         print(json.dumps({"uneven": run_uneven(args, ea, w) if args.uneven_batches > 0 else None,
                           "serving": run_serving(args, ea, w) if args.serve_requests > 0 else None}),
@@ -428,7 +578,8 @@ def main():
     if not args.no_cpu:
         import oracle
         oracle.build()
-        threads = min(os.cpu_count() or 1, 16)
+        host = host_cpu()
+        threads = host["threads_used"]
         t0 = time.time()
         ex = oracle.CpuExecutor(load_sorted, w.values_for(load_sorted), threads)
         log(f"cpu executor built in {time.time() - t0:.1f}s with {threads} threads")
@@ -447,7 +598,11 @@ def main():
         cpu = {"value": round(cpu_ops / cpu_sec / 1e6, 2), "unit": "Mops/s", "cores": threads,
                "kind": "port",
                "sample": f"first {b} batches of the same workload ({cpu_ops} ops) on a "
-                         f"{threads}-thread key-range-partitioned CPU B+tree"}
+                         f"{threads}-thread key-range-partitioned CPU B+tree (persistent "
+                         f"worker threads)",
+               "what": "this repository's CPU restatement of the batch semantics "
+                       "(oracle/cpu_baseline.cpp); the reference has no CPU path",
+               "host": host}
 
     def summary(r):
         x = res[r]
@@ -464,6 +619,8 @@ def main():
     uneven = None
     serving = None
     bk.free(); bv.free(); bo.free(); out_v.free(); out_f.free()
+    plain = run_plain(args, ea, w) if args.plain_trials > 0 else None
+    contended = run_contended(args, ea, w) if args.contended_batches > 0 else None
     if args.uneven_batches > 0:
         uneven = run_uneven(args, ea, w)
     if args.serve_requests > 0:
@@ -507,6 +664,8 @@ def main():
         "stm": {k: st[k] for k in ("upserts", "commits", "aborts", "leaf_splits", "inner_splits",
                                    "root_splits", "deferred", "n_keys", "height", "node_reads")},
         "roofline": roofline, "cpu_baseline": cpu, "cpu_gpu_first_batch_agree": agree,
+        "plain": plain,
+        "contended": contended,
         "uneven": uneven,
         "serving": serving,
     }

@@ -124,6 +124,7 @@ struct Scratch {
   uint32_t* ilist = nullptr;  // merged INSERTs of the batch
   uint32_t* upar = nullptr;   // level-1 node hint of every merged INSERT
   uint64_t* uver = nullptr;   // version of the hinted leaf
+  uint64_t* fanout = nullptr; // answer chunks of hot GET groups (fanout_capacity(n))
   BatchCounters* counters = nullptr;
   BatchCounters* h_counters = nullptr;  // pinned host copy for the per-batch readback
   // range scans
@@ -159,6 +160,7 @@ struct Scratch {
     A_(ilist, cap);
     A_(upar, cap);
     A_(uver, cap);
+    A_(fanout, fanout_capacity(cap));
     A_(counters, 1);
     if (staging) {
       A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
@@ -189,7 +191,7 @@ struct Scratch {
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
-                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, scount, ilist, upar, uver, counters,
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, scount, ilist, upar, uver, fanout, counters,
                   desc_table, desc_live, cursor, acc, soff, lastins, scan_tiles};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -497,6 +499,9 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
   la.uleaf = s.uleaf;
   la.uver = s.uver;
   la.upar = s.upar;
+  la.gval = s.uval;  // GET groups carry no merged value: a hot group's value goes there
+  la.fanout = s.fanout;
+  la.fanout_cap = fanout_capacity(in.n);
   if ((e = launch_lookup(t->view, la, st)) != hipSuccess) return e;
 
   if (in.scans) {  // on the tree as it was before the batch: before the upsert kernel
@@ -532,6 +537,9 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
   ua.slist = s.slist;
   ua.scount = s.scount;
   ua.lanes = t->cfg.upsert_lanes;
+  ua.fanout = s.fanout;
+  ua.fanout_cap = fanout_capacity(in.n);
+  ua.gval = s.uval;
   ua.host_counters = in.host_counters;
   if ((e = launch_upsert(t->view, ua, st)) != hipSuccess) return e;
   if (timed) phase_end(t, STM_PHASE_UPSERT);

@@ -71,7 +71,7 @@ struct BatchCounters {
   uint32_t scan_bad;    // scan ops rejected (length or output area)
   uint64_t scan_total;  // sum of the scan lengths (pairs the output area must hold)
   uint32_t sort_stall;  // the batch's sort gave up a look-back wait
-  uint32_t pad0;
+  uint32_t n_fanout;    // answer chunks of hot GET groups queued for k_fanout
   uint64_t hdr_error;   // the tree's sticky error bits after the batch (one readback per batch)
   uint32_t pad[4];
 };

@@ -263,7 +263,10 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
                                                 const BatchDesc* __restrict__ desc,
                                                 uint32_t* __restrict__ uleaf,
                                                 uint64_t* __restrict__ uver,
-                                                uint32_t* __restrict__ upar) {
+                                                uint32_t* __restrict__ upar,
+                                                uint64_t* __restrict__ gval,
+                                                uint64_t* __restrict__ fanout,
+                                                uint64_t fanout_cap) {
   constexpr int K = F / G;
   const int lane = lane_id();
   const int g = lane & (G - 1);
@@ -427,14 +430,26 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
     // All values of the sub-chunk in one parallel round trip, then the answers.
     if (mine) {
       const uint64_t val = found_l ? t.vals[vidx_l] : 0;
-      if (np_l) {
-        out_vals[pos_l] = val;
-        out_found[pos_l] = found_l ? 1 : 0;
-      }
-      for (uint32_t p = 1; p < np_l; ++p) {
-        const uint32_t pos = pos_of(sidx[gs_l + p]);
-        out_vals[pos] = val;
-        out_found[pos] = found_l ? 1 : 0;
+      if (np_l > FANOUT_DIRECT) {
+        // A hot key: its waiting ops are answered by the whole grid (fanout_answers), in
+        // chunks queued here, instead of one store after another from this lane.
+        gval[j] = val;
+        const uint32_t nch = (np_l + FANOUT_CHUNK - 1) / FANOUT_CHUNK;
+        const uint32_t c0 = atomicAdd(&const_cast<BatchCounters*>(cnt)->n_fanout, nch);
+        for (uint32_t c = 0; c < nch; ++c)
+          if ((uint64_t)c0 + c < fanout_cap)
+            fanout[c0 + c] = (uint64_t)j | ((uint64_t)(c * FANOUT_CHUNK) << 32) |
+                             (found_l ? (1ull << 63) : 0ull);
+      } else {
+        if (np_l) {
+          out_vals[pos_l] = val;
+          out_found[pos_l] = found_l ? 1 : 0;
+        }
+        for (uint32_t p = 1; p < np_l; ++p) {
+          const uint32_t pos = pos_of(sidx[gs_l + p]);
+          out_vals[pos] = val;
+          out_found[pos] = found_l ? 1 : 0;
+        }
       }
     }
   }
@@ -1157,6 +1172,33 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
 // hands back go to the same part of slist, from its start, and their number to scount[wave]:
 // no counter is shared (one hot address took an atomic from every wavefront at once, at the
 // end of the launch).
+// Answers of the hot GET groups the lookup kernel queued (more than FANOUT_DIRECT waiting
+// ops): chunk c of the queue is op range [first, first + FANOUT_CHUNK) of group g's waiting
+// ops, written by one workgroup, chunks spread over the grid.  Runs at the start of the upsert
+// fast path, whose writes (INSERT groups' answers) never touch a GET group's ops.  Reference:
+// the per-op answer write-back, writebackAns (src/gpu-stm-tree-v38/bpt-gpustm-kernel-aux.cuh:46-56).
+STM_DEV void fanout_answers(const uint64_t* __restrict__ fanout, uint64_t cap,
+                            const BatchCounters* cnt, const uint32_t* __restrict__ gstart,
+                            const uint32_t* __restrict__ npend, const uint64_t* __restrict__ gval,
+                            const uint32_t* __restrict__ sidx, uint64_t* __restrict__ out_vals,
+                            uint8_t* __restrict__ out_found) {
+  const uint32_t nc = (uint32_t)min<uint64_t>(cnt->n_fanout, cap);
+  for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    const uint64_t e = fanout[c];
+    const uint32_t g = (uint32_t)e;
+    const uint32_t first = (uint32_t)(e >> 32) & 0x7fffffffu;
+    const uint8_t found = (uint8_t)(e >> 63);
+    const uint32_t gs = gstart[g];
+    const uint32_t end = min(npend[g], first + FANOUT_CHUNK);
+    const uint64_t v = gval[g];
+    for (uint32_t p = first + threadIdx.x; p < end; p += blockDim.x) {
+      const uint32_t pos = pos_of(sidx[gs + p]);
+      out_vals[pos] = v;
+      out_found[pos] = found;
+    }
+  }
+}
+
 template <int F, int G>
 __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t* __restrict__ ukey,
                                                      const uint64_t* __restrict__ uval,
@@ -1170,13 +1212,17 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
                                                      uint32_t* __restrict__ scount,
                                                      uint64_t* __restrict__ out_vals,
                                                      uint8_t* __restrict__ out_found,
-                                                     const BatchDesc* __restrict__ desc) {
+                                                     const BatchDesc* __restrict__ desc,
+                                                     const uint64_t* __restrict__ fanout,
+                                                     uint64_t fanout_cap,
+                                                     const uint64_t* __restrict__ gval) {
   constexpr int NG = WAVE / G;
   const uint32_t N = cnt->n_upserts;
   if (desc) {
     out_vals = desc->out_vals;
     out_found = desc->out_found;
   }
+  if (fanout) fanout_answers(fanout, fanout_cap, cnt, gstart, npend, gval, sidx, out_vals, out_found);
   const uint32_t nw = gridDim.x * (blockDim.x / WAVE);
   const uint32_t w = uniform32((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   const uint32_t per = (N + nw - 1) / nw;
@@ -1506,17 +1552,20 @@ static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipSt
     k_lookup<F, G, true, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                        a.counters, a.chunk, a.out_vals, a.out_found,
                                                        a.route_hdr, a.fences, a.lids, a.segs, a.desc,
-                                                       a.uleaf, a.uver, a.upar);
+                                                       a.uleaf, a.uver, a.upar, a.gval, a.fanout,
+                                                       a.fanout_cap);
   else if (a.route_hdr)
     k_lookup<F, G, true, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                         a.counters, a.chunk, a.out_vals, a.out_found,
                                                         a.route_hdr, a.fences, a.lids, a.segs, a.desc,
-                                                        a.uleaf, a.uver, a.upar);
+                                                        a.uleaf, a.uver, a.upar, a.gval, a.fanout,
+                                                        a.fanout_cap);
   else
     k_lookup<F, G, false, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                          a.counters, a.chunk, a.out_vals, a.out_found,
                                                          nullptr, nullptr, nullptr, nullptr, a.desc,
-                                                         a.uleaf, a.uver, a.upar);
+                                                         a.uleaf, a.uver, a.upar, a.gval, a.fanout,
+                                                         a.fanout_cap);
   return hipGetLastError();
 }
 
@@ -1557,7 +1606,8 @@ static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hip
   waves = blocks * 4;  // one scount entry per wavefront
   k_upsert_fast<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx,
                                                 a.ilist, a.uleaf, a.uver, a.counters, a.slist,
-                                                a.scount, a.out_vals, a.out_found, a.desc);
+                                                a.scount, a.out_vals, a.out_found, a.desc,
+                                                a.fanout, a.fanout_cap, a.gval);
   return hipGetLastError();
 }
 

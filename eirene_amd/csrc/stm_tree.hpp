@@ -10,6 +10,14 @@ namespace stm {
 // it handed back.
 constexpr uint32_t FAST_WAVES_MAX = 8192;
 
+// Answers of a GET group with more than FANOUT_DIRECT waiting ops (a hot key) are not written
+// by the one lane that looked the key up: the lookup kernel queues the group's answer range in
+// chunks of FANOUT_CHUNK ops and k_fanout writes them with the whole grid.  A batch of n ops
+// queues at most fanout_capacity(n) chunks.
+constexpr uint32_t FANOUT_DIRECT = 32;
+constexpr uint32_t FANOUT_CHUNK = 1024;
+inline uint64_t fanout_capacity(uint64_t n) { return n / FANOUT_CHUNK + n / FANOUT_DIRECT + 1; }
+
 struct LookupArgs {
   const uint64_t* ukey;
   const uint8_t* uop;
@@ -32,6 +40,9 @@ struct LookupArgs {
   uint32_t* uleaf;        // optional: leaf of every merged INSERT (the upsert kernels' hint),
   uint64_t* uver;         //   that leaf's lock word when the lookup kernel ran,
   uint32_t* upar;         //   and the level-1 node that routed it (NODE_NONE if none)
+  uint64_t* gval;         // a hot GET group's value (per group; GET groups leave uval unused)
+  uint64_t* fanout;       // queued answer chunks: group | first op << 32 | found << 63
+  uint64_t fanout_cap;    // fanout_capacity(n_max)
 };
 
 struct UpsertArgs {
@@ -56,6 +67,9 @@ struct UpsertArgs {
                           //   wavefront's own range of the insert list
   uint32_t* scount;       // per fast-path wavefront: ops it handed back (capacity: FAST_WAVES_MAX)
   uint32_t lanes;         // lanes per op of the fast path: 8, 16 or 32
+  const uint64_t* fanout; // hot GET groups' answer chunks queued by the lookup kernel (the fast
+  uint64_t fanout_cap;    //   path writes them first); null: none queued
+  const uint64_t* gval;   // their values (per group)
   BatchCounters* host_counters;  // optional: pinned host memory the last kernel copies the
                                  //   counters into (null: the caller copies them back)
 };

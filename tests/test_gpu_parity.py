@@ -185,6 +185,54 @@ def test_forced_aborts_run_the_serial_pass(oracle_mod, fanout):
         assert_dump(t, o)
 
 
+@pytest.mark.parametrize("graphs", [False, True])
+def test_hot_get_groups_fan_out(oracle_mod, graphs):
+    """Keys read by many ops of one batch (more than 32 waiting ops: the lookup kernel queues
+    their answers in chunks of 1024 that the whole grid writes): group sizes on both sides of
+    both limits, present and absent keys, a hot key whose GETs precede an INSERT of it, batches
+    through the synchronous call and replayed graphs."""
+    from eirene_amd import DeviceBuffer
+    rng = np.random.default_rng(61)
+    load = rng.integers(0, 1 << 63, size=50000, dtype=np.uint64)
+    lv = rng.integers(0, 1 << 63, size=load.size, dtype=np.uint64)
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, lv)
+    parts, opp = [], []
+    absent = rng.integers(0, 1 << 63, size=4, dtype=np.uint64)
+    for i, m in enumerate([1, 31, 32, 33, 1023, 1024, 1025, 2049, 7000]):
+        parts.append(np.full(m, load[100 * i], dtype=np.uint64))
+        opp.append(np.ones(m, dtype=np.uint8))
+    for i, m in enumerate([33, 3000]):
+        parts.append(np.full(m, absent[i], dtype=np.uint64))
+        opp.append(np.ones(m, dtype=np.uint8))
+    # hot GETs then one INSERT of the same key (present and absent), then more GETs
+    for k in (load[5], absent[2]):
+        parts.append(np.full(2500, k, dtype=np.uint64))
+        op = np.ones(2500, dtype=np.uint8)
+        op[1500] = 0
+        opp.append(op)
+    parts.append(rng.choice(load, 20000))
+    opp.append((rng.random(20000) < 0.8).astype(np.uint8))
+    keys = np.concatenate(parts)
+    ops = np.concatenate(opp)
+    perm = rng.permutation(keys.size)
+    keys, ops = keys[perm], ops[perm]
+    vals = rng.integers(0, 1 << 63, size=keys.size, dtype=np.uint64)
+    offsets = np.array([0, keys.size // 3, keys.size], dtype=np.uint64)
+    want_v = np.zeros(keys.size, dtype=np.uint64)
+    want_f = np.zeros(keys.size, dtype=np.uint8)
+    for b in range(2):
+        a, z = int(offsets[b]), int(offsets[b + 1])
+        want_v[a:z], want_f[a:z] = o.run_batch(keys[a:z], vals[a:z], ops[a:z])
+    with tree(use_graphs=graphs) as t:
+        t.bulk_load(load, lv)
+        dk, dv, do = (DeviceBuffer.from_array(x) for x in (keys, vals, ops))
+        ov, of = DeviceBuffer(keys.size, np.uint64), DeviceBuffer(keys.size, np.uint8)
+        t.run_batches_device(dk, dv, do, ov, of, offsets)
+        assert_same((ov.download(), of.download()), (want_v, want_f), f"graphs={graphs}")
+        assert_dump(t, o)
+
+
 def test_plain_uniform_reduced(oracle_mod):
     from eirene_amd import workloads as w
     load = w.uniform_keys(1 << 20, 1)
