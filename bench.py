@@ -330,6 +330,7 @@ def run_contended(args, ea, w):
         "unique_ops": st["unique_ops"], "upserts": st["upserts"], "commits": st["commits"],
         "aborts": st["aborts"], "aborts_per_upsert": round(st["aborts"] / max(1, st["upserts"]), 4),
         "deferred": st["deferred"], "deferred_per_upsert": round(st["deferred"] / max(1, st["upserts"]), 5),
+        "sort_fallbacks": st["sort_fallbacks"],
         "leaf_splits": st["leaf_splits"], "inner_splits": st["inner_splits"],
         "whole_batch_gbs": round(nb * n * mix / elapsed / 1e9, 1), "bytes_per_op": mix,
         "tree_check_violations": bad,

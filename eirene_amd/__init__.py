@@ -64,7 +64,7 @@ class _Stats(C.Structure):
         "upserts", "commits", "aborts", "leaf_splits", "inner_splits", "root_splits",
         "deferred", "route_hits", "route_misses")] + [
         ("phase_ms", C.c_double * 8), ("phase_launches", C.c_uint64 * 8),
-        ("node_reads", C.c_uint64)]
+        ("node_reads", C.c_uint64), ("sort_fallbacks", C.c_uint64)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -96,6 +96,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "stm_run_batch_scan_device": (C.c_int, [_P, _P, _P, _P, C.c_size_t, _P, _P, _P, _P,
                                                 C.c_size_t]),
         "stm_capture_graphs": (C.c_int, [_P]),
+        "stm_sort_pairs_device": (C.c_int, [C.c_int, _P, C.c_size_t, _P, _P]),
         "stm_size": (C.c_int, [_P, _U64P]),
         "stm_dump": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
         "stm_check": (C.c_int, [_P, _U64P]),
@@ -206,6 +207,22 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def sort_pairs(keys, device: int = 0):
+    """The library's stable device sort of u64 keys: returns (sorted keys, input positions)."""
+    k = _u64(keys)
+    n = k.size
+    if n == 0:
+        return k.copy(), np.zeros(0, dtype=np.uint32)
+    src = DeviceBuffer.from_array(k)
+    ok, oi = DeviceBuffer(n, np.uint64), DeviceBuffer(n, np.uint32)
+    try:
+        _check(load_library().stm_sort_pairs_device(device, src.ptr, n, ok.ptr, oi.ptr))
+        return ok.download(), oi.download()
+    finally:
+        for b in (src, ok, oi):
+            b.free()
 
 
 @dataclass
@@ -388,6 +405,7 @@ class StmTree:
         d["phase_ms"] = {p: s.phase_ms[i] for i, p in enumerate(PHASES)}
         d["phase_launches"] = {p: s.phase_launches[i] for i, p in enumerate(PHASES)}
         d["node_reads"] = s.node_reads
+        d["sort_fallbacks"] = s.sort_fallbacks
         return d
 
     def reset_stats(self) -> None:

@@ -61,7 +61,8 @@ struct StreamAcc {
   unsigned long long batches;  // batches that ran
   unsigned long long skipped;  // batches skipped after an error
   unsigned long long sort_stalls;  // batches whose sort gave up a look-back wait
-  unsigned long long pad[3];
+  unsigned long long fallbacks;    // batches whose normalized sort fell back to the exact passes
+  unsigned long long pad[2];
 };
 
 constexpr uint32_t DESC_TABLE = 4096;  // descriptors uploaded per window of a stream
@@ -92,6 +93,7 @@ __global__ void k_epilogue(const BatchCounters* __restrict__ cnt, const BatchDes
   acc->unique += cnt->n_unique;
   acc->bad += cnt->n_bad;
   acc->sort_stalls += cnt->sort_stall;
+  acc->fallbacks += cnt->sort_fallback;
   acc->batches += 1;
 }
 
@@ -221,9 +223,18 @@ struct stm_tree {
   uint32_t batches_since_train = 0;
   // one instantiated graph per batch size class (null until first needed)
   hipGraphExec_t graphs[GRAPH_CLASSES] = {};
+  // Batches above 2^18 ops sort normalized keys unless exact_left > 0: a batch whose normalized
+  // sort had to fall back (long buckets of distinct keys: hot keys with neighbours) makes the
+  // next EXACT_HOLD batches use the eight exact passes directly.  graphs_exact: the form the
+  // captured graphs hold.
+  uint32_t exact_left = 0;
+  bool graphs_exact = false;
+  uint64_t sort_fallbacks = 0;  // batches whose normalized sort fell back (stm_stats)
 };
 
 namespace {
+
+constexpr uint32_t EXACT_HOLD = 16;
 
 uint64_t capacity_for(uint64_t max_keys, uint32_t F) {
   const uint64_t half = F / 2;
@@ -422,12 +433,12 @@ struct PipeIn {
   BatchCounters* host_counters;
 };
 
-hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool timed) {
+hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool timed, bool exact) {
   hipStream_t st = t->stream;
   hipError_t e;
   if (timed) phase_begin(t, STM_PHASE_SORT);
   e = radix_sort_pairs_desc(in.keys, in.ops, in.desc, in.n, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, 0,
-                            64, s.hist, (int)t->cfg.sort_tile_items, st);
+                            64, s.hist, (int)t->cfg.sort_tile_items, exact, st);
   if (e != hipSuccess) return e;
   if (timed) phase_switch(t, STM_PHASE_SORT, STM_PHASE_MERGE);
 
@@ -450,6 +461,7 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
   fa.lastins = in.scans ? s.lastins : nullptr;
   fa.ilist = s.ilist;
   fa.sort_stall = sort_stall_word(s.hist);
+  fa.sort_exact = sort_exact_word(s.hist);
   if ((e = fold_batch(fa, st)) != hipSuccess) return e;
   if (timed) phase_switch(t, STM_PHASE_MERGE, STM_PHASE_LOOKUP);
 
@@ -581,7 +593,8 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   } else {
     in.host_counters = s.h_counters;  // the scan answers kernel runs after the upsert kernels
   }
-  HIP_TRY(enqueue_pipeline(t, s, in, true));
+  const bool exact = t->exact_left > 0;
+  HIP_TRY(enqueue_pipeline(t, s, in, true, exact));
   // One readback per batch, into pinned memory: counters and the tree's error bits.
   if (so)
     HIP_TRY(hipMemcpyAsync(s.h_counters, s.counters, sizeof(BatchCounters), hipMemcpyDeviceToHost, st));
@@ -589,6 +602,12 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   const BatchCounters bc = *s.h_counters;
   phase_collect(t);
   if (bc.sort_stall) return fail(STM_ERR_DEVICE, "batch sort: a look-back wait gave up");
+  if (bc.sort_fallback) {
+    t->exact_left = EXACT_HOLD;
+    t->sort_fallbacks += 1;
+  } else if (exact) {
+    t->exact_left -= 1;
+  }
   t->st.batches += 1;
   t->st.ops += n;
   t->st.unique_ops += bc.n_unique;
@@ -626,7 +645,7 @@ int capture_class(stm_tree* t, int c) {
   hipError_t e = hipGetLastError();
   PipeIn in{nullptr, nullptr, nullptr, class_nmax(t, c), nullptr, nullptr, s.desc_live,
             false, nullptr, nullptr, 0};
-  if (e == hipSuccess) e = enqueue_pipeline(t, s, in, false);
+  if (e == hipSuccess) e = enqueue_pipeline(t, s, in, false, t->graphs_exact);
   if (e == hipSuccess) {
     k_epilogue<<<1, 64, 0, st>>>(s.counters, s.desc_live, s.acc);
     e = hipGetLastError();
@@ -662,6 +681,10 @@ int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const ui
   if (descs.empty()) return STM_OK;
   int rc = maybe_retrain(t);  // learned routing: the router's buffers exist before capture
   if (rc) return rc;
+  if (t->graphs_exact != (t->exact_left > 0)) {  // graphs of the other sort form
+    drop_graphs(t);
+    t->graphs_exact = t->exact_left > 0;
+  }
   for (uint8_t c : cls) {
     rc = capture_class(t, c);
     if (rc) return rc;
@@ -686,6 +709,9 @@ int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const ui
   HIP_TRY(hipMemcpyAsync(&acc, s.acc, sizeof(acc), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   t->st.unique_ops += acc.unique;
+  t->sort_fallbacks += acc.fallbacks;
+  if (acc.fallbacks) t->exact_left = EXACT_HOLD;
+  else if (t->graphs_exact) t->exact_left -= (uint32_t)std::min<uint64_t>(t->exact_left, acc.batches);
   if (acc.sort_stalls)
     return fail(STM_ERR_DEVICE, "a batch sort gave up a look-back wait (%llu later batches skipped)",
                 acc.skipped);
@@ -1221,12 +1247,46 @@ int stm_run_batch_scan(stm_tree* t, const uint64_t* keys, const uint64_t* vals, 
   return rc;
 }
 
+int stm_sort_pairs_device(int device, const uint64_t* d_keys, size_t n, uint64_t* d_keys_out,
+                          uint32_t* d_idx_out) {
+  if (n == 0) return STM_OK;
+  if (!d_keys || !d_keys_out || !d_idx_out) return fail(STM_ERR_INVALID, "null device array");
+  if (n > (1ull << 30)) return fail(STM_ERR_INVALID, "sort of %zu keys exceeds 2^30", n);
+  HIP_TRY(hipSetDevice(device));
+  uint64_t* ktmp = nullptr;
+  uint32_t *itmp = nullptr, *words = nullptr;
+  int rc = STM_OK;
+  if (dalloc(&ktmp, n) != hipSuccess || dalloc(&itmp, n) != hipSuccess ||
+      dalloc(&words, sort_hist_words(n)) != hipSuccess) {
+    rc = fail(STM_ERR_NOMEM, "sort scratch for %zu keys", n);
+  } else {
+    hipError_t e = sort_scratch_init(words);
+    if (e == hipSuccess)
+      e = radix_sort_pairs(d_keys, ktmp, d_keys_out, itmp, d_idx_out, (uint32_t)n, 0, 64, words,
+                           nullptr);
+    uint32_t stalled = 0;
+    if (e == hipSuccess)
+      e = hipMemcpy(&stalled, sort_stall_word(words), sizeof(stalled), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(STM_ERR_DEVICE, "sort: %s", hipGetErrorString(e));
+    else if (stalled) rc = fail(STM_ERR_DEVICE, "sort: a bounded wait between workgroups gave up");
+  }
+  (void)hipDeviceSynchronize();
+  void* ps[] = {ktmp, itmp, words};
+  for (void* q : ps)
+    if (q) (void)hipFree(q);
+  return rc;
+}
+
 int stm_capture_graphs(stm_tree* t) {
   if (!t) return fail(STM_ERR_INVALID, "null tree");
   int rc = set_device(t);
   if (rc) return rc;
   rc = maybe_retrain(t);  // learned routing: the router's buffers exist before capture
   if (rc) return rc;
+  if (t->graphs_exact != (t->exact_left > 0)) {
+    drop_graphs(t);
+    t->graphs_exact = t->exact_left > 0;
+  }
   const int last = class_of(t, (uint32_t)t->cfg.max_batch);
   for (int c = 0; c <= last; ++c) {
     rc = capture_class(t, c);
@@ -1382,6 +1442,7 @@ int stm_get_stats(stm_tree* t, stm_stats* out) {
   out->route_hits = c[ST_ROUTE_HITS];
   out->route_misses = c[ST_ROUTE_MISSES];
   out->node_reads = c[ST_NODE_READS];
+  out->sort_fallbacks = t->sort_fallbacks;
   return STM_OK;
 }
 
@@ -1471,6 +1532,7 @@ int stm_reset_stats(stm_tree* t) {
   int rc = set_device(t);
   if (rc) return rc;
   t->st = stm_stats{};
+  t->sort_fallbacks = 0;
   HIP_TRY(hipMemsetAsync(t->view.stats, 0, sizeof(unsigned long long) * STATS_RESET_WORDS, t->stream));
   HIP_TRY(hipStreamSynchronize(t->stream));
   return STM_OK;

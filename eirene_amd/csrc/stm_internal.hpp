@@ -54,11 +54,17 @@ hipError_t radix_sort_pairs(const uint64_t* keys_in, uint64_t* keys_tmp, uint64_
 // With desc the keys, op codes and their count are read from the device-side descriptor
 // (keys_in and ops_in unused) and n_max only bounds that count (grid and scratch sizing);
 // without it keys_in holds n_max keys.
+// Above 2^18 keys the normalized-key form runs unless `exact` asks for the eight-pass form
+// (the library asks for it after a batch whose normalized sort had to fall back).
 hipError_t radix_sort_pairs_desc(const uint64_t* keys_in, const uint8_t* ops_in,
                                  const BatchDesc* desc, uint32_t n_max,
                                  uint64_t* keys_tmp, uint64_t* keys_out, uint32_t* idx_tmp,
                                  uint32_t* idx_out, int begin_bit, int end_bit, uint32_t* hist_words,
-                                 int tile_items, hipStream_t stream);
+                                 int tile_items, bool exact, hipStream_t stream);
+// Non-zero after a normalized-key sort that found a long bucket of distinct keys and re-sorted
+// the batch with the eight exact passes inside one launch (correct, but slower); cleared when
+// the next sort on the same scratch starts.
+const uint32_t* sort_exact_word(const uint32_t* words);
 
 // ------------------------------------------------------------------ per-key fold (stm_fold.hip)
 // Per-batch counters written by the fold and the tree kernels (one 64-byte block).
@@ -73,7 +79,8 @@ struct BatchCounters {
   uint32_t sort_stall;  // the batch's sort gave up a look-back wait
   uint32_t n_fanout;    // answer chunks of hot GET groups queued for k_fanout
   uint64_t hdr_error;   // the tree's sticky error bits after the batch (one readback per batch)
-  uint32_t pad[4];
+  uint32_t sort_fallback;  // the batch's normalized sort fell back to the exact passes
+  uint32_t pad[3];
 };
 static_assert(sizeof(BatchCounters) == 64, "one 64-byte block");
 
@@ -99,6 +106,7 @@ struct FoldArgs {
   uint32_t* ilist;        // optional: the groups whose merged op is INSERT (n_upserts of them,
                           // key order within each fold tile)
   const uint32_t* sort_stall;  // optional: the sort's stall word, copied into the counters
+  const uint32_t* sort_exact;  // optional: the sort's fallback word, copied into the counters
 };
 constexpr uint8_t UOP_SKIP = 0xff;
 constexpr uint8_t OP_RANGE = 3;

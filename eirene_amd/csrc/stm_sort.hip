@@ -42,6 +42,7 @@
 // serves any batch size up to the one it was sized for.  The tile is SORT_BLOCK * ITEMS
 // elements (ITEMS = 8 or 16, chosen by the autotuner).
 #include <algorithm>
+#include <atomic>
 
 #include "stm_common.hpp"
 #include "stm_internal.hpp"
@@ -77,11 +78,15 @@ constexpr uint32_t CTL_TICKETS = 0;  // [MAX_PASSES][TICKET_CLASSES] * 32
 constexpr uint32_t CTL_EPOCH = MAX_PASSES * TICKET_CLASSES * 32;
 constexpr uint32_t CTL_WRAPPED = CTL_EPOCH + 32;
 constexpr uint32_t CTL_STALL = CTL_WRAPPED + 32;
-constexpr uint32_t GOFF_OFF = (CTL_STALL + 32 + 63) / 64 * 64;      // goff[MAX_PASSES][RADIX]
+constexpr uint32_t CTL_NEED_EXACT = CTL_STALL + 32;  // normalized sort: a long bucket holds distinct keys
+constexpr uint32_t CTL_BARRIER = CTL_NEED_EXACT + 32;  // grid-barrier arrivals of k_sort_exact
+constexpr uint32_t GOFF_OFF = (CTL_BARRIER + 32 + 63) / 64 * 64;    // goff[MAX_PASSES][RADIX]
 constexpr uint32_t PART_OFF = GOFF_OFF + MAX_PASSES * RADIX;        // [HIST_GRID_CAP][MAX_PASSES][RADIX]
 constexpr uint32_t STATUS_OFF = PART_OFF + HIST_GRID_CAP * MAX_PASSES * RADIX;  // u64 [OS_MAX_TILES][RADIX]
-constexpr uint32_t ONESWEEP_WORDS = STATUS_OFF + 2 * OS_MAX_TILES * RADIX;
-static_assert(STATUS_OFF % 2 == 0 && ONESWEEP_WORDS % 64 == 0, "aligned regions");
+constexpr uint32_t RANGE_BLOCKS = 256;       // k_range workgroups: one (min, max) partial each
+constexpr uint32_t RANGE_OFF = STATUS_OFF + 2 * OS_MAX_TILES * RADIX;  // u64 [RANGE_BLOCKS][2]
+constexpr uint32_t ONESWEEP_WORDS = RANGE_OFF + 4 * RANGE_BLOCKS;
+static_assert(STATUS_OFF % 2 == 0 && RANGE_OFF % 2 == 0 && ONESWEEP_WORDS % 64 == 0, "aligned regions");
 
 // Exclusive scan of one u32 per thread over a 256-thread block; returns the block total.
 STM_DEV uint32_t block_exclusive_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t& total) {
@@ -130,15 +135,14 @@ STM_DEV SortRange sort_range(const BatchDesc* d, uint32_t n_host, const uint64_t
   return r;
 }
 
+// The three kernels of one pass are device functions too: k_sort_exact runs them in turn
+// inside one persistent launch.
 template <int ITEMS>
-__global__ __launch_bounds__(SORT_BLOCK) void k_hist(const uint64_t* __restrict__ keys,
-                                                     const BatchDesc* __restrict__ desc,
-                                                     uint32_t n_host, int first_pass, int shift,
-                                                     uint32_t* __restrict__ hist, uint32_t stride,
-                                                     uint32_t* __restrict__ stall) {
+STM_DEV void hist_body(const uint64_t* __restrict__ keys, const BatchDesc* __restrict__ desc,
+                       uint32_t n_host, int first_pass, int shift, uint32_t* __restrict__ hist,
+                       uint32_t stride) {
   constexpr int TILE = SORT_BLOCK * ITEMS;
   __shared__ uint32_t s_h[SORT_WAVES][RADIX];
-  if (first_pass && blockIdx.x == 0 && threadIdx.x == 0) *stall = 0;  // no look-back in this form
   const SortRange rg = sort_range(desc, n_host, keys, first_pass);
   const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
   const int wave = threadIdx.x / WAVE;
@@ -159,16 +163,31 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_hist(const uint64_t* __restrict_
   }
 }
 
-// Block d scans hist[d][0..ntiles) in place (exclusive) and writes totals[d].
 template <int ITEMS>
-__global__ __launch_bounds__(SORT_BLOCK) void k_scan_hist(uint32_t* __restrict__ hist, uint32_t stride,
-                                                          const BatchDesc* __restrict__ desc,
-                                                          uint32_t n_host, uint32_t* __restrict__ totals) {
+__global__ __launch_bounds__(SORT_BLOCK) void k_hist(const uint64_t* __restrict__ keys,
+                                                     const BatchDesc* __restrict__ desc,
+                                                     uint32_t n_host, int first_pass, int shift,
+                                                     uint32_t* __restrict__ hist, uint32_t stride,
+                                                     uint32_t* __restrict__ words) {
+  if (first_pass && blockIdx.x == 0 && threadIdx.x == 0) {
+    words[CTL_STALL] = 0;  // no look-back in this form
+    words[CTL_NEED_EXACT] = 0;
+  }
+  hist_body<ITEMS>(keys, desc, n_host, first_pass, shift, hist, stride);
+}
+
+// Row d of hist (the tiles' counts of digit d) is scanned in place (exclusive) and its total
+// written to totals[d]; workgroup g takes rows g, g + gridDim.x, ...
+template <int ITEMS>
+STM_DEV void scan_hist_body(uint32_t* __restrict__ hist, uint32_t stride,
+                            const BatchDesc* __restrict__ desc, uint32_t n_host,
+                            uint32_t* __restrict__ totals) {
   constexpr int TILE = SORT_BLOCK * ITEMS;
   __shared__ uint32_t s_tmp[SORT_WAVES];
   const uint32_t n = desc ? desc->n : n_host;
   const uint32_t ntiles = (n + TILE - 1) / TILE;
-  uint32_t* row = hist + (uint64_t)blockIdx.x * stride;
+  for (uint32_t dg = blockIdx.x; dg < RADIX; dg += gridDim.x) {
+  uint32_t* row = hist + (uint64_t)dg * stride;
   const uint32_t per = (ntiles + SORT_BLOCK - 1) / SORT_BLOCK;
   const uint32_t b0 = min(threadIdx.x * per, ntiles);
   const uint32_t b1 = min(b0 + per, ntiles);
@@ -198,11 +217,20 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scan_hist(uint32_t* __restrict__
       run += c;
     }
   }
-  if (threadIdx.x == 0) totals[blockIdx.x] = total;
+  if (threadIdx.x == 0) totals[dg] = total;
+  __syncthreads();
+  }  // rows
 }
 
 template <int ITEMS>
-__global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
+__global__ __launch_bounds__(SORT_BLOCK) void k_scan_hist(uint32_t* __restrict__ hist, uint32_t stride,
+                                                          const BatchDesc* __restrict__ desc,
+                                                          uint32_t n_host, uint32_t* __restrict__ totals) {
+  scan_hist_body<ITEMS>(hist, stride, desc, n_host, totals);
+}
+
+template <int ITEMS>
+STM_DEV void scatter_body(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
     const uint8_t* __restrict__ ops_in,
     const BatchDesc* __restrict__ desc, uint32_t n_host, int first_pass, int shift,
@@ -336,7 +364,10 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
         const uint64_t kk = s_stage[p];
         const uint32_t d = digit_of(kk, shift);
         dg[r / 4] |= d << (8 * (r % 4));
-        keys_out[s_gbase[d] + (p - s_dbase[d])] = kk;
+        // In bounds whenever the histograms are complete; the check keeps a grid barrier of
+        // k_sort_exact that gave up waiting from writing outside the output.
+        const uint32_t dst = s_gbase[d] + (p - s_dbase[d]);
+        if (dst < rg.n) keys_out[dst] = kk;
       }
     }
     __syncthreads();
@@ -349,11 +380,23 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
       const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
       if (p < tile_n) {
         const uint32_t d = (dg[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
-        idx_out[s_gbase[d] + (p - s_dbase[d])] = s_idx[p];
+        const uint32_t dst = s_gbase[d] + (p - s_dbase[d]);
+        if (dst < rg.n) idx_out[dst] = s_idx[p];
       }
     }
     __syncthreads();
   }
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
+    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
+    const uint8_t* __restrict__ ops_in,
+    const BatchDesc* __restrict__ desc, uint32_t n_host, int first_pass, int shift,
+    const uint32_t* __restrict__ hist, uint32_t stride, const uint32_t* __restrict__ totals,
+    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out) {
+  scatter_body<ITEMS>(keys_in, idx_in, ops_in, desc, n_host, first_pass, shift, hist, stride,
+                      totals, keys_out, idx_out);
 }
 
 // Histograms of every digit position in one read of the keys.  Workgroup w counts a
@@ -653,6 +696,424 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
   }
 }
 
+// ================================================================ normalized-key form
+// Batches above the single-sweep bound sort an order-preserving NB-bit image of each key
+// (NB = 8 * npass, 24 or 32 bits):
+//   nk = (key - kmin) >> shift,  shift = max(0, bitlen(kmax - kmin) - NB)
+// by npass stable LSD passes over 8-byte records (nk, payload), half the bytes of a
+// (u64 key, payload) pass and three or four passes instead of eight.  Equal nk form buckets of
+// keys within 2^shift of each other, in input order.  k_fix then puts every bucket of at most
+// FIX_SPAN elements in key order (rank by key, ties by position: stable) and writes the sorted
+// u64 keys with their payloads.  A longer bucket keeps input order, which is already the
+// sorted order when all its keys are equal (the keys of a hot key); if one holds distinct keys
+// k_fix raises CTL_NEED_EXACT and k_sort_exact, one persistent launch that otherwise returns at
+// once, re-sorts the whole batch with the eight exact 8-bit passes.
+constexpr uint32_t FIX_TILE = 2048;  // sorted positions per k_fix step
+constexpr uint32_t FIX_SPAN = 64;    // longest bucket k_fix orders itself
+constexpr uint32_t FIX_WIN = FIX_TILE + 2 * FIX_SPAN;
+constexpr uint32_t BARRIER_SPIN_LIMIT = 1u << 20;  // polls (most sleep ~2k cycles)
+
+STM_DEV uint32_t bitlen64(uint64_t x) { return x ? 64u - (uint32_t)__clzll(x) : 0u; }
+
+struct NormKey {
+  uint64_t kmin;
+  uint32_t shift;
+};
+
+// Every workgroup reduces the RANGE_BLOCKS partials of k_range itself (one per thread).
+STM_DEV NormKey norm_params(const uint32_t* words, int npass) {
+  static_assert(SORT_BLOCK == RANGE_BLOCKS, "one partial per thread");
+  __shared__ uint64_t s_mm[2][SORT_WAVES];
+  const uint64_t* range = reinterpret_cast<const uint64_t*>(words + RANGE_OFF);
+  uint64_t mn = range[2 * threadIdx.x], mx = range[2 * threadIdx.x + 1];
+#pragma unroll
+  for (int off = WAVE / 2; off >= 1; off >>= 1) {
+    const uint64_t a = shfl64(mn, lane_id() ^ off), b = shfl64(mx, lane_id() ^ off);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  const int wave = threadIdx.x / WAVE;
+  if (lane_id() == 0) {
+    s_mm[0][wave] = mn;
+    s_mm[1][wave] = mx;
+  }
+  __syncthreads();
+  mn = s_mm[0][0];
+  mx = s_mm[1][0];
+#pragma unroll
+  for (int w = 1; w < SORT_WAVES; ++w) {
+    mn = s_mm[0][w] < mn ? s_mm[0][w] : mn;
+    mx = s_mm[1][w] > mx ? s_mm[1][w] : mx;
+  }
+  __syncthreads();
+  const uint64_t r = mx >= mn ? mx - mn : 0ull;
+  const uint32_t bl = bitlen64(r);
+  NormKey k;
+  k.kmin = mx >= mn ? mn : 0ull;
+  k.shift = bl > (uint32_t)(8 * npass) ? bl - (uint32_t)(8 * npass) : 0u;
+  return k;
+}
+
+STM_DEV uint32_t norm_of(uint64_t key, const NormKey& k) {
+  return (uint32_t)((key - k.kmin) >> k.shift);
+}
+
+// Minimum and maximum of the batch's keys, one partial per workgroup.  Workgroup 0 also opens
+// the sort: the stall word, the exact-sort request and the barrier counter start at zero.
+__global__ __launch_bounds__(SORT_BLOCK) void k_range(const uint64_t* __restrict__ keys,
+                                                      const BatchDesc* __restrict__ desc,
+                                                      uint32_t n_host, uint32_t* __restrict__ words) {
+  __shared__ uint64_t s_mm[2][SORT_WAVES];
+  const SortRange rg = sort_range(desc, n_host, keys, 1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    words[CTL_STALL] = 0;
+    words[CTL_NEED_EXACT] = 0;
+    words[CTL_BARRIER] = 0;
+  }
+  uint64_t mn = ~0ull, mx = 0ull;
+  const uint64_t stride = (uint64_t)gridDim.x * SORT_BLOCK;
+  uint64_t i = (uint64_t)blockIdx.x * SORT_BLOCK + threadIdx.x;
+  for (; i + 3 * stride < rg.n; i += 4 * stride) {
+    uint64_t k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) k[u] = gload(rg.keys + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      mn = k[u] < mn ? k[u] : mn;
+      mx = k[u] > mx ? k[u] : mx;
+    }
+  }
+  for (; i < rg.n; i += stride) {
+    const uint64_t k = gload(rg.keys + i);
+    mn = k < mn ? k : mn;
+    mx = k > mx ? k : mx;
+  }
+#pragma unroll
+  for (int off = WAVE / 2; off >= 1; off >>= 1) {
+    const uint64_t a = shfl64(mn, lane_id() ^ off), b = shfl64(mx, lane_id() ^ off);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  const int wave = threadIdx.x / WAVE;
+  if (lane_id() == 0) {
+    s_mm[0][wave] = mn;
+    s_mm[1][wave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < SORT_WAVES; ++w) {
+      mn = s_mm[0][w] < mn ? s_mm[0][w] : mn;
+      mx = s_mm[1][w] > mx ? s_mm[1][w] : mx;
+    }
+    uint64_t* range = reinterpret_cast<uint64_t*>(words + RANGE_OFF);
+    range[2 * blockIdx.x] = mn;
+    range[2 * blockIdx.x + 1] = mx;
+  }
+}
+
+// Tile digit histograms of one normalized pass.  FIRST: the records are computed from the
+// batch's keys; otherwise nk_in holds them.
+template <int ITEMS, bool FIRST>
+__global__ __launch_bounds__(SORT_BLOCK) void k_nhist(const uint64_t* __restrict__ keys,
+                                                      const BatchDesc* __restrict__ desc,
+                                                      uint32_t n_host, const uint32_t* __restrict__ nk_in,
+                                                      const uint32_t* __restrict__ words, int npass,
+                                                      int shift, uint32_t* __restrict__ hist,
+                                                      uint32_t stride) {
+  constexpr int TILE = SORT_BLOCK * ITEMS;
+  __shared__ uint32_t s_h[SORT_WAVES][RADIX];
+  const SortRange rg = sort_range(desc, n_host, keys, 1);
+  NormKey nkp{0, 0};
+  if constexpr (FIRST) nkp = norm_params(words, npass);
+  const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
+  const int wave = threadIdx.x / WAVE;
+  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_h[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)b * TILE;
+#pragma unroll 4
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint64_t e = base + (uint64_t)r * SORT_BLOCK + threadIdx.x;
+      if (e < rg.n) {
+        const uint32_t v = FIRST ? norm_of(gload(rg.keys + e), nkp) : gload(nk_in + e);
+        atomicAdd(&s_h[wave][(v >> shift) & (RADIX - 1)], 1u);
+      }
+    }
+    __syncthreads();
+    const int d = threadIdx.x;  // SORT_BLOCK == RADIX
+    hist[(uint64_t)d * stride + b] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
+    __syncthreads();
+  }
+}
+
+// One normalized pass: stable in-tile ranking (as k_scatter), records staged in LDS in digit
+// order (nk, then payload: 4 bytes per element of LDS), coalesced runs out.  FIRST: nk from the
+// batch's keys, payload = position | op code << POS_BITS.
+template <int ITEMS, bool FIRST>
+__global__ __launch_bounds__(SORT_BLOCK) void k_nscatter(
+    const uint64_t* __restrict__ keys, const uint8_t* __restrict__ ops_in,
+    const BatchDesc* __restrict__ desc, uint32_t n_host,
+    const uint32_t* __restrict__ nk_in, const uint32_t* __restrict__ pl_in,
+    const uint32_t* __restrict__ words, int npass, int shift,
+    const uint32_t* __restrict__ hist, uint32_t stride, const uint32_t* __restrict__ totals,
+    uint32_t* __restrict__ nk_out, uint32_t* __restrict__ pl_out) {
+  constexpr int TILE = SORT_BLOCK * ITEMS;
+  constexpr int SEG = TILE / SORT_WAVES;
+  __shared__ uint32_t s_wcnt[SORT_WAVES][RADIX];
+  __shared__ uint32_t s_dbase[RADIX];
+  __shared__ uint32_t s_tbase[RADIX];
+  __shared__ uint32_t s_gbase[RADIX];
+  __shared__ uint32_t s_tmp[SORT_WAVES];
+  __shared__ uint32_t s_stage[TILE];
+
+  const SortRange rg = sort_range(desc, n_host, keys, 1);
+  const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
+  const uint8_t* ops = desc ? desc->ops : ops_in;
+  NormKey nkp{0, 0};
+  if constexpr (FIRST) nkp = norm_params(words, npass);
+  const int lane = lane_id();
+  const int wave = threadIdx.x / WAVE;
+  {
+    uint32_t tot;
+    s_tbase[threadIdx.x] = block_exclusive_scan_256(totals[threadIdx.x], s_tmp, tot);
+  }
+  const uint64_t lt = lanemask_lt();
+  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
+    const uint64_t tile0 = (uint64_t)b * TILE;
+    const uint32_t tile_n = (uint32_t)min<uint64_t>(TILE, rg.n - tile0);
+    for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wcnt[0][0])[i] = 0;
+    s_gbase[threadIdx.x] = s_tbase[threadIdx.x] + hist[(uint64_t)threadIdx.x * stride + b];
+    __syncthreads();
+    uint32_t k[ITEMS];
+    uint32_t rk[ITEMS];
+    uint32_t lane0 = (uint32_t)wave * SEG + lane;
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(lane0), "+v"(tid));
+    const uint32_t last = tile_n - 1;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t le = min(lane0 + (uint32_t)r * WAVE, last);
+      if constexpr (FIRST) k[r] = norm_of(gload(rg.keys + tile0 + le), nkp);
+      else k[r] = gload(nk_in + tile0 + le);
+    }
+    asm volatile("" : "+v"(lane0));
+    uint32_t dk[(ITEMS + 3) / 4];
+#pragma unroll
+    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dk[q] = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
+      const uint32_t d = (k[r] >> shift) & (RADIX - 1);
+      dk[r / 4] |= d << (8 * (r % 4));
+      uint64_t peers = ballot(valid);
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        const bool on = (d >> bit) & 1u;
+        const uint64_t bb = ballot(on);
+        peers &= on ? bb : ~bb;
+      }
+      uint32_t base = valid ? s_wcnt[wave][d] : 0u;
+      rk[r] = base + (uint32_t)__popcll(peers & lt);
+      asm volatile("" : "+v"(rk[r]));
+      const bool leader = valid && (lane == 63 - __clzll(peers));
+      if (leader) s_wcnt[wave][d] = base + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+      const int d = threadIdx.x;
+      uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
+      uint32_t tot;
+      s_dbase[d] = block_exclusive_scan_256(c0 + c1 + c2 + c3, s_tmp, tot);
+      s_wcnt[0][d] = 0;
+      s_wcnt[1][d] = c0;
+      s_wcnt[2][d] = c0 + c1;
+      s_wcnt[3][d] = c0 + c1 + c2;
+    }
+    __syncthreads();
+    asm volatile("" : "+v"(lane0));
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      if (lane0 + (uint32_t)r * WAVE < tile_n) {
+        const uint32_t d = (dk[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
+        rk[r] += s_dbase[d] + s_wcnt[wave][d];
+        s_stage[rk[r]] = k[r];
+      }
+    }
+    asm volatile("" : "+v"(lane0));
+    uint32_t id[ITEMS];
+    if constexpr (FIRST) {
+      if (ops) {
+        const uint8_t* ot = ops + tile0;
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r) {
+          const uint32_t le = lane0 + (uint32_t)r * WAVE;
+          const uint32_t o = gload(ot + min(le, last));
+          id[r] = ((uint32_t)tile0 + le) | ((o > 3u ? 1u : o) << POS_BITS);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < ITEMS; ++r) id[r] = (uint32_t)tile0 + lane0 + (uint32_t)r * WAVE;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < ITEMS; ++r) id[r] = gload(pl_in + tile0 + min(lane0 + (uint32_t)r * WAVE, last));
+    }
+    __syncthreads();
+    uint32_t dg[(ITEMS + 3) / 4];
+#pragma unroll
+    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dg[q] = 0;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
+      if (p < tile_n) {
+        const uint32_t v = s_stage[p];
+        const uint32_t d = (v >> shift) & (RADIX - 1);
+        dg[r / 4] |= d << (8 * (r % 4));
+        const uint32_t dst = s_gbase[d] + (p - s_dbase[d]);
+        if (dst < rg.n) nk_out[dst] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r)
+      if (lane0 + (uint32_t)r * WAVE < tile_n) s_stage[rk[r]] = id[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
+      if (p < tile_n) {
+        const uint32_t d = (dg[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
+        const uint32_t dst = s_gbase[d] + (p - s_dbase[d]);
+        if (dst < rg.n) pl_out[dst] = s_stage[p];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Buckets (runs of equal nk) into key order.  A step covers FIX_TILE sorted positions with
+// FIX_SPAN positions of context on each side in LDS; the batch's key of every position is
+// gathered through its payload.  Position i's bucket is found by scanning at most FIX_SPAN
+// positions each way: a bucket of at most FIX_SPAN elements lies wholly in the window, and its
+// element i goes to bucket start + its rank by (key, position).  Elements of longer buckets stay
+// where they are; a longer bucket with two different adjacent keys raises CTL_NEED_EXACT.
+__global__ __launch_bounds__(SORT_BLOCK) void k_fix(const uint32_t* __restrict__ nk,
+                                                    const uint32_t* __restrict__ pl,
+                                                    const uint64_t* __restrict__ keys_in,
+                                                    const BatchDesc* __restrict__ desc,
+                                                    uint32_t n_host, uint64_t* __restrict__ keys_out,
+                                                    uint32_t* __restrict__ idx_out,
+                                                    uint32_t* __restrict__ words) {
+  __shared__ uint32_t s_nk[FIX_WIN];
+  __shared__ uint32_t s_pl[FIX_WIN];
+  __shared__ uint64_t s_key[FIX_WIN];
+  const SortRange rg = sort_range(desc, n_host, keys_in, 1);
+  const uint32_t n = rg.n;
+  bool need = false;
+  for (uint32_t t = blockIdx.x; (uint64_t)t * FIX_TILE < n; t += gridDim.x) {
+    const uint32_t t0 = t * FIX_TILE, t1 = min(n, t0 + FIX_TILE);
+    const uint32_t w0 = t0 > FIX_SPAN ? t0 - FIX_SPAN : 0u;
+    const uint32_t w1 = min(n, t1 + FIX_SPAN);
+    for (uint32_t i = w0 + threadIdx.x; i < w1; i += SORT_BLOCK) {
+      const uint32_t x = i - w0;
+      const uint32_t p = gload(pl + i);
+      s_nk[x] = gload(nk + i);
+      s_pl[x] = p;
+      s_key[x] = gload(rg.keys + pos_of(p));
+    }
+    __syncthreads();
+    const uint32_t wn = w1 - w0;
+    for (uint32_t i = t0 + threadIdx.x; i < t1; i += SORT_BLOCK) {
+      const uint32_t x = i - w0;
+      const uint32_t v = s_nk[x];
+      uint32_t l = x;
+      while (l > 0 && x - l < FIX_SPAN && s_nk[l - 1] == v) --l;
+      uint32_t r = x + 1;
+      while (r < wn && r - x <= FIX_SPAN && s_nk[r] == v) ++r;
+      const uint64_t key = s_key[x];
+      if (x - l >= FIX_SPAN || r - x > FIX_SPAN || r - l > FIX_SPAN) {
+        // A long bucket: keep input order; it is sorted already unless its keys differ.
+        keys_out[i] = key;
+        idx_out[i] = s_pl[x];
+        if (x > 0 && s_nk[x - 1] == v && s_key[x - 1] != key) need = true;
+        continue;
+      }
+      uint32_t rank = 0;
+      for (uint32_t j = l; j < r; ++j) {
+        const uint64_t kj = s_key[j];
+        rank += (kj < key || (kj == key && j < x)) ? 1u : 0u;
+      }
+      const uint32_t dst = w0 + l + rank;
+      keys_out[dst] = key;
+      idx_out[dst] = s_pl[x];
+    }
+    __syncthreads();
+  }
+  if (need) words[CTL_NEED_EXACT] = 1;
+}
+
+// Grid barrier of k_sort_exact: every workgroup of the (co-resident) grid arrives, then waits
+// until all have.  Each wave first waits for its own stores to be acknowledged by the L2; then
+// one lane per workgroup releases at agent scope (the XCD's L2 is written back, so the data is
+// visible to the other XCDs), arrives, polls with growing sleeps (fewer pollers contending on
+// the counter) and acquires (stale lines dropped) before the workgroup goes on.  A wait beyond
+// BARRIER_SPIN_LIMIT polls raises the stall word and returns: the batch is then reported failed
+// instead of the device hanging.
+STM_DEV void grid_barrier(uint32_t* words, uint32_t target) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(words + CTL_BARRIER, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while (__hip_atomic_load(words + CTL_BARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins >= BARRIER_SPIN_LIMIT) {
+        __hip_atomic_store(words + CTL_STALL, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      if (spins < 8) __builtin_amdgcn_s_sleep(2);
+      else if (spins < 64) __builtin_amdgcn_s_sleep(8);
+      else __builtin_amdgcn_s_sleep(32);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// The exact sort, run only when k_fix found a long bucket of distinct keys: the eight 8-bit
+// passes of the three-kernel form (histograms, scan, scatter), separated by grid barriers
+// instead of launches, from the batch's input to keys_out / idx_out.  The grid must fit on the
+// device at once (the launcher sizes it from the occupancy the kernel allows).
+template <int ITEMS>
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_exact(
+    const uint64_t* __restrict__ keys_in, const uint8_t* __restrict__ ops_in,
+    const BatchDesc* __restrict__ desc, uint32_t n_host, uint64_t* __restrict__ keys_tmp,
+    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_tmp, uint32_t* __restrict__ idx_out,
+    uint32_t* __restrict__ words, uint32_t stride) {
+  if (__hip_atomic_load(words + CTL_NEED_EXACT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    return;
+  uint32_t* hist = words + ONESWEEP_WORDS;
+  uint32_t* totals = hist + (size_t)stride * RADIX;
+  const uint64_t* kin = keys_in;
+  const uint32_t* iin = nullptr;
+  uint32_t arrivals = 0;
+  for (int p = 0; p < MAX_PASSES; ++p) {
+    const bool to_out = ((MAX_PASSES - 1 - p) % 2) == 0;
+    uint64_t* kout = to_out ? keys_out : keys_tmp;
+    uint32_t* iout = to_out ? idx_out : idx_tmp;
+    const int first = p == 0 ? 1 : 0;
+    hist_body<ITEMS>(kin, desc, n_host, first, 8 * p, hist, stride);
+    grid_barrier(words, (++arrivals) * gridDim.x);
+    scan_hist_body<ITEMS>(hist, stride, desc, n_host, totals);
+    grid_barrier(words, (++arrivals) * gridDim.x);
+    scatter_body<ITEMS>(kin, iin, first ? ops_in : nullptr, desc, n_host, first, 8 * p, hist,
+                        stride, totals, kout, iout);
+    grid_barrier(words, (++arrivals) * gridDim.x);
+    kin = kout;
+    iin = iout;
+  }
+}
+
 __global__ void k_iota_copy(const uint64_t* __restrict__ keys_in, const BatchDesc* __restrict__ desc,
                             uint32_t n_host, uint64_t* __restrict__ keys_out,
                             uint32_t* __restrict__ idx_out) {
@@ -680,12 +1141,89 @@ const uint32_t* sort_stall_word(const uint32_t* words) { return words + CTL_STAL
 
 uint32_t* sort_spare_words(uint32_t* words) { return words + ONESWEEP_WORDS; }
 
+// Workgroups of k_sort_exact<ITEMS> the device holds at once (its grid barriers need them all
+// resident), at most SORT_GRID_CAP.
+template <int ITEMS>
+static hipError_t exact_grid(uint32_t& grid) {
+  static std::atomic<uint32_t> resident{0};
+  if (resident.load(std::memory_order_relaxed) == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sort_exact<ITEMS>, SORT_BLOCK, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1 || cus < 1) return hipErrorInvalidConfiguration;
+    // Two workgroups per CU at most: the phases are memory-bound, and each further workgroup
+    // adds a poller to every grid barrier.
+    resident.store((uint32_t)std::min<int64_t>((int64_t)cus * std::min(per_cu, 2), SORT_GRID_CAP),
+                   std::memory_order_relaxed);
+  }
+  grid = resident.load(std::memory_order_relaxed);
+  return hipSuccess;
+}
+
+// Batches above the single-sweep bound: k_range, npass normalized passes (k_nhist,
+// k_scan_hist, k_nscatter), k_fix, k_sort_exact (returns at once unless k_fix asked for it).
+// Records (nk, payload) ping-pong between keys_tmp and keys_out, each viewed as two u32 arrays
+// of n_max; the last pass lands in keys_tmp so k_fix can write keys_out.
+template <int ITEMS>
+static hipError_t sort_normalized(const uint64_t* keys_in, const uint8_t* ops_in,
+                                  const BatchDesc* desc, uint32_t n_host, uint32_t n_max,
+                                  uint64_t* keys_tmp, uint64_t* keys_out, uint32_t* idx_tmp,
+                                  uint32_t* idx_out, uint32_t* words, hipStream_t stream) {
+  const uint32_t tile = sort_tile(ITEMS);
+  const uint32_t stride = (n_max + tile - 1) / tile;
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(stride, SORT_GRID_CAP));
+  const int npass = n_max > (1u << 23) ? 4 : 3;
+  uint32_t* hist = words + ONESWEEP_WORDS;
+  uint32_t* totals = hist + (size_t)stride * RADIX;
+  uint32_t* rec_a = reinterpret_cast<uint32_t*>(keys_tmp);
+  uint32_t* rec_b = reinterpret_cast<uint32_t*>(keys_out);
+  uint32_t xgrid = 0;
+  hipError_t e = exact_grid<ITEMS>(xgrid);
+  if (e != hipSuccess) return e;
+  k_range<<<RANGE_BLOCKS, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, words);
+  const uint32_t *nk_in = nullptr, *pl_in = nullptr;
+  for (int p = 0; p < npass; ++p) {
+    uint32_t* out = ((npass - 1 - p) % 2) == 0 ? rec_a : rec_b;
+    uint32_t* nk_out = out;
+    uint32_t* pl_out = out + n_max;
+    if (p == 0) {
+      k_nhist<ITEMS, true><<<grid, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, nullptr, words,
+                                                            npass, 0, hist, stride);
+      k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
+      k_nscatter<ITEMS, true><<<grid, SORT_BLOCK, 0, stream>>>(
+          keys_in, ops_in, desc, n_host, nullptr, nullptr, words, npass, 0, hist, stride, totals,
+          nk_out, pl_out);
+    } else {
+      k_nhist<ITEMS, false><<<grid, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, nk_in, words,
+                                                             npass, 8 * p, hist, stride);
+      k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
+      k_nscatter<ITEMS, false><<<grid, SORT_BLOCK, 0, stream>>>(
+          keys_in, ops_in, desc, n_host, nk_in, pl_in, words, npass, 8 * p, hist, stride, totals,
+          nk_out, pl_out);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    nk_in = nk_out;
+    pl_in = pl_out;
+  }
+  const uint32_t fgrid = std::max<uint32_t>(1, std::min<uint32_t>((n_max + FIX_TILE - 1) / FIX_TILE,
+                                                                  SORT_GRID_CAP));
+  k_fix<<<fgrid, SORT_BLOCK, 0, stream>>>(nk_in, pl_in, keys_in, desc, n_host, keys_out, idx_out,
+                                          words);
+  k_sort_exact<ITEMS><<<xgrid, SORT_BLOCK, 0, stream>>>(keys_in, ops_in, desc, n_host, keys_tmp,
+                                                       keys_out, idx_tmp, idx_out, words, stride);
+  return hipGetLastError();
+}
+
 template <int ITEMS>
 static hipError_t sort_passes(const uint64_t* keys_in, const uint8_t* ops_in, const BatchDesc* desc,
                               uint32_t n_host,
                               uint32_t n_max, uint64_t* keys_tmp, uint64_t* keys_out,
                               uint32_t* idx_tmp, uint32_t* idx_out, int begin_bit, int end_bit,
-                              uint32_t* words, hipStream_t stream) {
+                              uint32_t* words, bool exact, hipStream_t stream) {
   const uint32_t tile = sort_tile(ITEMS);
   const uint32_t stride = (n_max + tile - 1) / tile;  // tiles of the largest batch
   const int passes = (end_bit - begin_bit + 7) / 8;
@@ -723,10 +1261,13 @@ static hipError_t sort_passes(const uint64_t* keys_in, const uint8_t* ops_in, co
     }
     return hipSuccess;
   }
+  if (!exact && begin_bit == 0 && end_bit == 64)
+    return sort_normalized<ITEMS>(keys_in, ops_in, desc, n_host, n_max, keys_tmp, keys_out,
+                                  idx_tmp, idx_out, words, stream);
+  // The exact form: eight 8-bit passes of three kernels each.
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(stride, SORT_GRID_CAP));
   uint32_t* hist = words + ONESWEEP_WORDS;
   uint32_t* totals = hist + (size_t)stride * RADIX;
-  uint32_t* stall = words + CTL_STALL;
   for (int p = 0; p < passes; ++p) {
     const int shift = begin_bit + 8 * p;
     const bool to_out = ((passes - 1 - p) % 2) == 0;
@@ -734,7 +1275,7 @@ static hipError_t sort_passes(const uint64_t* keys_in, const uint8_t* ops_in, co
     uint32_t* iout = to_out ? idx_out : idx_tmp;
     const int first = p == 0 ? 1 : 0;
     k_hist<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, desc, n_host, first, shift, hist, stride,
-                                                   stall);
+                                                   words);
     k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
     k_scatter<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, first ? ops_in : nullptr, desc,
                                                       n_host, first, shift, hist, stride,
@@ -751,14 +1292,14 @@ hipError_t radix_sort_pairs_desc(const uint64_t* keys_in, const uint8_t* ops_in,
                                  const BatchDesc* desc, uint32_t n_max,
                                  uint64_t* keys_tmp, uint64_t* keys_out, uint32_t* idx_tmp,
                                  uint32_t* idx_out, int begin_bit, int end_bit, uint32_t* hist_words,
-                                 int tile_items, hipStream_t stream) {
+                                 int tile_items, bool exact, hipStream_t stream) {
   if (n_max == 0) return hipSuccess;
   const uint32_t n_host = desc ? 0u : n_max;
   if (tile_items == 8)
     return sort_passes<8>(keys_in, ops_in, desc, n_host, n_max, keys_tmp, keys_out, idx_tmp,
-                          idx_out, begin_bit, end_bit, hist_words, stream);
+                          idx_out, begin_bit, end_bit, hist_words, exact, stream);
   return sort_passes<16>(keys_in, ops_in, desc, n_host, n_max, keys_tmp, keys_out, idx_tmp,
-                         idx_out, begin_bit, end_bit, hist_words, stream);
+                         idx_out, begin_bit, end_bit, hist_words, exact, stream);
 }
 
 hipError_t radix_sort_pairs(const uint64_t* keys_in, uint64_t* keys_tmp, uint64_t* keys_out,
@@ -766,7 +1307,9 @@ hipError_t radix_sort_pairs(const uint64_t* keys_in, uint64_t* keys_tmp, uint64_
                             int end_bit, uint32_t* hist_words, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   return sort_passes<16>(keys_in, nullptr, nullptr, n, n, keys_tmp, keys_out, idx_tmp, idx_out,
-                         begin_bit, end_bit, hist_words, stream);
+                         begin_bit, end_bit, hist_words, false, stream);
 }
+
+const uint32_t* sort_exact_word(const uint32_t* words) { return words + CTL_NEED_EXACT; }
 
 }  // namespace stm

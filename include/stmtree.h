@@ -112,6 +112,9 @@ typedef struct stm_stats {
     uint64_t phase_launches[STM_NUM_PHASES];
     uint64_t node_reads;      /* tree nodes read by lookups and transactions (the reference's
                                  ABORT_COUNT traversal steps, src/gpu-stm-tree-v38/bpt-gpustm.cpp:142-160) */
+    uint64_t sort_fallbacks;  /* batches above 2^18 ops whose sort of normalized keys met a long
+                                 run of close, distinct keys (hot keys with neighbours) and re-sorted
+                                 with the exact passes; the next 16 batches then sort exactly */
 } stm_stats;
 
 typedef struct stm_tree stm_tree;
@@ -210,6 +213,14 @@ int stm_run_batches_device(stm_tree* tree, const uint64_t* d_keys, const uint64_
  * inside its stream.  No reference counterpart (the reference launches every kernel from the
  * host, src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:264-332). */
 int stm_capture_graphs(stm_tree* tree);
+
+/* Stable ascending sort of n device-resident u64 keys on `device`: d_keys_out receives the
+ * keys in order, d_idx_out the input position of each (ties keep input order).  The batch
+ * path's own sort, exposed for callers and tests; it allocates its scratch per call.
+ * Replaces cub::DeviceRadixSort::SortPairs of (key, position)
+ * (src/gpu-stm-tree-v38/bpt-gpustm-kernel.cu:293). */
+int stm_sort_pairs_device(int device, const uint64_t* d_keys, size_t n, uint64_t* d_keys_out,
+                          uint32_t* d_idx_out);
 
 /* Number of keys in the tree. */
 int stm_size(stm_tree* tree, uint64_t* n_keys);

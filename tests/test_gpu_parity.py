@@ -233,6 +233,56 @@ def test_hot_get_groups_fan_out(oracle_mod, graphs):
         assert_dump(t, o)
 
 
+def test_sort_fallback_then_exact_batches(oracle_mod):
+    """Batches above 2^18 ops sort normalized keys; a hot key with many close, distinct
+    neighbours leaves a long bucket of distinct keys, so that batch re-sorts with the exact
+    passes (sort_fallbacks = 1) and the following batches sort exactly from the start (no new
+    fallback), through the synchronous call and through replayed graphs (recaptured in the
+    exact form).  Every answer and the dump against the oracle."""
+    from eirene_amd import DeviceBuffer
+    rng = np.random.default_rng(83)
+    load = rng.integers(0, 1 << 63, size=100000, dtype=np.uint64)
+    lv = rng.integers(0, 1 << 63, size=load.size, dtype=np.uint64)
+    hot = load[7]
+
+    def hot_batch(n):
+        k = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+        m = n // 4
+        k[:m] = hot
+        k[m: 2 * m] = hot + rng.integers(1, 1 << 30, m).astype(np.uint64)
+        rng.shuffle(k)
+        v = rng.integers(0, 1 << 63, size=n, dtype=np.uint64)
+        o = (rng.random(n) < 0.6).astype(np.uint8)
+        return k, v, o
+
+    o = oracle_mod.Oracle()
+    o.bulk_load(load, lv)
+    with tree(max_keys=1 << 22, max_batch=1 << 19) as t:
+        t.bulk_load(load, lv)
+        for b in range(2):
+            keys, vals, ops = hot_batch(300000)
+            assert_same(t.run_batch(keys, vals, ops), o.run_batch(keys, vals, ops), f"batch {b}")
+            assert t.stats()["sort_fallbacks"] == 1  # the second batch sorted exactly at once
+        parts = [hot_batch(m) for m in (300000, 270000)]
+        keys = np.concatenate([p_[0] for p_ in parts])
+        vals = np.concatenate([p_[1] for p_ in parts])
+        ops = np.concatenate([p_[2] for p_ in parts])
+        offsets = np.array([0, 300000, 570000], dtype=np.uint64)
+        want_v, want_f = [], []
+        for b in range(2):
+            a, z = int(offsets[b]), int(offsets[b + 1])
+            wv, wf = o.run_batch(keys[a:z], vals[a:z], ops[a:z])
+            want_v.append(wv)
+            want_f.append(wf)
+        dk, dv, do = (DeviceBuffer.from_array(x) for x in (keys, vals, ops))
+        ov, of = DeviceBuffer(keys.size, np.uint64), DeviceBuffer(keys.size, np.uint8)
+        t.run_batches_device(dk, dv, do, ov, of, offsets)
+        assert_same((ov.download(), of.download()), (np.concatenate(want_v), np.concatenate(want_f)),
+                    "stream")
+        assert t.stats()["sort_fallbacks"] == 1
+        assert_dump(t, o)
+
+
 def test_plain_uniform_reduced(oracle_mod):
     from eirene_amd import workloads as w
     load = w.uniform_keys(1 << 20, 1)
