@@ -80,7 +80,9 @@ constexpr uint32_t CTL_WRAPPED = CTL_EPOCH + 32;
 constexpr uint32_t CTL_STALL = CTL_WRAPPED + 32;
 constexpr uint32_t CTL_NEED_EXACT = CTL_STALL + 32;  // normalized sort: a long bucket holds distinct keys
 constexpr uint32_t CTL_BARRIER = CTL_NEED_EXACT + 32;  // grid-barrier arrivals of k_sort_exact
-constexpr uint32_t GOFF_OFF = (CTL_BARRIER + 32 + 63) / 64 * 64;    // goff[MAX_PASSES][RADIX]
+constexpr uint32_t CTL_RANGE_DONE = CTL_BARRIER + 32;  // k_range workgroups finished (mod RANGE_BLOCKS)
+constexpr uint32_t CTL_NORM = CTL_RANGE_DONE + 32;     // u64 kmin, then u32 nshift (k_range's last workgroup)
+constexpr uint32_t GOFF_OFF = (CTL_NORM + 32 + 63) / 64 * 64;       // goff[MAX_PASSES][RADIX]
 constexpr uint32_t PART_OFF = GOFF_OFF + MAX_PASSES * RADIX;        // [HIST_GRID_CAP][MAX_PASSES][RADIX]
 constexpr uint32_t STATUS_OFF = PART_OFF + HIST_GRID_CAP * MAX_PASSES * RADIX;  // u64 [OS_MAX_TILES][RADIX]
 constexpr uint32_t RANGE_BLOCKS = 256;       // k_range workgroups: one (min, max) partial each
@@ -120,6 +122,30 @@ STM_DEV uint32_t digit_of(uint64_t k, int shift) {
   return w & (RADIX - 1);
 }
 
+constexpr uint32_t FIX_TILE = 2048;  // sorted positions per k_fix step
+constexpr uint32_t FIX_SPAN = 64;    // longest bucket k_fix orders itself
+constexpr uint32_t FIX_WIN = FIX_TILE + 2 * FIX_SPAN;
+constexpr uint32_t BARRIER_SPIN_LIMIT = 1u << 20;  // polls (most sleep ~2k cycles)
+
+STM_DEV uint32_t bitlen64(uint64_t x) { return x ? 64u - (uint32_t)__clzll(x) : 0u; }
+
+struct NormKey {
+  uint64_t kmin;
+  uint32_t shift;
+};
+
+// The normalization k_range's last workgroup stored for this sort.
+STM_DEV NormKey norm_params(const uint32_t* words) {
+  NormKey k;
+  k.kmin = *reinterpret_cast<const uint64_t*>(words + CTL_NORM);
+  k.shift = words[CTL_NORM + 2];
+  return k;
+}
+
+STM_DEV uint32_t norm_of(uint64_t key, const NormKey& k) {
+  return (uint32_t)((key - k.kmin) >> k.shift);
+}
+
 struct SortRange {
   uint32_t n;
   const uint64_t* keys;
@@ -137,13 +163,21 @@ STM_DEV SortRange sort_range(const BatchDesc* d, uint32_t n_host, const uint64_t
 
 // The three kernels of one pass are device functions too: k_sort_exact runs them in turn
 // inside one persistent launch.
+// norm: the pass sorts normalized keys (key - kmin) >> nshift from k_range's partials, digit
+// `shift` of them; without it the key's own digit.
 template <int ITEMS>
 STM_DEV void hist_body(const uint64_t* __restrict__ keys, const BatchDesc* __restrict__ desc,
                        uint32_t n_host, int first_pass, int shift, uint32_t* __restrict__ hist,
-                       uint32_t stride) {
+                       uint32_t stride, const uint32_t* __restrict__ norm, int npass) {
   constexpr int TILE = SORT_BLOCK * ITEMS;
   __shared__ uint32_t s_h[SORT_WAVES][RADIX];
   const SortRange rg = sort_range(desc, n_host, keys, first_pass);
+  uint64_t kmin = 0;
+  if (norm) {
+    const NormKey nk = norm_params(norm);
+    kmin = nk.kmin;
+    shift += (int)nk.shift;
+  }
   const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
   const int wave = threadIdx.x / WAVE;
   const uint64_t* kp = rg.keys;
@@ -154,7 +188,7 @@ STM_DEV void hist_body(const uint64_t* __restrict__ keys, const BatchDesc* __res
 #pragma unroll 4
     for (int r = 0; r < ITEMS; ++r) {
       const uint64_t e = base + (uint64_t)r * SORT_BLOCK + threadIdx.x;
-      if (e < rg.n) atomicAdd(&s_h[wave][digit_of(gload(kp + e), shift)], 1u);
+      if (e < rg.n) atomicAdd(&s_h[wave][digit_of(gload(kp + e) - kmin, shift)], 1u);
     }
     __syncthreads();
     const int d = threadIdx.x;  // SORT_BLOCK == RADIX
@@ -168,12 +202,14 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_hist(const uint64_t* __restrict_
                                                      const BatchDesc* __restrict__ desc,
                                                      uint32_t n_host, int first_pass, int shift,
                                                      uint32_t* __restrict__ hist, uint32_t stride,
-                                                     uint32_t* __restrict__ words) {
-  if (first_pass && blockIdx.x == 0 && threadIdx.x == 0) {
-    words[CTL_STALL] = 0;  // no look-back in this form
+                                                     uint32_t* __restrict__ words, int norm,
+                                                     int npass) {
+  if (first_pass && !norm && blockIdx.x == 0 && threadIdx.x == 0) {
+    words[CTL_STALL] = 0;  // no look-back in this form (k_range opens a normalized sort)
     words[CTL_NEED_EXACT] = 0;
   }
-  hist_body<ITEMS>(keys, desc, n_host, first_pass, shift, hist, stride);
+  hist_body<ITEMS>(keys, desc, n_host, first_pass, shift, hist, stride, norm ? words : nullptr,
+                   npass);
 }
 
 // Row d of hist (the tiles' counts of digit d) is scanned in place (exclusive) and its total
@@ -235,7 +271,8 @@ STM_DEV void scatter_body(
     const uint8_t* __restrict__ ops_in,
     const BatchDesc* __restrict__ desc, uint32_t n_host, int first_pass, int shift,
     const uint32_t* __restrict__ hist, uint32_t stride, const uint32_t* __restrict__ totals,
-    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out) {
+    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
+    const uint32_t* __restrict__ norm, int npass) {
   constexpr int TILE = SORT_BLOCK * ITEMS;
   constexpr int SEG = TILE / SORT_WAVES;  // contiguous elements owned by one wavefront
   __shared__ uint32_t s_wcnt[SORT_WAVES][RADIX];  // running per-wave digit counts, then wave bases
@@ -249,6 +286,12 @@ STM_DEV void scatter_body(
   uint32_t* s_idx = reinterpret_cast<uint32_t*>(s_stage);
 
   const SortRange rg = sort_range(desc, n_host, keys_in, first_pass);
+  uint64_t kmin = 0;
+  if (norm) {
+    const NormKey nk = norm_params(norm);
+    kmin = nk.kmin;
+    shift += (int)nk.shift;
+  }
   const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
   const uint64_t* kin = rg.keys;
   const uint8_t* ops = (first_pass && desc) ? desc->ops : ops_in;  // read on the first pass only
@@ -292,7 +335,7 @@ STM_DEV void scatter_body(
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
       const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
-      const uint32_t d = digit_of(k[r], shift);
+      const uint32_t d = digit_of(k[r] - kmin, shift);
       dk[r / 4] |= d << (8 * (r % 4));
       // Lanes holding the same digit: intersect 8 bit-ballots.
       uint64_t peers = ballot(valid);
@@ -362,7 +405,7 @@ STM_DEV void scatter_body(
       const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
       if (p < tile_n) {
         const uint64_t kk = s_stage[p];
-        const uint32_t d = digit_of(kk, shift);
+        const uint32_t d = digit_of(kk - kmin, shift);
         dg[r / 4] |= d << (8 * (r % 4));
         // In bounds whenever the histograms are complete; the check keeps a grid barrier of
         // k_sort_exact that gave up waiting from writing outside the output.
@@ -394,9 +437,10 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
     const uint8_t* __restrict__ ops_in,
     const BatchDesc* __restrict__ desc, uint32_t n_host, int first_pass, int shift,
     const uint32_t* __restrict__ hist, uint32_t stride, const uint32_t* __restrict__ totals,
-    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out) {
+    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
+    const uint32_t* __restrict__ norm, int npass) {
   scatter_body<ITEMS>(keys_in, idx_in, ops_in, desc, n_host, first_pass, shift, hist, stride,
-                      totals, keys_out, idx_out);
+                      totals, keys_out, idx_out, norm, npass);
 }
 
 // Histograms of every digit position in one read of the keys.  Workgroup w counts a
@@ -697,73 +741,25 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
 }
 
 // ================================================================ normalized-key form
-// Batches above the single-sweep bound sort an order-preserving NB-bit image of each key
-// (NB = 8 * npass, 24 or 32 bits):
-//   nk = (key - kmin) >> shift,  shift = max(0, bitlen(kmax - kmin) - NB)
-// by npass stable LSD passes over 8-byte records (nk, payload), half the bytes of a
-// (u64 key, payload) pass and three or four passes instead of eight.  Equal nk form buckets of
-// keys within 2^shift of each other, in input order.  k_fix then puts every bucket of at most
-// FIX_SPAN elements in key order (rank by key, ties by position: stable) and writes the sorted
-// u64 keys with their payloads.  A longer bucket keeps input order, which is already the
-// sorted order when all its keys are equal (the keys of a hot key); if one holds distinct keys
-// k_fix raises CTL_NEED_EXACT and k_sort_exact, one persistent launch that otherwise returns at
-// once, re-sorts the whole batch with the eight exact 8-bit passes.
-constexpr uint32_t FIX_TILE = 2048;  // sorted positions per k_fix step
-constexpr uint32_t FIX_SPAN = 64;    // longest bucket k_fix orders itself
-constexpr uint32_t FIX_WIN = FIX_TILE + 2 * FIX_SPAN;
-constexpr uint32_t BARRIER_SPIN_LIMIT = 1u << 20;  // polls (most sleep ~2k cycles)
-
-STM_DEV uint32_t bitlen64(uint64_t x) { return x ? 64u - (uint32_t)__clzll(x) : 0u; }
-
-struct NormKey {
-  uint64_t kmin;
-  uint32_t shift;
-};
-
-// Every workgroup reduces the RANGE_BLOCKS partials of k_range itself (one per thread).
-STM_DEV NormKey norm_params(const uint32_t* words, int npass) {
-  static_assert(SORT_BLOCK == RANGE_BLOCKS, "one partial per thread");
-  __shared__ uint64_t s_mm[2][SORT_WAVES];
-  const uint64_t* range = reinterpret_cast<const uint64_t*>(words + RANGE_OFF);
-  uint64_t mn = range[2 * threadIdx.x], mx = range[2 * threadIdx.x + 1];
-#pragma unroll
-  for (int off = WAVE / 2; off >= 1; off >>= 1) {
-    const uint64_t a = shfl64(mn, lane_id() ^ off), b = shfl64(mx, lane_id() ^ off);
-    mn = a < mn ? a : mn;
-    mx = b > mx ? b : mx;
-  }
-  const int wave = threadIdx.x / WAVE;
-  if (lane_id() == 0) {
-    s_mm[0][wave] = mn;
-    s_mm[1][wave] = mx;
-  }
-  __syncthreads();
-  mn = s_mm[0][0];
-  mx = s_mm[1][0];
-#pragma unroll
-  for (int w = 1; w < SORT_WAVES; ++w) {
-    mn = s_mm[0][w] < mn ? s_mm[0][w] : mn;
-    mx = s_mm[1][w] > mx ? s_mm[1][w] : mx;
-  }
-  __syncthreads();
-  const uint64_t r = mx >= mn ? mx - mn : 0ull;
-  const uint32_t bl = bitlen64(r);
-  NormKey k;
-  k.kmin = mx >= mn ? mn : 0ull;
-  k.shift = bl > (uint32_t)(8 * npass) ? bl - (uint32_t)(8 * npass) : 0u;
-  return k;
-}
-
-STM_DEV uint32_t norm_of(uint64_t key, const NormKey& k) {
-  return (uint32_t)((key - k.kmin) >> k.shift);
-}
-
+// Batches above the single-sweep bound sort an order-preserving image of each key, of
+// NB = 8 * npass bits (24, or 32 above 2^23 keys):
+//   nk = (key - kmin) >> nshift,  nshift = max(0, bitlen(kmax - kmin) - NB)
+// with npass stable passes of the three-kernel form (the digits of nk instead of the key's
+// own: three or four passes instead of eight).  Equal nk form buckets of keys within 2^nshift
+// of each other, in input order.  k_fix then puts every bucket of at most FIX_SPAN elements in
+// key order (rank by key, ties by position: stable).  A longer bucket keeps input order, which
+// is already the sorted order when all its keys are equal (the ops of a hot key); if one holds
+// distinct keys, k_fix raises CTL_NEED_EXACT and k_sort_exact, one persistent launch that
+// otherwise returns at once, re-sorts the whole batch with the eight exact 8-bit passes.
 // Minimum and maximum of the batch's keys, one partial per workgroup.  Workgroup 0 also opens
 // the sort: the stall word, the exact-sort request and the barrier counter start at zero.
 __global__ __launch_bounds__(SORT_BLOCK) void k_range(const uint64_t* __restrict__ keys,
                                                       const BatchDesc* __restrict__ desc,
-                                                      uint32_t n_host, uint32_t* __restrict__ words) {
+                                                      uint32_t n_host, uint32_t* __restrict__ words,
+                                                      int npass) {
+  static_assert(SORT_BLOCK == RANGE_BLOCKS, "the last workgroup reduces one partial per thread");
   __shared__ uint64_t s_mm[2][SORT_WAVES];
+  __shared__ uint32_t s_last;
   const SortRange rg = sort_range(desc, n_host, keys, 1);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     words[CTL_STALL] = 0;
@@ -773,12 +769,13 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_range(const uint64_t* __restrict
   uint64_t mn = ~0ull, mx = 0ull;
   const uint64_t stride = (uint64_t)gridDim.x * SORT_BLOCK;
   uint64_t i = (uint64_t)blockIdx.x * SORT_BLOCK + threadIdx.x;
-  for (; i + 3 * stride < rg.n; i += 4 * stride) {
-    uint64_t k[4];
+  // Eight loads in flight per thread.
+  for (; i + 7 * stride < rg.n; i += 8 * stride) {
+    uint64_t k[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) k[u] = gload(rg.keys + i + u * stride);
+    for (int u = 0; u < 8; ++u) k[u] = gload(rg.keys + i + u * stride);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       mn = k[u] < mn ? k[u] : mn;
       mx = k[u] > mx ? k[u] : mx;
     }
@@ -808,207 +805,59 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_range(const uint64_t* __restrict
     uint64_t* range = reinterpret_cast<uint64_t*>(words + RANGE_OFF);
     range[2 * blockIdx.x] = mn;
     range[2 * blockIdx.x + 1] = mx;
+    // The workgroup that finishes last (arrival count RANGE_BLOCKS - 1 modulo RANGE_BLOCKS: the
+    // counter is never reset) reduces the partials into the sort's normalization.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint32_t done = __hip_atomic_fetch_add(words + CTL_RANGE_DONE, 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (done % RANGE_BLOCKS) == RANGE_BLOCKS - 1 ? 1u : 0u;
   }
-}
-
-// Tile digit histograms of one normalized pass.  FIRST: the records are computed from the
-// batch's keys; otherwise nk_in holds them.
-template <int ITEMS, bool FIRST>
-__global__ __launch_bounds__(SORT_BLOCK) void k_nhist(const uint64_t* __restrict__ keys,
-                                                      const BatchDesc* __restrict__ desc,
-                                                      uint32_t n_host, const uint32_t* __restrict__ nk_in,
-                                                      const uint32_t* __restrict__ words, int npass,
-                                                      int shift, uint32_t* __restrict__ hist,
-                                                      uint32_t stride) {
-  constexpr int TILE = SORT_BLOCK * ITEMS;
-  __shared__ uint32_t s_h[SORT_WAVES][RADIX];
-  const SortRange rg = sort_range(desc, n_host, keys, 1);
-  NormKey nkp{0, 0};
-  if constexpr (FIRST) nkp = norm_params(words, npass);
-  const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
-  const int wave = threadIdx.x / WAVE;
-  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
-    for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_h[0][0])[i] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)b * TILE;
-#pragma unroll 4
-    for (int r = 0; r < ITEMS; ++r) {
-      const uint64_t e = base + (uint64_t)r * SORT_BLOCK + threadIdx.x;
-      if (e < rg.n) {
-        const uint32_t v = FIRST ? norm_of(gload(rg.keys + e), nkp) : gload(nk_in + e);
-        atomicAdd(&s_h[wave][(v >> shift) & (RADIX - 1)], 1u);
-      }
-    }
-    __syncthreads();
-    const int d = threadIdx.x;  // SORT_BLOCK == RADIX
-    hist[(uint64_t)d * stride + b] = s_h[0][d] + s_h[1][d] + s_h[2][d] + s_h[3][d];
-    __syncthreads();
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const uint64_t* range = reinterpret_cast<const uint64_t*>(words + RANGE_OFF);
+  mn = range[2 * threadIdx.x];
+  mx = range[2 * threadIdx.x + 1];
+#pragma unroll
+  for (int off = WAVE / 2; off >= 1; off >>= 1) {
+    const uint64_t a = shfl64(mn, lane_id() ^ off), b = shfl64(mx, lane_id() ^ off);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
   }
-}
-
-// One normalized pass: stable in-tile ranking (as k_scatter), records staged in LDS in digit
-// order (nk, then payload: 4 bytes per element of LDS), coalesced runs out.  FIRST: nk from the
-// batch's keys, payload = position | op code << POS_BITS.
-template <int ITEMS, bool FIRST>
-__global__ __launch_bounds__(SORT_BLOCK) void k_nscatter(
-    const uint64_t* __restrict__ keys, const uint8_t* __restrict__ ops_in,
-    const BatchDesc* __restrict__ desc, uint32_t n_host,
-    const uint32_t* __restrict__ nk_in, const uint32_t* __restrict__ pl_in,
-    const uint32_t* __restrict__ words, int npass, int shift,
-    const uint32_t* __restrict__ hist, uint32_t stride, const uint32_t* __restrict__ totals,
-    uint32_t* __restrict__ nk_out, uint32_t* __restrict__ pl_out) {
-  constexpr int TILE = SORT_BLOCK * ITEMS;
-  constexpr int SEG = TILE / SORT_WAVES;
-  __shared__ uint32_t s_wcnt[SORT_WAVES][RADIX];
-  __shared__ uint32_t s_dbase[RADIX];
-  __shared__ uint32_t s_tbase[RADIX];
-  __shared__ uint32_t s_gbase[RADIX];
-  __shared__ uint32_t s_tmp[SORT_WAVES];
-  __shared__ uint32_t s_stage[TILE];
-
-  const SortRange rg = sort_range(desc, n_host, keys, 1);
-  const uint32_t ntiles = (rg.n + TILE - 1) / TILE;
-  const uint8_t* ops = desc ? desc->ops : ops_in;
-  NormKey nkp{0, 0};
-  if constexpr (FIRST) nkp = norm_params(words, npass);
-  const int lane = lane_id();
-  const int wave = threadIdx.x / WAVE;
-  {
-    uint32_t tot;
-    s_tbase[threadIdx.x] = block_exclusive_scan_256(totals[threadIdx.x], s_tmp, tot);
+  __syncthreads();
+  if (lane_id() == 0) {
+    s_mm[0][wave] = mn;
+    s_mm[1][wave] = mx;
   }
-  const uint64_t lt = lanemask_lt();
-  for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
-    const uint64_t tile0 = (uint64_t)b * TILE;
-    const uint32_t tile_n = (uint32_t)min<uint64_t>(TILE, rg.n - tile0);
-    for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wcnt[0][0])[i] = 0;
-    s_gbase[threadIdx.x] = s_tbase[threadIdx.x] + hist[(uint64_t)threadIdx.x * stride + b];
-    __syncthreads();
-    uint32_t k[ITEMS];
-    uint32_t rk[ITEMS];
-    uint32_t lane0 = (uint32_t)wave * SEG + lane;
-    uint32_t tid = threadIdx.x;
-    asm volatile("" : "+v"(lane0), "+v"(tid));
-    const uint32_t last = tile_n - 1;
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-      const uint32_t le = min(lane0 + (uint32_t)r * WAVE, last);
-      if constexpr (FIRST) k[r] = norm_of(gload(rg.keys + tile0 + le), nkp);
-      else k[r] = gload(nk_in + tile0 + le);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < SORT_WAVES; ++w) {
+      mn = s_mm[0][w] < mn ? s_mm[0][w] : mn;
+      mx = s_mm[1][w] > mx ? s_mm[1][w] : mx;
     }
-    asm volatile("" : "+v"(lane0));
-    uint32_t dk[(ITEMS + 3) / 4];
-#pragma unroll
-    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dk[q] = 0;
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-      const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
-      const uint32_t d = (k[r] >> shift) & (RADIX - 1);
-      dk[r / 4] |= d << (8 * (r % 4));
-      uint64_t peers = ballot(valid);
-#pragma unroll
-      for (int bit = 0; bit < 8; ++bit) {
-        const bool on = (d >> bit) & 1u;
-        const uint64_t bb = ballot(on);
-        peers &= on ? bb : ~bb;
-      }
-      uint32_t base = valid ? s_wcnt[wave][d] : 0u;
-      rk[r] = base + (uint32_t)__popcll(peers & lt);
-      asm volatile("" : "+v"(rk[r]));
-      const bool leader = valid && (lane == 63 - __clzll(peers));
-      if (leader) s_wcnt[wave][d] = base + (uint32_t)__popcll(peers);
-    }
-    __syncthreads();
-    {
-      const int d = threadIdx.x;
-      uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
-      uint32_t tot;
-      s_dbase[d] = block_exclusive_scan_256(c0 + c1 + c2 + c3, s_tmp, tot);
-      s_wcnt[0][d] = 0;
-      s_wcnt[1][d] = c0;
-      s_wcnt[2][d] = c0 + c1;
-      s_wcnt[3][d] = c0 + c1 + c2;
-    }
-    __syncthreads();
-    asm volatile("" : "+v"(lane0));
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-      if (lane0 + (uint32_t)r * WAVE < tile_n) {
-        const uint32_t d = (dk[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
-        rk[r] += s_dbase[d] + s_wcnt[wave][d];
-        s_stage[rk[r]] = k[r];
-      }
-    }
-    asm volatile("" : "+v"(lane0));
-    uint32_t id[ITEMS];
-    if constexpr (FIRST) {
-      if (ops) {
-        const uint8_t* ot = ops + tile0;
-#pragma unroll
-        for (int r = 0; r < ITEMS; ++r) {
-          const uint32_t le = lane0 + (uint32_t)r * WAVE;
-          const uint32_t o = gload(ot + min(le, last));
-          id[r] = ((uint32_t)tile0 + le) | ((o > 3u ? 1u : o) << POS_BITS);
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < ITEMS; ++r) id[r] = (uint32_t)tile0 + lane0 + (uint32_t)r * WAVE;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < ITEMS; ++r) id[r] = gload(pl_in + tile0 + min(lane0 + (uint32_t)r * WAVE, last));
-    }
-    __syncthreads();
-    uint32_t dg[(ITEMS + 3) / 4];
-#pragma unroll
-    for (int q = 0; q < (ITEMS + 3) / 4; ++q) dg[q] = 0;
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
-      if (p < tile_n) {
-        const uint32_t v = s_stage[p];
-        const uint32_t d = (v >> shift) & (RADIX - 1);
-        dg[r / 4] |= d << (8 * (r % 4));
-        const uint32_t dst = s_gbase[d] + (p - s_dbase[d]);
-        if (dst < rg.n) nk_out[dst] = v;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-      if (lane0 + (uint32_t)r * WAVE < tile_n) s_stage[rk[r]] = id[r];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-      const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
-      if (p < tile_n) {
-        const uint32_t d = (dg[r / 4] >> (8 * (r % 4))) & (RADIX - 1);
-        const uint32_t dst = s_gbase[d] + (p - s_dbase[d]);
-        if (dst < rg.n) pl_out[dst] = s_stage[p];
-      }
-    }
-    __syncthreads();
+    const uint64_t r = mx >= mn ? mx - mn : 0ull;
+    const uint32_t bl = bitlen64(r);
+    *reinterpret_cast<uint64_t*>(words + CTL_NORM) = mx >= mn ? mn : 0ull;
+    words[CTL_NORM + 2] = bl > (uint32_t)(8 * npass) ? bl - (uint32_t)(8 * npass) : 0u;
   }
 }
 
 // Buckets (runs of equal nk) into key order.  A step covers FIX_TILE sorted positions with
-// FIX_SPAN positions of context on each side in LDS; the batch's key of every position is
-// gathered through its payload.  Position i's bucket is found by scanning at most FIX_SPAN
-// positions each way: a bucket of at most FIX_SPAN elements lies wholly in the window, and its
-// element i goes to bucket start + its rank by (key, position).  Elements of longer buckets stay
-// where they are; a longer bucket with two different adjacent keys raises CTL_NEED_EXACT.
-__global__ __launch_bounds__(SORT_BLOCK) void k_fix(const uint32_t* __restrict__ nk,
-                                                    const uint32_t* __restrict__ pl,
-                                                    const uint64_t* __restrict__ keys_in,
+// FIX_SPAN positions of context on each side in LDS.  Position i's bucket is found by scanning
+// at most FIX_SPAN positions each way: a bucket of at most FIX_SPAN elements lies wholly in the
+// window, and its element i goes to bucket start + its rank by (key, position).  Elements of
+// longer buckets stay where they are; a longer bucket with two different adjacent keys raises
+// CTL_NEED_EXACT.
+__global__ __launch_bounds__(SORT_BLOCK) void k_fix(const uint64_t* __restrict__ keys_in,
+                                                    const uint32_t* __restrict__ idx_in,
                                                     const BatchDesc* __restrict__ desc,
                                                     uint32_t n_host, uint64_t* __restrict__ keys_out,
                                                     uint32_t* __restrict__ idx_out,
-                                                    uint32_t* __restrict__ words) {
-  __shared__ uint32_t s_nk[FIX_WIN];
+                                                    uint32_t* __restrict__ words, int npass) {
   __shared__ uint32_t s_pl[FIX_WIN];
   __shared__ uint64_t s_key[FIX_WIN];
-  const SortRange rg = sort_range(desc, n_host, keys_in, 1);
-  const uint32_t n = rg.n;
+  const uint32_t n = desc ? desc->n : n_host;
+  const NormKey nkp = norm_params(words);
   bool need = false;
   for (uint32_t t = blockIdx.x; (uint64_t)t * FIX_TILE < n; t += gridDim.x) {
     const uint32_t t0 = t * FIX_TILE, t1 = min(n, t0 + FIX_TILE);
@@ -1016,26 +865,24 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_fix(const uint32_t* __restrict__
     const uint32_t w1 = min(n, t1 + FIX_SPAN);
     for (uint32_t i = w0 + threadIdx.x; i < w1; i += SORT_BLOCK) {
       const uint32_t x = i - w0;
-      const uint32_t p = gload(pl + i);
-      s_nk[x] = gload(nk + i);
-      s_pl[x] = p;
-      s_key[x] = gload(rg.keys + pos_of(p));
+      s_key[x] = gload(keys_in + i);
+      s_pl[x] = gload(idx_in + i);
     }
     __syncthreads();
     const uint32_t wn = w1 - w0;
     for (uint32_t i = t0 + threadIdx.x; i < t1; i += SORT_BLOCK) {
       const uint32_t x = i - w0;
-      const uint32_t v = s_nk[x];
-      uint32_t l = x;
-      while (l > 0 && x - l < FIX_SPAN && s_nk[l - 1] == v) --l;
-      uint32_t r = x + 1;
-      while (r < wn && r - x <= FIX_SPAN && s_nk[r] == v) ++r;
       const uint64_t key = s_key[x];
+      const uint32_t v = norm_of(key, nkp);
+      uint32_t l = x;
+      while (l > 0 && x - l < FIX_SPAN && norm_of(s_key[l - 1], nkp) == v) --l;
+      uint32_t r = x + 1;
+      while (r < wn && r - x <= FIX_SPAN && norm_of(s_key[r], nkp) == v) ++r;
       if (x - l >= FIX_SPAN || r - x > FIX_SPAN || r - l > FIX_SPAN) {
         // A long bucket: keep input order; it is sorted already unless its keys differ.
         keys_out[i] = key;
         idx_out[i] = s_pl[x];
-        if (x > 0 && s_nk[x - 1] == v && s_key[x - 1] != key) need = true;
+        if (x > 0 && norm_of(s_key[x - 1], nkp) == v && s_key[x - 1] != key) need = true;
         continue;
       }
       uint32_t rank = 0;
@@ -1102,12 +949,12 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_sort_exact(
     uint64_t* kout = to_out ? keys_out : keys_tmp;
     uint32_t* iout = to_out ? idx_out : idx_tmp;
     const int first = p == 0 ? 1 : 0;
-    hist_body<ITEMS>(kin, desc, n_host, first, 8 * p, hist, stride);
+    hist_body<ITEMS>(kin, desc, n_host, first, 8 * p, hist, stride, nullptr, 0);
     grid_barrier(words, (++arrivals) * gridDim.x);
     scan_hist_body<ITEMS>(hist, stride, desc, n_host, totals);
     grid_barrier(words, (++arrivals) * gridDim.x);
     scatter_body<ITEMS>(kin, iin, first ? ops_in : nullptr, desc, n_host, first, 8 * p, hist,
-                        stride, totals, kout, iout);
+                        stride, totals, kout, iout, nullptr, 0);
     grid_barrier(words, (++arrivals) * gridDim.x);
     kin = kout;
     iin = iout;
@@ -1163,10 +1010,9 @@ static hipError_t exact_grid(uint32_t& grid) {
   return hipSuccess;
 }
 
-// Batches above the single-sweep bound: k_range, npass normalized passes (k_nhist,
-// k_scan_hist, k_nscatter), k_fix, k_sort_exact (returns at once unless k_fix asked for it).
-// Records (nk, payload) ping-pong between keys_tmp and keys_out, each viewed as two u32 arrays
-// of n_max; the last pass lands in keys_tmp so k_fix can write keys_out.
+// Batches above the single-sweep bound: k_range, npass normalized passes of the three-kernel
+// form, k_fix, k_sort_exact (returns at once unless k_fix asked for it).  The passes ping-pong
+// between the tmp and out arrays; the last lands in tmp so k_fix can write out.
 template <int ITEMS>
 static hipError_t sort_normalized(const uint64_t* keys_in, const uint8_t* ops_in,
                                   const BatchDesc* desc, uint32_t n_host, uint32_t n_max,
@@ -1178,41 +1024,32 @@ static hipError_t sort_normalized(const uint64_t* keys_in, const uint8_t* ops_in
   const int npass = n_max > (1u << 23) ? 4 : 3;
   uint32_t* hist = words + ONESWEEP_WORDS;
   uint32_t* totals = hist + (size_t)stride * RADIX;
-  uint32_t* rec_a = reinterpret_cast<uint32_t*>(keys_tmp);
-  uint32_t* rec_b = reinterpret_cast<uint32_t*>(keys_out);
   uint32_t xgrid = 0;
   hipError_t e = exact_grid<ITEMS>(xgrid);
   if (e != hipSuccess) return e;
-  k_range<<<RANGE_BLOCKS, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, words);
-  const uint32_t *nk_in = nullptr, *pl_in = nullptr;
+  k_range<<<RANGE_BLOCKS, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, words, npass);
+  const uint64_t* kin = keys_in;
+  const uint32_t* iin = nullptr;
   for (int p = 0; p < npass; ++p) {
-    uint32_t* out = ((npass - 1 - p) % 2) == 0 ? rec_a : rec_b;
-    uint32_t* nk_out = out;
-    uint32_t* pl_out = out + n_max;
-    if (p == 0) {
-      k_nhist<ITEMS, true><<<grid, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, nullptr, words,
-                                                            npass, 0, hist, stride);
-      k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
-      k_nscatter<ITEMS, true><<<grid, SORT_BLOCK, 0, stream>>>(
-          keys_in, ops_in, desc, n_host, nullptr, nullptr, words, npass, 0, hist, stride, totals,
-          nk_out, pl_out);
-    } else {
-      k_nhist<ITEMS, false><<<grid, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, nk_in, words,
-                                                             npass, 8 * p, hist, stride);
-      k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
-      k_nscatter<ITEMS, false><<<grid, SORT_BLOCK, 0, stream>>>(
-          keys_in, ops_in, desc, n_host, nk_in, pl_in, words, npass, 8 * p, hist, stride, totals,
-          nk_out, pl_out);
-    }
+    const bool to_tmp = ((npass - 1 - p) % 2) == 0;
+    uint64_t* kout = to_tmp ? keys_tmp : keys_out;
+    uint32_t* iout = to_tmp ? idx_tmp : idx_out;
+    const int first = p == 0 ? 1 : 0;
+    k_hist<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, desc, n_host, first, 8 * p, hist, stride,
+                                                   words, 1, npass);
+    k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
+    k_scatter<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, first ? ops_in : nullptr, desc,
+                                                      n_host, first, 8 * p, hist, stride, totals,
+                                                      kout, iout, words, npass);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    nk_in = nk_out;
-    pl_in = pl_out;
+    kin = kout;
+    iin = iout;
   }
   const uint32_t fgrid = std::max<uint32_t>(1, std::min<uint32_t>((n_max + FIX_TILE - 1) / FIX_TILE,
                                                                   SORT_GRID_CAP));
-  k_fix<<<fgrid, SORT_BLOCK, 0, stream>>>(nk_in, pl_in, keys_in, desc, n_host, keys_out, idx_out,
-                                          words);
+  k_fix<<<fgrid, SORT_BLOCK, 0, stream>>>(keys_tmp, idx_tmp, desc, n_host, keys_out, idx_out, words,
+                                          npass);
   k_sort_exact<ITEMS><<<xgrid, SORT_BLOCK, 0, stream>>>(keys_in, ops_in, desc, n_host, keys_tmp,
                                                        keys_out, idx_tmp, idx_out, words, stride);
   return hipGetLastError();
@@ -1275,11 +1112,11 @@ static hipError_t sort_passes(const uint64_t* keys_in, const uint8_t* ops_in, co
     uint32_t* iout = to_out ? idx_out : idx_tmp;
     const int first = p == 0 ? 1 : 0;
     k_hist<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, desc, n_host, first, shift, hist, stride,
-                                                   words);
+                                                   words, 0, 0);
     k_scan_hist<ITEMS><<<RADIX, SORT_BLOCK, 0, stream>>>(hist, stride, desc, n_host, totals);
     k_scatter<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, first ? ops_in : nullptr, desc,
                                                       n_host, first, shift, hist, stride,
-                                                      totals, kout, iout);
+                                                      totals, kout, iout, nullptr, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     kin = kout;
