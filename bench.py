@@ -80,6 +80,8 @@ def parse():
                     help="tuning record to use for the uneven stream instead of autotuning")
     ap.add_argument("--tune-batches", type=int, default=40,
                     help="batches of the stream the autotuner measures")
+    ap.add_argument("--no-small-batches", dest="small_batches", action="store_false",
+                    help="skip the per-route streams of 2^12..2^16-op batches")
     ap.add_argument("--plain-trials", type=int, default=3,
                     help="plain-uniform section: fresh-tree trials (0 = skip the section)")
     ap.add_argument("--contended-batches", type=int, default=8,
@@ -498,6 +500,58 @@ def main():
         log(f"[{route} stream] {args.steps} batches in {elapsed:.3f}s -> {mops:.1f} Mops/s")
         return {"mops": mops, "elapsed": elapsed, "tail": tail()}
 
+    def small_batch_routes():
+        """Routing at small batch sizes (the sizes of the uneven stream and of serving
+        requests): per route, on a fresh tree from the same bulk load, streams of batches of
+        2^12, 2^14 and 2^16 ops (2^21 ops each, 95/5, Gamma keys) replayed from graphs.  The
+        router is trained once before the streams (no periodic retrain inside them), so the
+        comparison is the routing itself."""
+        sizes = [(12, 512), (14, 128), (16, 32)]
+        sets = []
+        for lg, cnt in sizes:
+            nb_ = 1 << lg
+            sk = np.empty(cnt * nb_, dtype=np.uint64)
+            sv = np.empty_like(sk)
+            so = np.empty(cnt * nb_, dtype=np.uint8)
+            for b in range(cnt):
+                s_ = slice(b * nb_, (b + 1) * nb_)
+                sk[s_], sv[s_], so[s_] = w.mixed_batch(load_sorted, nb_, args.seed * 7000 + lg * 1000 + b,
+                                                       0.95, 0.9, w.DIST_GAMMA, 2.0, 2.0 ** 56)
+            offs = np.arange(cnt + 1, dtype=np.uint64) * np.uint64(nb_)
+            sets.append((lg, cnt, [ea.DeviceBuffer.from_array(a) for a in (sk, sv, so)], offs))
+        n_small = max(c * (1 << lg) for lg, c in sizes)
+        sov, sof = ea.DeviceBuffer(n_small, np.uint64), ea.DeviceBuffer(n_small, np.uint8)
+        out = {}
+        for route in ("descent", "learned", "index"):
+            cfg = ea.StmConfig(fanout=args.fanout, max_keys=n_load + (1 << 24), max_batch=n_batch,
+                               lanes_per_op=args.lanes, ops_per_group=args.group,
+                               route={"learned": ea.ROUTE_LEARNED, "index": ea.ROUTE_INDEX}.get(
+                                   route, ea.ROUTE_DESCENT), retrain_every=1 << 20)
+            with ea.StmTree(cfg) as t:
+                dk, dv = ea.DeviceBuffer.from_array(load), ea.DeviceBuffer.from_array(load_vals)
+                t.bulk_load_device(dk, dv)
+                dk.free()
+                dv.free()
+                t.capture_graphs()
+                r = {}
+                for lg, cnt, (sk, sv, so), offs in sets:
+                    t.sync()
+                    t0 = time.perf_counter()
+                    t.run_batches_device(sk, sv, so, sov, sof, offs)
+                    r[f"2^{lg}"] = round(cnt * (1 << lg) / (time.perf_counter() - t0) / 1e6, 1)
+                out[route] = r
+            log(f"[small batches] {route}: {out[route]} Mops/s")
+        for _, _, bufs, _ in sets:
+            for b_ in bufs:
+                b_.free()
+        sov.free()
+        sof.free()
+        out["learned_over_descent"] = {k: round(out["learned"][k] / out["descent"][k], 3)
+                                       for k in out["descent"]}
+        out["learned_over_index"] = {k: round(out["learned"][k] / out["index"][k], 3)
+                                     for k in out["descent"]}
+        return out
+
     bk, bv, bo = (ea.DeviceBuffer.from_array(a) for a in (keys, vals, ops))
     out_v = ea.DeviceBuffer(keys.size, np.uint64)
     out_f = ea.DeviceBuffer(keys.size, np.uint8)
@@ -518,7 +572,8 @@ def main():
     streams = {}
     if args.stream and head_name.startswith("learned"):
         plain = head_name.replace("learned", "descent")
-        for r in (head_name, plain):
+        # the learned route at the defaults too, so the tuning's gain is measured on the stream
+        for r in dict.fromkeys((head_name, plain, "learned")):
             if r in res:
                 streams[r] = run_stream(r)
                 x = streams[r]
@@ -620,6 +675,7 @@ def main():
     uneven = None
     serving = None
     bk.free(); bv.free(); bo.free(); out_v.free(); out_f.free()
+    small = small_batch_routes() if args.small_batches and args.route == "both" else None
     plain = run_plain(args, ea, w) if args.plain_trials > 0 else None
     contended = run_contended(args, ea, w) if args.contended_batches > 0 else None
     if args.uneven_batches > 0:
@@ -654,7 +710,11 @@ def main():
                     for r, x in streams.items()},
         "learned_over_plain_stream": (round(streams[head_name]["mops"] /
                                             streams[head_name.replace("learned", "descent")]["mops"], 4)
-                                      if len(streams) == 2 else None),
+                                      if head_name.replace("learned", "descent") in streams else None),
+        # the tuning record's launch parameters over the defaults, both on the learned stream
+        "tuned_over_default_stream": (round(streams["learned_tuned"]["mops"] / streams["learned"]["mops"], 4)
+                                      if "learned_tuned" in streams and "learned" in streams else None),
+        "small_batches": small,
         # learned over plain descent at the same launch parameters (defaults, tuned)
         "learned_over_plain": {("descent" + k): round(res["learned" + k]["mops"] / res["descent" + k]["mops"], 4)
                                for k in ("", "_tuned") if "learned" + k in res and "descent" + k in res},

@@ -55,6 +55,7 @@ class _Config(C.Structure):
         ("max_attempts", C.c_uint32), ("route", C.c_uint32), ("timing", C.c_uint32),
         ("retrain_every", C.c_uint32), ("train_steps", C.c_uint32),
         ("sort_tile_items", C.c_uint32), ("use_graphs", C.c_uint32), ("upsert_lanes", C.c_uint32),
+        ("retrain_splits", C.c_uint32),
     ]
 
 
@@ -243,13 +244,15 @@ class StmConfig:
     sort_tile_items: int = 16
     use_graphs: bool = True
     upsert_lanes: int = 0  # 0 = automatic
+    retrain_splits: int = 0  # 0 = automatic: a 32nd of the directory's leaves
 
     def to_c(self) -> _Config:
         c = _Config()
         _check(load_library().stm_config_init(C.byref(c)))
         for f in ("fanout", "leaf_fill", "max_keys", "max_batch", "device", "lanes_per_op",
                   "ops_per_group", "upsert_ops_per_wave", "max_attempts", "route",
-                  "retrain_every", "train_steps", "sort_tile_items", "use_graphs", "upsert_lanes"):
+                  "retrain_every", "train_steps", "sort_tile_items", "use_graphs", "upsert_lanes",
+                  "retrain_splits"):
             setattr(c, f, int(getattr(self, f)))
         c.timing = 1 if self.timing else 0
         return c

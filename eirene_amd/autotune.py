@@ -14,7 +14,8 @@ more than `min_gain`.  A trial bulk-loads a fresh tree from device-resident keys
 sample stream once untimed (first touch), then times the stream on fresh trees loaded from the
 same keys, so every trial starts from the same tree.  Each timed tree captures its graphs and
 (with learned routing) trains its router before the clock starts; throughput is ops / wall
-time of stm_run_batches_device, which returns after the stream has completed.
+time of stm_run_batches_device, which returns after the stream has completed, and a trial's
+figure is the median of three such timed streams.
 
 The result is a JSON tuning file:
     {"device": ..., "workload": ..., "best": {<StmConfig fields>}, "best_mops": ...,
@@ -68,10 +69,10 @@ class Sample:
             b.free()
 
 
-def _trial(sample: Sample, cfg: StmConfig, repeats: int = 2) -> float:
-    """Mops/s of the sample stream under cfg: best of `repeats` timed replays, each on a
+def _trial(sample: Sample, cfg: StmConfig, repeats: int = 3) -> float:
+    """Mops/s of the sample stream under cfg: the median of `repeats` timed replays, each on a
     fresh tree bulk-loaded from the same keys (so every trial starts from the same tree)."""
-    best = 0.0
+    runs: List[float] = []
     routed = cfg.route != ROUTE_DESCENT
     with StmTree(cfg) as t:
         t.bulk_load_device(sample.load_k, sample.load_v, sample.n_load)
@@ -89,8 +90,8 @@ def _trial(sample: Sample, cfg: StmConfig, repeats: int = 2) -> float:
             t.run_batches_device(sample.k, sample.v, sample.o, sample.out_v, sample.out_f,
                                  sample.offsets)
             dt = time.perf_counter() - t0
-        best = max(best, sample.n_ops / dt / 1e6)
-    return best
+        runs.append(sample.n_ops / dt / 1e6)
+    return float(np.median(runs))
 
 
 def tune(sample: Sample, base: StmConfig, fanouts=FANOUTS, sweeps: int = 2,

@@ -62,7 +62,8 @@ struct StreamAcc {
   unsigned long long skipped;  // batches skipped after an error
   unsigned long long sort_stalls;  // batches whose sort gave up a look-back wait
   unsigned long long fallbacks;    // batches whose normalized sort fell back to the exact passes
-  unsigned long long pad[2];
+  unsigned long long leaf_splits;  // leaves the batches split
+  unsigned long long pad[1];
 };
 
 constexpr uint32_t DESC_TABLE = 4096;  // descriptors uploaded per window of a stream
@@ -94,6 +95,7 @@ __global__ void k_epilogue(const BatchCounters* __restrict__ cnt, const BatchDes
   acc->bad += cnt->n_bad;
   acc->sort_stalls += cnt->sort_stall;
   acc->fallbacks += cnt->sort_fallback;
+  acc->leaf_splits += cnt->leaf_splits;
   acc->batches += 1;
 }
 
@@ -221,6 +223,8 @@ struct stm_tree {
   bool route_valid = false;       // directory + model describe a recent tree
   bool route_warm = false;        // the segment models were trained on this tree
   uint32_t batches_since_train = 0;
+  uint64_t splits_since_train = 0;  // leaf splits counted since the last directory rebuild
+  uint64_t dir_leaves = 0;          // leaves of the directory at the last rebuild
   // one instantiated graph per batch size class (null until first needed)
   hipGraphExec_t graphs[GRAPH_CLASSES] = {};
   // Batches above 2^18 ops sort normalized keys unless exact_left > 0: a batch whose normalized
@@ -410,6 +414,11 @@ int train_router(stm_tree* t, uint32_t steps, bool periodic) {
   }
   t->route_valid = true;
   t->batches_since_train = 0;
+  t->splits_since_train = 0;
+  RouteHeader rh{};
+  HIP_TRY(hipMemcpyAsync(&rh, t->route.hdr, sizeof(rh), hipMemcpyDeviceToHost, t->stream));
+  HIP_TRY(hipStreamSynchronize(t->stream));
+  t->dir_leaves = rh.L;
   return STM_OK;
 }
 
@@ -561,9 +570,16 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
 
 // Learned routing and the static index: (re)build the directory, and retrain, when it is
 // missing or stale.
+// Also after split-heavy batches: once the leaves split since the last rebuild reach
+// retrain_splits (a 32nd of the directory's leaves by default), lookups would walk right
+// siblings from too many stale entries.
 int maybe_retrain(stm_tree* t) {
   if (t->cfg.route == STM_ROUTE_DESCENT) return STM_OK;
-  if (t->route_valid && t->batches_since_train < t->cfg.retrain_every) return STM_OK;
+  const uint64_t split_limit =
+      t->cfg.retrain_splits ? t->cfg.retrain_splits : std::max<uint64_t>(1, t->dir_leaves / 32);
+  if (t->route_valid && t->batches_since_train < t->cfg.retrain_every &&
+      t->splits_since_train < split_limit)
+    return STM_OK;
   return train_router(t, t->cfg.train_steps, true);
 }
 
@@ -612,6 +628,7 @@ int run_chunk(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uin
   t->st.ops += n;
   t->st.unique_ops += bc.n_unique;
   t->batches_since_train += 1;
+  t->splits_since_train += bc.leaf_splits;
   if (bc.n_bad) return fail(STM_ERR_INVALID, "batch holds the reserved key 0xffffffffffffffff");
   if (so) {
     so->used = bc.scan_total;
@@ -710,6 +727,7 @@ int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const ui
   HIP_TRY(hipStreamSynchronize(st));
   t->st.unique_ops += acc.unique;
   t->sort_fallbacks += acc.fallbacks;
+  t->splits_since_train += acc.leaf_splits;
   if (acc.fallbacks) t->exact_left = EXACT_HOLD;
   else if (t->graphs_exact) t->exact_left -= (uint32_t)std::min<uint64_t>(t->exact_left, acc.batches);
   if (acc.sort_stalls)

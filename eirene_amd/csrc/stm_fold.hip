@@ -202,6 +202,7 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_scan_tiles(Acc* tile_agg, F
     counters->n_fanout = 0;
     counters->sort_stall = stalled;
     counters->sort_fallback = a.sort_exact ? *a.sort_exact : 0u;
+    counters->leaf_splits = 0;
   }
 }
 

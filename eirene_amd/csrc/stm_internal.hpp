@@ -80,7 +80,8 @@ struct BatchCounters {
   uint32_t n_fanout;    // answer chunks of hot GET groups queued for k_fanout
   uint64_t hdr_error;   // the tree's sticky error bits after the batch (one readback per batch)
   uint32_t sort_fallback;  // the batch's normalized sort fell back to the exact passes
-  uint32_t pad[3];
+  uint32_t leaf_splits;    // leaves the batch's transactions split
+  uint32_t pad[2];
 };
 static_assert(sizeof(BatchCounters) == 64, "one 64-byte block");
 

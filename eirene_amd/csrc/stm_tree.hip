@@ -1037,6 +1037,12 @@ STM_DEV void flush_counters(const TreeView& t, const TxCounters& c) {
   }
 }
 
+// Also adds the wavefront's leaf splits to the batch's counter (the host's retrain trigger).
+STM_DEV void flush_counters(const TreeView& t, const TxCounters& c, BatchCounters* cnt) {
+  if (lane_id() == 0 && c.leaf_splits) atomicAdd(&cnt->leaf_splits, (uint32_t)c.leaf_splits);
+  flush_counters(t, c);
+}
+
 // Lane-group fast path: NG = 64 / G merged INSERTs of one wavefront at once, each by G lanes
 // holding its leaf strided (slot e*G + g in element e of lane g, so every load and store
 // instruction covers G consecutive slots).  It is the leaf-only transaction of the wave-wide
@@ -1349,7 +1355,7 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
       }
     }
   }
-  flush_counters(t, c);
+  flush_counters(t, c, cnt);
   if (lane_id() == 0) {
     stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
     stat_add(t.stats, ST_DEFERRED, wave_stripe(), deferred);
@@ -1384,7 +1390,7 @@ __global__ __launch_bounds__(64) void k_upsert_serial(TreeView t, const uint64_t
       if (lane_id() == 0) atomicOr((unsigned long long*)&t.hdr->error, (unsigned long long)ERR_LIVELOCK);
     }
   }
-  flush_counters(t, c);
+  flush_counters(t, c, const_cast<BatchCounters*>(cnt));
   // The last kernel of a batch: the tree's error bits travel back with the batch counters.
   wave_order();
   const uint64_t err = ld_relaxed(&t.hdr->error);

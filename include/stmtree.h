@@ -78,6 +78,11 @@ typedef struct stm_config {
                              8, 16 or 32, at most fanout (64 / this many inserts per
                              wavefront at once); 0 = automatic (default): 8 up to fanout 64,
                              16 above */
+    uint32_t retrain_splits; /* learned routing / static index: also rebuild the directory (and
+                                retrain) before a batch once this many leaves have split since
+                                the last retrain; 0 = automatic (default): a 32nd of the leaves
+                                in the directory.  Counted per synchronous batch; a batch stream
+                                counts its splits when it ends */
 } stm_config;
 
 /* Per-phase timing slots of stm_stats.phase_ms / phase_launches. */

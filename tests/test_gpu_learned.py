@@ -130,3 +130,90 @@ def test_trained_router_beats_untrained():
     assert trained["window_misses"] * 2 < untrained["window_misses"]
     assert rate_t > 0.9
     t.close()
+
+
+@pytest.mark.parametrize("per_group", [1, 16])
+def test_trained_router_reads_fewer_nodes_than_descent(oracle_mod, per_group):
+    """What the network buys on the lookup path: on the same small batch (2^14 lookups over
+    2^22 Gamma keys, so lookups rarely share a leaf), root-to-leaf descent reads inner nodes to
+    route (every lookup when a lane group walks one op; the first of a span and every parent
+    change when it walks 16 sorted ops), while the trained router routes nearly every key from a
+    directory window straight to its leaf, and the untrained router (linear root, identity-like
+    segment models) misses its window more often.  Tree nodes read per merged lookup, with
+    identical answers (checked against the oracle).  A window the untrained model misses costs
+    one more directory window load rather than a node read, so between the two routers the
+    difference shows in the model's window hit rate."""
+    import eirene_amd as ea
+    from eirene_amd import workloads as w
+    n = 1 << 22
+    load = w.gamma_keys(n, 31, 2.0, 2.0 ** 56)
+    vals = w.values_for(load)
+    srt = w.sorted_copy(load)
+    rng = np.random.default_rng(5)
+    keys = rng.choice(load, 1 << 14)
+    ops = np.full(keys.size, ea.OP_GET, dtype=np.uint8)
+    o = oracle_mod.Oracle()
+    o.bulk_load(srt, w.values_for(srt))
+    want = o.run_batch(keys, np.zeros_like(keys), ops)
+    reads, hit_rate = {}, {}
+    for name, route, steps in (("descent", ea.ROUTE_DESCENT, None), ("untrained", ea.ROUTE_LEARNED, 0),
+                               ("trained", ea.ROUTE_LEARNED, 300)):
+        with ea.StmTree(ea.StmConfig(max_keys=2 * n, max_batch=1 << 18, route=route,
+                                     ops_per_group=per_group)) as t:
+            t.bulk_load(load, vals)
+            if steps is not None:
+                t.retrain(steps)
+            t.reset_stats()
+            got = t.run_batch(keys, np.zeros_like(keys), ops)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), name
+            st = t.stats()
+            reads[name] = st["node_reads"] / st["unique_ops"]
+            hit_rate[name] = st["route_hits"] / max(1, st["route_hits"] + st["route_misses"])
+    print(f"tree nodes read per merged lookup, {per_group} ops per lane group: {reads}; "
+          f"model window hit rate {hit_rate}")
+    assert reads["trained"] < reads["descent"] and reads["trained"] <= reads["untrained"]
+    assert reads["trained"] < 1.2  # the leaf, and rarely more
+    if per_group == 1:
+        assert reads["descent"] >= 3.0  # root, inner levels and the leaf of every lookup
+    assert hit_rate["trained"] >= hit_rate["untrained"] and hit_rate["trained"] > 0.99
+
+
+def test_split_heavy_batches_trigger_a_retrain(oracle_mod):
+    """With periodic retraining out of the way (retrain_every = 1000), insert-heavy batches that
+    split more than a 32nd of the directory's leaves make the next batch rebuild the directory
+    and retrain: its leaf count then covers the splits.  Answers against the oracle throughout."""
+    import eirene_amd as ea
+    from eirene_amd import workloads as w
+    n = 1 << 18
+    load = w.gamma_keys(n, 12, 2.0, 2.0 ** 56)
+    vals = w.values_for(load)
+    srt = w.sorted_copy(load)
+    o = oracle_mod.Oracle()
+    o.bulk_load(srt, w.values_for(srt))
+    cfg = ea.StmConfig(max_keys=4 * n, max_batch=1 << 18, route=ea.ROUTE_LEARNED,
+                       retrain_every=1000, train_steps=50)
+    with ea.StmTree(cfg) as t:
+        t.bulk_load(load, vals)
+        probe = srt[::97].copy()
+        gets = np.full(probe.size, ea.OP_GET, dtype=np.uint8)
+        got = t.run_batch(probe, np.zeros_like(probe), gets)  # first batch: directory + training
+        want = o.run_batch(probe, np.zeros_like(probe), gets)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        leaves0 = len(t.route_directory()[0])
+        splits = 0
+        b = 0
+        while splits < leaves0 // 32:
+            keys, bv, ops = w.mixed_batch(srt, 1 << 16, 90 + b, 0.0, 0.0, w.DIST_GAMMA, 2.0,
+                                          2.0 ** 56)  # all inserts
+            got = t.run_batch(keys, bv, ops)
+            want = o.run_batch(keys, bv, ops)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), b
+            splits = t.stats()["leaf_splits"]
+            assert len(t.route_directory()[0]) == leaves0  # no periodic retrain yet
+            b += 1
+        got = t.run_batch(probe, np.zeros_like(probe), gets)
+        want = o.run_batch(probe, np.zeros_like(probe), gets)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        leaves1 = len(t.route_directory()[0])
+        assert leaves1 >= leaves0 + splits, (leaves0, splits, leaves1)
+        assert t.check() == 0
