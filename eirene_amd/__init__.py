@@ -65,7 +65,7 @@ class _Stats(C.Structure):
         "upserts", "commits", "aborts", "leaf_splits", "inner_splits", "root_splits",
         "deferred", "route_hits", "route_misses")] + [
         ("phase_ms", C.c_double * 8), ("phase_launches", C.c_uint64 * 8),
-        ("node_reads", C.c_uint64), ("sort_fallbacks", C.c_uint64)]
+        ("node_reads", C.c_uint64), ("leaf_steps", C.c_uint64), ("sort_fallbacks", C.c_uint64)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -408,6 +408,7 @@ class StmTree:
         d["phase_ms"] = {p: s.phase_ms[i] for i, p in enumerate(PHASES)}
         d["phase_launches"] = {p: s.phase_launches[i] for i, p in enumerate(PHASES)}
         d["node_reads"] = s.node_reads
+        d["leaf_steps"] = s.leaf_steps
         d["sort_fallbacks"] = s.sort_fallbacks
         return d
 

@@ -1460,6 +1460,7 @@ int stm_get_stats(stm_tree* t, stm_stats* out) {
   out->route_hits = c[ST_ROUTE_HITS];
   out->route_misses = c[ST_ROUTE_MISSES];
   out->node_reads = c[ST_NODE_READS];
+  out->leaf_steps = c[ST_LEAF_STEPS];
   out->sort_fallbacks = t->sort_fallbacks;
   return STM_OK;
 }

@@ -69,6 +69,7 @@ enum StatSlot : int {
   ST_ROUTE_HITS,
   ST_ROUTE_MISSES,
   ST_NODE_READS,  // tree nodes read by lookups and transactions (the reference's traversal steps)
+  ST_LEAF_STEPS,  // right-sibling steps along the leaf level (the reference's leaf traversal steps)
   ST_KEYS,  // keys added since the last bulk build (kept by stm_reset_stats); must stay last
   ST_NUM
 };

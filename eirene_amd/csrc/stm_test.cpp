@@ -3,8 +3,10 @@
 //   stm_test -v <method> -i <op file> [-i <op file> ...] [-t]
 //     -v 38 | 39 | plain   root-to-leaf descent (the reference's STM versions)
 //     -v learned           learned leaf routing
-//     -i file              run the file's ops as batches ("R k", "P k v", "G a b")
-//     -t                   print every op's answer of the following -i files
+//     -i file              run the file's ops as batches ("R k", "P k v", "G a b"; signed
+//                          keys and values, as the reference's Data_Collector<int, int>)
+//     -t                   print every op's answer of the following -i files ("G a b": the
+//                          number of pairs with a <= key <= b, then one "pair:" line each)
 // Unlike the reference, -i before -v is an error instead of a crash (main.cpp:13,25-27),
 // and -t switches printing on (the reference toggles a bool with ~, main.cpp:23).
 #include <unistd.h>

@@ -354,14 +354,18 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
             reads += 1;
             if (dirty && key >= mh) {  // the key lives in a right sibling made by a split
               routed = false;
+              uint32_t steps = 0;
               for (int step = 0; step < 64; ++step) {
                 leaf = (uint32_t)min<uint64_t>(t.meta[leaf].next, cap - 1);
                 mh = t.meta[leaf].high;
+                steps = (uint32_t)step + 1;
                 if (key < mh) {
                   routed = true;
                   break;
                 }
               }
+              // Rare (only after splits): counted where it happens, so no register carries it.
+              if (g == 0) stat_add(t.stats, ST_LEAF_STEPS, blockIdx.x, steps);
               if (routed) {
                 load_slots<F, G>(t.keys, leaf, g, lk);
                 reads += 1;

@@ -117,6 +117,9 @@ typedef struct stm_stats {
     uint64_t phase_launches[STM_NUM_PHASES];
     uint64_t node_reads;      /* tree nodes read by lookups and transactions (the reference's
                                  ABORT_COUNT traversal steps, src/gpu-stm-tree-v38/bpt-gpustm.cpp:142-160) */
+    uint64_t leaf_steps;      /* right-sibling steps taken along the leaf level (learned and index
+                                 routing after leaf splits; the reference's leaf traversal steps,
+                                 src/gpu-stm-tree-v38/bpt-gpustm.cpp:142-160) */
     uint64_t sort_fallbacks;  /* batches above 2^18 ops whose sort of normalized keys met a long
                                  run of close, distinct keys (hot keys with neighbours) and re-sorted
                                  with the exact passes; the next 16 batches then sort exactly */

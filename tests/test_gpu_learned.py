@@ -203,14 +203,18 @@ def test_split_heavy_batches_trigger_a_retrain(oracle_mod):
         splits = 0
         b = 0
         while splits < leaves0 // 32:
-            keys, bv, ops = w.mixed_batch(srt, 1 << 16, 90 + b, 0.0, 0.0, w.DIST_GAMMA, 2.0,
-                                          2.0 ** 56)  # all inserts
+            # all inserts, small batches: leaves fill over many batches, so later batches
+            # route through a directory whose leaves have already split
+            keys, bv, ops = w.mixed_batch(srt, 1 << 12, 90 + b, 0.0, 0.0, w.DIST_GAMMA, 2.0,
+                                          2.0 ** 56)
             got = t.run_batch(keys, bv, ops)
             want = o.run_batch(keys, bv, ops)
             assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), b
             splits = t.stats()["leaf_splits"]
             assert len(t.route_directory()[0]) == leaves0  # no periodic retrain yet
             b += 1
+        # routing through the stale directory walked right siblings of split leaves
+        assert b >= 2 and t.stats()["leaf_steps"] > 0, (b, t.stats()["leaf_steps"])
         got = t.run_batch(probe, np.zeros_like(probe), gets)
         want = o.run_batch(probe, np.zeros_like(probe), gets)
         assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
