@@ -128,6 +128,7 @@ struct Scratch {
   uint32_t* ilist = nullptr;  // merged INSERTs of the batch
   uint32_t* upar = nullptr;   // level-1 node hint of every merged INSERT
   uint64_t* uver = nullptr;   // version of the hinted leaf
+  uint32_t* upos = nullptr;   // slot hint in the hinted leaf
   uint64_t* fanout = nullptr; // answer chunks of hot GET groups (fanout_capacity(n))
   BatchCounters* counters = nullptr;
   BatchCounters* h_counters = nullptr;  // pinned host copy for the per-batch readback
@@ -164,6 +165,7 @@ struct Scratch {
     A_(ilist, cap);
     A_(upar, cap);
     A_(uver, cap);
+    A_(upos, cap);
     A_(fanout, fanout_capacity(cap));
     A_(counters, 1);
     if (staging) {
@@ -195,7 +197,7 @@ struct Scratch {
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
-                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, scount, ilist, upar, uver, fanout, counters,
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, scount, ilist, upar, uver, upos, fanout, counters,
                   desc_table, desc_live, cursor, acc, soff, lastins, scan_tiles};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -519,6 +521,7 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
   la.desc = in.desc;
   la.uleaf = s.uleaf;
   la.uver = s.uver;
+  la.upos = s.upos;
   la.upar = s.upar;
   la.gval = s.uval;  // GET groups carry no merged value: a hot group's value goes there
   la.fanout = s.fanout;
@@ -553,6 +556,7 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
   ua.desc = in.desc;
   ua.uleaf = s.uleaf;
   ua.uver = s.uver;
+  ua.upos = s.upos;
   ua.upar = s.upar;
   ua.ilist = s.ilist;
   ua.slist = s.slist;

@@ -263,6 +263,7 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
                                                 const BatchDesc* __restrict__ desc,
                                                 uint32_t* __restrict__ uleaf,
                                                 uint64_t* __restrict__ uver,
+                                                uint32_t* __restrict__ upos,
                                                 uint32_t* __restrict__ upar,
                                                 uint64_t* __restrict__ gval,
                                                 uint64_t* __restrict__ fanout,
@@ -419,16 +420,27 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
       const uint64_t at = group_at<F, G>(lk, max(s, 0), gbase);
       if (g == (int)jj) {
         found_l = s >= 0 && at == key;
-        vidx_l = (uint64_t)leaf * F + (uint32_t)max(s, 0);
+        // The key's slot when found, else its insert position (F when every key precedes
+        // it: only read for a found key).
+        vidx_l = (uint64_t)leaf * F + (uint32_t)(found_l ? s : s + 1);
         leaf_l = leaf;
         par_l = par;
       }
     }
     if (walk && !mine) {
-      // Hints for the upsert kernels: the leaf, its version now (no writer runs during this
-      // kernel) and the level-1 node last used for routing.
+      // Hints for the upsert kernels: the leaf, its version and entry count now (no writer
+      // runs during this kernel), the key's slot in it and the level-1 node last used for
+      // routing.
       uleaf[j] = leaf_l;
-      uver[j] = leaf_l < t.capacity ? t.meta[leaf_l].lock : 1ull;
+      uint64_t ver = 1ull;
+      uint32_t lcount = 0;
+      if (leaf_l < t.capacity) {
+        ver = t.meta[leaf_l].lock;
+        lcount = meta_count(t.meta[leaf_l].info);
+      }
+      uver[j] = ver;
+      upos[j] = (uint32_t)(vidx_l - (uint64_t)leaf_l * F) | (found_l ? UPOS_FOUND : 0u) |
+                (min(lcount, 255u) << UPOS_COUNT_SHIFT);
       upar[j] = par_l;
     }
     // All values of the sub-chunk in one parallel round trip, then the answers.
@@ -1048,22 +1060,24 @@ STM_DEV void flush_counters(const TreeView& t, const TxCounters& c, BatchCounter
 }
 
 // Lane-group fast path: NG = 64 / G merged INSERTs of one wavefront at once, each by G lanes
-// holding its leaf strided (slot e*G + g in element e of lane g, so every load and store
-// instruction covers G consecutive slots).  It is the leaf-only transaction of the wave-wide
-// path (test-and-set of the leaf's lock word, read, update in place or shift-insert, stores
-// acknowledged, unlock with version + 2) run for NG ops in lock-step, started from the leaf the
-// lookup kernel routed the key to.  It never waits for a lock and never splits: an op whose
-// leaf is locked, whose hint is stale (the leaf split since: its fences no longer hold the key)
-// or whose leaf is full is handed back to the wave-wide path.
+// (slot e*G + g of the leaf in element e of lane g, so every load and store instruction covers
+// G consecutive slots).  It is the leaf-only transaction of the wave-wide path (test-and-set
+// of the leaf's lock word, update in place or shift-insert, unlock with version + 2) run for
+// NG ops in lock-step, started from the leaf, slot and entry count the lookup kernel found
+// for the key.  It never waits for a lock and never splits: an op whose leaf is locked, whose
+// hint is stale (the version changed since the lookup) or whose leaf is full is handed back
+// to the wave-wide path.  Only the slots that move are read: for an insert at position p of
+// a leaf with c entries, lane g loads slot x - 1 for each of its slots x in (p, c] and stores
+// it at x; a found key reads and writes its one value.  (Stores of whole 128-byte lines, two
+// slots per lane, measured no faster.)
 // src = round lane of this group's op (-1: none).  Returns the round lanes whose op committed.
 template <int F, int G>
 STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uint64_t v_l,
-                                   uint32_t leaf_l, uint64_t ver_l, uint32_t gs_l, uint32_t np_l,
-                                   uint32_t p0_l, const uint32_t* __restrict__ sidx,
+                                   uint32_t leaf_l, uint64_t ver_l, uint32_t up_l, uint32_t gs_l,
+                                   uint32_t np_l, uint32_t p0_l, const uint32_t* __restrict__ sidx,
                                    uint64_t* __restrict__ out_vals,
                                    uint8_t* __restrict__ out_found, TxCounters& c) {
   constexpr int K = F / G;
-  constexpr int NG = WAVE / G;
   const int lane = lane_id();
   const int g = lane & (G - 1);
   const int gbase = lane - g;
@@ -1074,84 +1088,75 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   const uint32_t gs = (uint32_t)__shfl((int)gs_l, sl, WAVE);
   const uint32_t np = (uint32_t)__shfl((int)np_l, sl, WAVE);
   const uint32_t p0 = (uint32_t)__shfl((int)p0_l, sl, WAVE);
+  const uint32_t up = (uint32_t)__shfl((int)up_l, sl, WAVE);
   const uint64_t ver = shfl64(ver_l, sl);
-  const bool act = src >= 0 && leaf < t.capacity;
+  const uint32_t pos = up & UPOS_SLOT;
+  const bool hfound = (up & UPOS_FOUND) != 0;
+  const uint32_t hcount = up >> UPOS_COUNT_SHIFT;
+  // A hint out of range (never written by the lookup kernel) is not acted on.
+  const bool act = src >= 0 && leaf < t.capacity && hcount <= (uint32_t)F &&
+                   (hfound ? pos < hcount : pos <= hcount);
+  const size_t base = (size_t)(act ? leaf : 0u) * F;
   NodeMeta* m = t.meta + (act ? leaf : 0u);
-  uint64_t* kp = t.keys + (size_t)leaf * F + g;
-  uint64_t* vp = t.vals + (size_t)leaf * F + g;
-  uint64_t lk[K], lv[K];
-  uint64_t info = 0, low = 0, high = 0, word = 0;
+  uint64_t* kp = t.keys + base + g;
+  uint64_t* vp = t.vals + base + g;
+  uint64_t sk[K], sv[K];  // slot e*G + g - 1, for the slots in (pos, hcount]
+  uint64_t info = 0, low = 0, high = 0, word = 0, old = 0;
+#pragma unroll
+  for (int e = 0; e < K; ++e) {
+    sk[e] = 0;
+    sv[e] = 0;
+  }
   // Test-and-set and the content loads go out together.  The content is the leaf's if the
   // test-and-set returns exactly the version the lookup kernel saw (unlocked): every commit
   // changes the version and an aborted writer writes nothing, so no group of this launch has
-  // written the leaf, and memory holds what the previous kernels left there, which plain loads
-  // see across the kernel boundary.  Any other word (locked, or a version a group of this
-  // launch committed) hands the op back to the transactional kernel that runs after this
-  // launch.  So no group here reads keys or values another group here wrote: those stores
-  // need neither device-wide visibility nor ordering before the unlock; the launch boundary
-  // publishes them.  The lock and the count|level word share a line with other groups'
-  // atomics and keep device-coherent stores.
+  // written the leaf, and memory holds what the previous kernels left there (and what the
+  // lookup kernel's hint describes), which plain loads see across the kernel boundary.  Any
+  // other word (locked, or a version a group of this launch committed) hands the op back to
+  // the transactional kernel that runs after this launch.  So no group here reads keys or
+  // values another group here wrote: those stores need neither device-wide visibility nor
+  // ordering before the unlock; the launch boundary publishes them.  The lock and the
+  // count|level word share a line with other groups' atomics and keep device-coherent stores.
   if (act) {
     if (g == 0)
       word = __hip_atomic_fetch_or(&m->lock, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     info = m->info;
     low = m->low;
     high = m->high;
+    if (hfound) {
+      if (g == 0) old = t.vals[base + pos];
+    } else {
 #pragma unroll
-    for (int e = 0; e < K; ++e) {
-      lk[e] = kp[e * G];
-      lv[e] = vp[e * G];
-    }
-  } else {
-#pragma unroll
-    for (int e = 0; e < K; ++e) {
-      lk[e] = KEY_EMPTY;
-      lv[e] = 0;
+      for (int e = 0; e < K; ++e) {
+        const uint32_t x = (uint32_t)(e * G + g);
+        if (x > pos && x <= hcount) {
+          sk[e] = kp[e * G - 1];
+          sv[e] = vp[e * G - 1];
+        }
+      }
     }
   }
   word = shfl64(word, gbase);
+  old = shfl64(old, gbase);
   const bool held = act && !(word & 1ull);
   const bool fresh = held && word == ver;
-  if (!fresh) {
-#pragma unroll
-    for (int e = 0; e < K; ++e) {
-      lk[e] = KEY_EMPTY;
-      lv[e] = 0;
-    }
-    info = 0;
-  }
   const uint32_t count = meta_count(info);
-  const bool fits = fresh && meta_level(info) == 0 && count <= (uint32_t)F && key >= low && key < high;
-  uint32_t le = 0;
-#pragma unroll
-  for (int e = 0; e < K; ++e) le += lk[e] <= key ? 1u : 0u;
-  const int cle = (int)group_sum<G>(le);
-  const int s = cle > 0 ? cle - 1 : 0;
-  const uint64_t at_k = shfl64(select_k<K>(lk, s / G), gbase + s % G);
-  const uint64_t at_v = shfl64(select_k<K>(lv, s / G), gbase + s % G);
-  const bool found = fits && cle > 0 && at_k == key;
-  const bool insert = fits && !found && count < (uint32_t)F;
+  const bool fits = fresh && meta_level(info) == 0 && count == hcount && key >= low && key < high;
+  const bool found = fits && hfound;
+  const bool insert = fits && !hfound && count < (uint32_t)F;
   const bool commit = found || insert;
-  const uint64_t old = found ? at_v : 0;
   if (found) {
-    if (g == s % G) t.vals[(size_t)leaf * F + s] = val;
+    if (g == 0) t.vals[base + pos] = val;
+  } else {
+    old = 0;
   }
-  // Shift insert at pos = cle: slot x takes old[x] below pos, the new pair at pos and
-  // old[x - 1] above; only slots pos .. count change.  old[x - 1] is element e of lane g - 1,
-  // or for g = 0 element e - 1 of lane G - 1.
-  const int pos = cle;
+  // Shift insert at pos: slot x takes slot x - 1 for x in (pos, count], the new pair at pos.
 #pragma unroll
   for (int e = 0; e < K; ++e) {
-    const uint64_t pk_in = shfl64(lk[e], g > 0 ? lane - 1 : lane);
-    const uint64_t pv_in = shfl64(lv[e], g > 0 ? lane - 1 : lane);
-    const uint64_t pk_wr = shfl64(lk[e > 0 ? e - 1 : 0], gbase + G - 1);
-    const uint64_t pv_wr = shfl64(lv[e > 0 ? e - 1 : 0], gbase + G - 1);
-    const int x = e * G + g;
-    if (insert && x >= pos && x <= (int)count) {
-      const uint64_t ok_ = g > 0 ? pk_in : pk_wr;
-      const uint64_t ov_ = g > 0 ? pv_in : pv_wr;
-      kp[e * G] = x == pos ? key : ok_;
-      vp[e * G] = x == pos ? val : ov_;
+    const uint32_t x = (uint32_t)(e * G + g);
+    if (insert && x >= pos && x <= count) {
+      kp[e * G] = x == pos ? key : sk[e];
+      vp[e * G] = x == pos ? val : sv[e];
     }
   }
   if (insert && g == 0) st_relaxed(&m->info, make_info(count + 1, 0));
@@ -1170,7 +1175,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   c.node_reads += (uint32_t)__popcll(ballot(act && g == 0));
   uint64_t done = 0;
 #pragma unroll
-  for (int q = 0; q < NG; ++q) {
+  for (int q = 0; q < WAVE / G; ++q) {
     const int sq = __builtin_amdgcn_readlane(src, q * G);
     if ((leaders >> (q * G)) & 1ull) done |= 1ull << sq;
   }
@@ -1218,6 +1223,7 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
                                                      const uint32_t* __restrict__ ilist,
                                                      const uint32_t* __restrict__ uleaf,
                                                      const uint64_t* __restrict__ uver,
+                                                     const uint32_t* __restrict__ upos,
                                                      BatchCounters* cnt, uint32_t* __restrict__ slist,
                                                      uint32_t* __restrict__ scount,
                                                      uint64_t* __restrict__ out_vals,
@@ -1254,12 +1260,14 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
     const uint32_t np_raw = npend[gi];
     const uint32_t leaf_raw = uleaf[gi];
     const uint64_t ver_raw = uver[gi];
+    const uint32_t up_raw = upos[gi];
     const uint64_t k_l = mine ? k_raw : 0;
     const uint64_t v_l = mine ? v_raw : 0;
     const uint32_t gs_l = mine ? gs_raw : 0;
     const uint32_t np_l = mine ? np_raw : 0;
     const uint32_t leaf_l = mine ? leaf_raw : NODE_NONE;
     const uint64_t ver_l = mine ? ver_raw : 1ull;
+    const uint32_t up_l = mine ? up_raw : 0u;
     // The first answered op's input position, loaded with the op data rather than inside
     // the round (gs_l is a sorted position of the batch).
     const uint32_t sx_raw = sidx[gs_l];
@@ -1273,8 +1281,8 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
         if (l >= 0) rest &= rest - 1;
         if (lane_id() / G == q) my_src = l;
       }
-      const uint64_t done = fast_leaf_upserts<F, G>(t, my_src, k_l, v_l, leaf_l, ver_l, gs_l,
-                                                    np_l, p0_l, sidx, out_vals, out_found, c);
+      const uint64_t done = fast_leaf_upserts<F, G>(t, my_src, k_l, v_l, leaf_l, ver_l, up_l,
+                                                    gs_l, np_l, p0_l, sidx, out_vals, out_found, c);
       upserts += (uint32_t)__popcll(done);
       todo &= ~done;
     }
@@ -1562,19 +1570,19 @@ static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipSt
     k_lookup<F, G, true, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                        a.counters, a.chunk, a.out_vals, a.out_found,
                                                        a.route_hdr, a.fences, a.lids, a.segs, a.desc,
-                                                       a.uleaf, a.uver, a.upar, a.gval, a.fanout,
+                                                       a.uleaf, a.uver, a.upos, a.upar, a.gval, a.fanout,
                                                        a.fanout_cap);
   else if (a.route_hdr)
     k_lookup<F, G, true, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                         a.counters, a.chunk, a.out_vals, a.out_found,
                                                         a.route_hdr, a.fences, a.lids, a.segs, a.desc,
-                                                        a.uleaf, a.uver, a.upar, a.gval, a.fanout,
+                                                        a.uleaf, a.uver, a.upos, a.upar, a.gval, a.fanout,
                                                         a.fanout_cap);
   else
     k_lookup<F, G, false, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                          a.counters, a.chunk, a.out_vals, a.out_found,
                                                          nullptr, nullptr, nullptr, nullptr, a.desc,
-                                                         a.uleaf, a.uver, a.upar, a.gval, a.fanout,
+                                                         a.uleaf, a.uver, a.upos, a.upar, a.gval, a.fanout,
                                                          a.fanout_cap);
   return hipGetLastError();
 }
@@ -1615,7 +1623,7 @@ static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hip
       FAST_WAVES_MAX / 4);
   waves = blocks * 4;  // one scount entry per wavefront
   k_upsert_fast<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx,
-                                                a.ilist, a.uleaf, a.uver, a.counters, a.slist,
+                                                a.ilist, a.uleaf, a.uver, a.upos, a.counters, a.slist,
                                                 a.scount, a.out_vals, a.out_found, a.desc,
                                                 a.fanout, a.fanout_cap, a.gval);
   return hipGetLastError();
@@ -1624,7 +1632,7 @@ static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hip
 template <int F>
 static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
   hipError_t e = hipSuccess;
-  const bool fast = a.uleaf && a.uver && a.ilist && a.slist && a.scount;
+  const bool fast = a.uleaf && a.uver && a.upos && a.ilist && a.slist && a.scount;
   uint32_t fast_waves = 0;
   if (fast) {
     switch (a.lanes) {
