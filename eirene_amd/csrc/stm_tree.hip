@@ -571,6 +571,17 @@ STM_DEV bool tx_read(const TreeView& t, uint32_t id, WNode<F>& n) {
   return n.count <= (uint32_t)F;
 }
 
+// The metadata words of node id (count | level, fences, right sibling), one lane each.
+STM_DEV void tx_write_meta(const TreeView& t, uint32_t id, uint32_t count, uint32_t level,
+                           uint64_t low, uint64_t high, uint32_t next) {
+  NodeMeta* m = t.meta + id;
+  const int l = lane_id();
+  if (l == 1) st_relaxed(&m->info, make_info(count, level));
+  if (l == 2) st_relaxed(&m->low, low);
+  if (l == 3) st_relaxed(&m->high, high);
+  if (l == 4) st_relaxed(&m->next, (uint64_t)next);
+}
+
 template <int F>
 STM_DEV void tx_write_node(const TreeView& t, uint32_t id, const uint64_t (&k)[WNode<F>::SPL],
                            const uint64_t (&v)[WNode<F>::SPL], uint32_t count, uint32_t level,
@@ -585,12 +596,7 @@ STM_DEV void tx_write_node(const TreeView& t, uint32_t id, const uint64_t (&k)[W
       st_relaxed(vp + s, v[j]);
     }
   }
-  NodeMeta* m = t.meta + id;
-  const int l = lane_id();
-  if (l == 1) st_relaxed(&m->info, make_info(count, level));
-  if (l == 2) st_relaxed(&m->low, low);
-  if (l == 3) st_relaxed(&m->high, high);
-  if (l == 4) st_relaxed(&m->next, (uint64_t)next);
+  tx_write_meta(t, id, count, level, low, high, next);
 }
 
 // Try-lock of up to three version words (two nodes and optionally the tree header).
@@ -914,15 +920,17 @@ struct ParentCache {
   bool fresh = false;  // read (version, then content) during the current attempt
 };
 
-// One attempt of the upsert transaction for (key, val).
+// Routing part of an upsert attempt: the leaf that holds key, locked (tx_lock_leaf) into cur,
+// with its parent par when has_par (read with its version during this attempt when
+// par_fresh) and, after a descent from the root, the header version hver.  TX_DONE: the
+// leaf is locked and its fences hold key.  Any other result holds no lock.
 template <int F>
-STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val, bool& found,
-                                   uint64_t& old, TxCounters& c, ParentCache<F>& pc,
-                                   uint32_t par_hint) {
-  WNode<F> par, cur;
-  bool has_par = false;
-  bool par_fresh = false;  // par was read with its version during this attempt
-  uint64_t hver = 0;
+STM_DEV TxResult tx_route_leaf(const TreeView& t, uint64_t key, TxCounters& c, ParentCache<F>& pc,
+                               uint32_t par_hint, WNode<F>& par, WNode<F>& cur, bool& has_par,
+                               bool& par_fresh, uint64_t& hver) {
+  has_par = false;
+  par_fresh = false;
+  hver = 0;
   uint32_t leaf_id;
   pc.fresh = false;
   if (!(pc.valid && key >= pc.n.low && key < pc.n.high) && par_hint < t.capacity) {
@@ -983,20 +991,264 @@ STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val
     pc.valid = false;
     return TX_ABORT;
   }
-  TxResult r = tx_leaf_upsert<F>(t, cur, par, has_par, par_fresh, hver, key, val, found, old, c);
+  return TX_DONE;
+}
+
+// Re-reads the parent of the locked leaf cur with its version before a split (the leaf stays
+// locked); false releases the leaf.
+template <int F>
+STM_DEV bool tx_fresh_parent(const TreeView& t, uint64_t key, const WNode<F>& cur, WNode<F>& par,
+                             TxCounters& c, ParentCache<F>& pc) {
+  c.node_reads++;
+  if (!tx_read<F>(t, par.id, par) || par.level != 1 || key < par.low || key >= par.high) {
+    leaf_release(&t.meta[cur.id].lock, cur.ver);
+    pc.valid = false;
+    return false;
+  }
+  return true;
+}
+
+// One attempt of the upsert transaction for (key, val).
+template <int F>
+STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val, bool& found,
+                                   uint64_t& old, TxCounters& c, ParentCache<F>& pc,
+                                   uint32_t par_hint) {
+  WNode<F> par, cur;
+  bool has_par, par_fresh;
+  uint64_t hver;
+  TxResult r = tx_route_leaf<F>(t, key, c, pc, par_hint, par, cur, has_par, par_fresh, hver);
+  if (r != TX_DONE) return r;
+  r = tx_leaf_upsert<F>(t, cur, par, has_par, par_fresh, hver, key, val, found, old, c);
   if (r == TX_NEED_PARENT) {
-    // The leaf splits: the parent's current version and content are needed (the leaf
-    // stays locked meanwhile).
-    c.node_reads++;
-    if (!tx_read<F>(t, par.id, par) || par.level != 1 || key < par.low || key >= par.high) {
-      leaf_release(&t.meta[leaf_id].lock, cur.ver);
-      pc.valid = false;
-      return TX_ABORT;
-    }
+    // The leaf splits: the parent's current version and content are needed.
+    if (!tx_fresh_parent<F>(t, key, cur, par, c, pc)) return TX_ABORT;
     r = tx_leaf_upsert<F>(t, cur, par, true, true, hver, key, val, found, old, c);
   }
   if (r != TX_DONE || c.split_happened) pc.valid = false;
   c.split_happened = false;
+  return r;
+}
+
+// Merged upsert of the round's ops whose keys lie in the locked leaf n (lane mask R, each
+// lane's key k_l and value v_l; keys are distinct): one transaction writes them all.  Lane j
+// finds its key's place cle (keys <= k_l) by a binary search over the leaf, a new key goes to
+// cle + (new keys of R below it), an old slot s to s + (new keys of R below its key); a key
+// already present takes its new value in place.  With T = count + new keys <= F the leaf is
+// rewritten in place; with F < T <= 2F it splits into T/2 + (T - T/2) entries and one
+// separator goes into the parent, as the single-op split does.  Returns as tx_leaf_upsert;
+// found_l / old_l are each lane's answer.
+template <int F>
+STM_DEV TxResult tx_leaf_merge(const TreeView& t, const WNode<F>& n, const WNode<F>& pn,
+                               bool has_par, bool par_fresh, uint64_t hver, uint64_t key0,
+                               uint64_t R, uint64_t k_l, uint64_t v_l, bool& found_l,
+                               uint64_t& old_l, TxCounters& c) {
+  constexpr int SPL = WNode<F>::SPL;
+  constexpr int STEPS = F <= 32 ? 6 : (F <= 64 ? 7 : 8);  // binary search over <= F keys
+  const int lane = lane_id();
+  const bool mine = (R >> lane) & 1ull;
+  uint64_t* lw = &t.meta[n.id].lock;
+  uint32_t lo = 0, hi = n.count;
+#pragma unroll
+  for (int it = 0; it < STEPS; ++it) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint64_t km = wgather<F>(n.k, (int)mid, KEY_EMPTY);
+    if (lo < hi) {
+      if (km <= k_l) lo = mid + 1;
+      else hi = mid;
+    }
+  }
+  const uint32_t cle = lo;
+  const uint64_t at_k = wgather<F>(n.k, (int)cle - 1, KEY_EMPTY);
+  const uint64_t at_v = wgather<F>(n.v, (int)cle - 1, 0);
+  const bool fnd = mine && cle > 0 && at_k == k_l;
+  const uint64_t newm = ballot(mine && !fnd);
+  const uint32_t ins = (uint32_t)__popcll(newm);
+  const uint32_t T = n.count + ins;
+  // rank of a new key among the new keys, shift of each old slot, new values of found keys
+  uint32_t rnk = 0;
+  uint32_t sh[SPL];
+  uint64_t uv[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    sh[j] = 0;
+    uv[j] = n.v[j];
+  }
+  for (uint64_t rest = R; rest; rest &= rest - 1) {
+    const int i = __builtin_ctzll(rest);
+    const uint64_t ki = readlane64(k_l, i);
+    if ((newm >> i) & 1ull) {
+      rnk += ki < k_l ? 1u : 0u;
+#pragma unroll
+      for (int j = 0; j < SPL; ++j) sh[j] += ki < n.k[j] ? 1u : 0u;
+    } else {
+      const uint64_t vi = readlane64(v_l, i);
+#pragma unroll
+      for (int j = 0; j < SPL; ++j)
+        if (n.k[j] == ki) uv[j] = vi;
+    }
+  }
+  const uint32_t P = cle + rnk;  // final place of this lane's key when it is new
+  found_l = fnd;
+  old_l = fnd ? at_v : 0;
+  uint64_t* kp = t.keys + (size_t)n.id * F;
+  uint64_t* vp = t.vals + (size_t)n.id * F;
+  if (T <= (uint32_t)F) {
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      const uint32_t sl = (uint32_t)(j * WAVE + lane);
+      if (sl < n.count && (sh[j] || uv[j] != n.v[j])) {
+        st_relaxed(kp + sl + sh[j], n.k[j]);
+        st_relaxed(vp + sl + sh[j], uv[j]);
+      }
+    }
+    if (mine && !fnd) {
+      st_relaxed(kp + P, k_l);
+      st_relaxed(vp + P, v_l);
+    }
+    if (ins && lane == 1) st_relaxed(&t.meta[n.id].info, make_info(T, 0));
+    tx_unlock(lw, n.ver, nullptr, 0, nullptr, 0);
+    c.commits++;
+    c.new_keys += ins;
+    return TX_DONE;
+  }
+  // Split: entries at final places [0, h) stay, [h, T) move to a new right sibling.
+  if (has_par && !par_fresh) return TX_NEED_PARENT;
+  const uint32_t h = T / 2;
+  uint32_t ps = 0;
+  if (has_par) {
+    if (pn.count >= (uint32_t)F) {
+      leaf_release(lw, n.ver);
+      return TX_ABORT;
+    }
+    ps = parent_slot<F>(pn, key0, n.id);
+    if (ps == NODE_NONE) {
+      leaf_release(lw, n.ver);
+      return TX_ABORT;
+    }
+  }
+  // the separator: the entry whose final place is h
+  uint64_t cand = 0;
+  bool has = false;
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const uint32_t sl = (uint32_t)(j * WAVE + lane);
+    if (sl < n.count && sl + sh[j] == h) {
+      cand = n.k[j];
+      has = true;
+    }
+  }
+  if (mine && !fnd && P == h) {
+    cand = k_l;
+    has = true;
+  }
+  const uint64_t sep = readlane64(cand, __builtin_ctzll(ballot(has) | (1ull << 63)));
+  uint64_t* pw = has_par ? &t.meta[pn.id].lock : nullptr;
+  uint64_t* hw = has_par ? nullptr : &t.hdr->lock;
+  if (!tx_lock(pw, has_par ? pn.ver : 0, hw, hver, nullptr, 0)) {
+    leaf_release(lw, n.ver);
+    return TX_ABORT;
+  }
+  const uint32_t base = tx_alloc(t, has_par ? 1u : 2u);
+  if (base == NODE_NONE) {
+    tx_unlock(lw, n.ver - 2, pw, has_par ? pn.ver - 2 : 0, hw, hver - 2);
+    return TX_FAIL;
+  }
+  const uint32_t rid = base;
+  const uint32_t qid = has_par ? NODE_NONE : base + 1;
+  tx_claim_new(t, rid, qid);
+  uint64_t* rkp = t.keys + (size_t)rid * F;
+  uint64_t* rvp = t.vals + (size_t)rid * F;
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const uint32_t sl = (uint32_t)(j * WAVE + lane);
+    if (sl < n.count) {
+      const uint32_t to = sl + sh[j];
+      if (to >= h) {
+        st_relaxed(rkp + (to - h), n.k[j]);
+        st_relaxed(rvp + (to - h), uv[j]);
+      } else if (sh[j] || uv[j] != n.v[j]) {
+        st_relaxed(kp + to, n.k[j]);
+        st_relaxed(vp + to, uv[j]);
+      }
+      if (sl >= h) st_relaxed(kp + sl, KEY_EMPTY);  // the left node ends at h
+    }
+    if (sl >= T - h && sl < (uint32_t)F) st_relaxed(rkp + sl, KEY_EMPTY);
+  }
+  if (mine && !fnd) {
+    if (P >= h) {
+      st_relaxed(rkp + (P - h), k_l);
+      st_relaxed(rvp + (P - h), v_l);
+    } else {
+      st_relaxed(kp + P, k_l);
+      st_relaxed(vp + P, v_l);
+    }
+  }
+  tx_write_meta(t, rid, T - h, 0, sep, n.high, n.next);
+  tx_write_meta(t, n.id, h, 0, n.low, sep, rid);
+  if (has_par) {
+    uint64_t pk[SPL], pv[SPL];
+    parent_with_child<F>(pn, ps, sep, rid, pk, pv);
+    tx_write_node<F>(t, pn.id, pk, pv, pn.count + 1, pn.level, pn.low, pn.high, pn.next);
+    tx_unlock(lw, n.ver, pw, pn.ver, &t.meta[rid].lock, 0ull);
+  } else {
+    uint64_t qk[SPL], qv[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      const int sl = j * WAVE + lane;
+      qk[j] = sl == 0 ? n.low : (sl == 1 ? sep : KEY_EMPTY);
+      qv[j] = sl == 0 ? (uint64_t)n.id : (sl == 1 ? (uint64_t)rid : 0);
+    }
+    tx_write_node<F>(t, qid, qk, qv, 2, 1, n.low, n.high, NODE_NONE);
+    if (lane == 0) {
+      st_relaxed(&t.hdr->root, (uint64_t)qid);
+      st_relaxed(&t.hdr->height, 2ull);
+    }
+    tx_unlock(lw, n.ver, &t.meta[rid].lock, 0ull, &t.hdr->lock, hver);
+    if (lane == 0) st_relaxed(&t.meta[qid].lock, 2ull);
+    c.root_splits++;
+  }
+  if (t.rank_of && lane == 0) {
+    const uint32_t r = t.rank_of[n.id];
+    if (r != NODE_NONE) t.dir[r].pflags |= DIR_SPLIT;
+  }
+  c.leaf_splits++;
+  c.commits++;
+  c.new_keys += ins;
+  c.split_happened = true;
+  return TX_DONE;
+}
+
+// One attempt of a merged upsert: routes key0 (the lowest remaining lane's key) to its leaf,
+// locks it and merges every remaining op of the round (todo) whose key the leaf's fences
+// hold, at most F of them.  On TX_DONE, done is the lanes whose ops committed.
+template <int F>
+STM_DEV TxResult tx_merge_attempt(const TreeView& t, uint64_t key0, uint64_t todo, uint64_t k_l,
+                                  uint64_t v_l, TxCounters& c, ParentCache<F>& pc,
+                                  uint32_t par_hint, uint64_t& done, bool& found_l,
+                                  uint64_t& old_l) {
+  WNode<F> par, cur;
+  bool has_par, par_fresh;
+  uint64_t hver;
+  TxResult r = tx_route_leaf<F>(t, key0, c, pc, par_hint, par, cur, has_par, par_fresh, hver);
+  if (r != TX_DONE) return r;
+  uint64_t R = ballot(((todo >> lane_id()) & 1ull) && k_l >= cur.low && k_l < cur.high);
+  if constexpr (F < WAVE) {
+    // at most F ops, so that T <= 2F (the lowest lanes: key0's lane is the lowest of todo)
+    uint64_t keep = 0;
+    for (int q = 0; q < F && R; ++q) {
+      keep |= R & (~R + 1);
+      R &= R - 1;
+    }
+    R = keep;
+  }
+  r = tx_leaf_merge<F>(t, cur, par, has_par, par_fresh, hver, key0, R, k_l, v_l, found_l, old_l, c);
+  if (r == TX_NEED_PARENT) {
+    if (!tx_fresh_parent<F>(t, key0, cur, par, c, pc)) return TX_ABORT;
+    r = tx_leaf_merge<F>(t, cur, par, true, true, hver, key0, R, k_l, v_l, found_l, old_l, c);
+  }
+  if (r != TX_DONE || c.split_happened) pc.valid = false;
+  c.split_happened = false;
+  done = r == TX_DONE ? R : 0;
   return r;
 }
 
@@ -1348,23 +1600,77 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
     const uint32_t gs_l = mine ? gstart[gi] : 0;
     const uint32_t np_l = mine ? npend[gi] : 0;
     const uint32_t par_l = (mine && upar) ? upar[gi] : NODE_NONE;
+    // Ops of the round whose keys share a leaf commit together (tx_merge_attempt): a hot
+    // leaf takes one transaction per round instead of one per op.
     while (todo) {
       const int l = __builtin_ctzll(todo);
-      todo &= todo - 1;
       const uint64_t key = readlane64(k_l, l);
-      const uint64_t val = readlane64(v_l, l);
-      const uint32_t gs = __builtin_amdgcn_readlane(gs_l, l);
-      const uint32_t np = __builtin_amdgcn_readlane(np_l, l);
-      const uint32_t g = __builtin_amdgcn_readlane(gi, l);
       const uint32_t ph = __builtin_amdgcn_readlane(par_l, l);
-      upserts++;
-      if (!run_upsert<F>(t, key, val, gs, np, sidx, out_vals, out_found, max_attempts, c, pc, ph)) {
+      uint32_t aborts = 0, restarts = 0;
+      uint64_t done = 0;
+      bool found_l = false;
+      uint64_t old_l = 0;
+      bool gave_up = false;
+      for (;;) {
+        // The hint is only tried on the first attempt; retries descend from the root.
+        const TxResult r = tx_merge_attempt<F>(t, key, todo, k_l, v_l, c, pc,
+                                               aborts + restarts == 0 ? ph : NODE_NONE, done,
+                                               found_l, old_l);
+        if (r == TX_DONE) break;
+        if (r == TX_FAIL) {  // pool exhausted: error bit raised, the op is dropped
+          done = 1ull << l;
+          found_l = false;
+          old_l = 0;
+          break;
+        }
+        if (r == TX_ABORT) {
+          c.aborts++;
+          if (++aborts >= max_attempts) {
+            gave_up = true;
+            break;
+          }
+          backoff(aborts);
+        } else if (++restarts >= 4 * max_attempts) {  // TX_RESTART after a preemptive split
+          gave_up = true;
+          break;
+        }
+      }
+      if (gave_up) {  // the op of lane l goes to the serial pass
         uint32_t slot = 0;
         if (lane_id() == 0) slot = atomicAdd(&cnt->n_deferred, 1u);
         slot = uniform32(__shfl(slot, 0, WAVE));
+        const uint32_t g = __builtin_amdgcn_readlane(gi, l);
         if (lane_id() == 0) dlist[slot] = g;
         deferred++;
+        upserts++;
+        todo &= ~(1ull << l);
+        continue;
       }
+      // Answers: a lane writes its own group's few waiting ops; a group with many (a hot
+      // key) is written by the whole wavefront.
+      const bool ans = (done >> lane_id()) & 1ull;
+      const bool many = ans && np_l > 8;
+      if (ans && !many) {
+        for (uint32_t p = 0; p < np_l; ++p) {
+          const uint32_t pos = pos_of(sidx[gs_l + p]);
+          out_vals[pos] = old_l;
+          out_found[pos] = found_l ? 1 : 0;
+        }
+      }
+      for (uint64_t bm = ballot(many); bm; bm &= bm - 1) {
+        const int b = __builtin_ctzll(bm);
+        const uint32_t gsb = __builtin_amdgcn_readlane(gs_l, b);
+        const uint32_t npb = __builtin_amdgcn_readlane(np_l, b);
+        const uint64_t ob = readlane64(old_l, b);
+        const uint8_t fb = (uint8_t)__builtin_amdgcn_readlane(found_l ? 1 : 0, b);
+        for (uint32_t p = lane_id(); p < npb; p += WAVE) {
+          const uint32_t pos = pos_of(sidx[gsb + p]);
+          out_vals[pos] = ob;
+          out_found[pos] = fb;
+        }
+      }
+      upserts += (uint32_t)__popcll(done);
+      todo &= ~done;
     }
   }
   flush_counters(t, c, cnt);
