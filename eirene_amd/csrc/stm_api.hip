@@ -99,6 +99,22 @@ __global__ void k_epilogue(const BatchCounters* __restrict__ cnt, const BatchDes
   acc->batches += 1;
 }
 
+// First node of a back half: the front half of this batch may have run beside the back half of
+// a batch that failed, so the failure is checked again here, after every earlier back half
+// (same stream).  A batch after a failure keeps no groups: its lookups and upserts do nothing.
+__global__ void k_gate(const StreamAcc* __restrict__ acc, const TreeHeader* __restrict__ hdr,
+                       BatchDesc* __restrict__ live, BatchCounters* __restrict__ cnt) {
+  if (threadIdx.x != 0) return;
+  if (live->n == 0) return;
+  if (acc->bad || acc->sort_stalls || hdr->error) {
+    live->n = 0;
+    cnt->n_unique = 0;
+    cnt->n_upserts = 0;
+    cnt->n_fanout = 0;
+    cnt->n_deferred = 0;
+  }
+}
+
 // Plain streaming copy of n16 16-byte words: the known-size reference kernel for HBM traffic
 // counters and the measured bandwidth ceiling.  One word per thread over a grid that covers the
 // whole copy: on gfx950 this outruns both a resident grid-stride loop and several words in
@@ -168,8 +184,9 @@ struct Scratch {
     A_(upos, cap);
     A_(fanout, fanout_capacity(cap));
     A_(counters, 1);
+    A_(desc_live, 1);
     if (staging) {
-      A_(desc_table, DESC_TABLE); A_(desc_live, 1); A_(cursor, 1); A_(acc, 1);
+      A_(desc_table, DESC_TABLE); A_(cursor, 1); A_(acc, 1);
       A_(soff, cap); A_(lastins, cap); A_(scan_tiles, scan_tile_words(cap));
     }
 #undef A_
@@ -227,8 +244,18 @@ struct stm_tree {
   uint32_t batches_since_train = 0;
   uint64_t splits_since_train = 0;  // leaf splits counted since the last directory rebuild
   uint64_t dir_leaves = 0;          // leaves of the directory at the last rebuild
-  // one instantiated graph per batch size class (null until first needed)
-  hipGraphExec_t graphs[GRAPH_CLASSES] = {};
+  // Batch streams (stm_run_batches_device) run each batch as two captured halves on two
+  // scratch sets, so the front half (descriptor, sort, fold) of batch i + 1 runs on
+  // stream_front while the back half (lookups, upserts, counters) of batch i runs on `stream`.
+  // One graph per half, size class and scratch set (null until first needed).
+  Scratch scr2;
+  bool scr2_alloc = false;
+  hipStream_t stream_front = nullptr;
+  hipEvent_t ev_front[2] = {};  // front half of the slot's latest batch done
+  hipEvent_t ev_back[2] = {};   // back half of the slot's latest batch done (its scratch is free)
+  hipEvent_t ev_sync = nullptr;
+  hipGraphExec_t graphs_front[GRAPH_CLASSES][2] = {};
+  hipGraphExec_t graphs_back[GRAPH_CLASSES][2] = {};
   // Batches above 2^18 ops sort normalized keys unless exact_left > 0: a batch whose normalized
   // sort had to fall back (long buckets of distinct keys: hot keys with neighbours) makes the
   // next EXACT_HOLD batches use the eight exact passes directly.  graphs_exact: the form the
@@ -346,10 +373,14 @@ int init_empty(stm_tree* t) {
 }
 
 void drop_graphs(stm_tree* t) {
-  for (hipGraphExec_t& g : t->graphs) {
-    if (g) (void)hipGraphExecDestroy(g);
-    g = nullptr;
-  }
+  for (int c = 0; c < GRAPH_CLASSES; ++c)
+    for (int k = 0; k < 2; ++k) {
+      hipGraphExec_t* gs[] = {&t->graphs_front[c][k], &t->graphs_back[c][k]};
+      for (hipGraphExec_t* g : gs) {
+        if (*g) (void)hipGraphExecDestroy(*g);
+        *g = nullptr;
+      }
+    }
 }
 
 int alloc_router(stm_tree* t) {
@@ -444,8 +475,11 @@ struct PipeIn {
   BatchCounters* host_counters;
 };
 
-hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool timed, bool exact) {
-  hipStream_t st = t->stream;
+// Front half of a batch: sort and fold.  Reads only the batch and writes only the scratch and
+// the outputs of ops the fold answers, never the tree, so it may run on `st` beside the back
+// half of the previous batch (on another scratch).
+hipError_t enqueue_front(stm_tree* t, Scratch& s, const PipeIn& in, bool timed, bool exact,
+                         hipStream_t st) {
   hipError_t e;
   if (timed) phase_begin(t, STM_PHASE_SORT);
   e = radix_sort_pairs_desc(in.keys, in.ops, in.desc, in.n, s.skeys_tmp, s.skeys, s.sidx_tmp, s.sidx, 0,
@@ -475,7 +509,14 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
   fa.sort_exact = sort_exact_word(s.hist);
   if ((e = fold_batch(fa, st)) != hipSuccess) return e;
   if (timed) phase_switch(t, STM_PHASE_MERGE, STM_PHASE_LOOKUP);
+  return hipSuccess;
+}
 
+// Back half: lookups, scans, upserts (and the counters' copy to the host when asked), on the
+// tree's stream.
+hipError_t enqueue_back(stm_tree* t, Scratch& s, const PipeIn& in, bool timed) {
+  hipStream_t st = t->stream;
+  hipError_t e;
   ScanArgs sa{};
   if (in.scans) {
     sa.keys = in.keys;
@@ -572,6 +613,12 @@ hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool time
   return hipSuccess;
 }
 
+hipError_t enqueue_pipeline(stm_tree* t, Scratch& s, const PipeIn& in, bool timed, bool exact) {
+  hipError_t e = enqueue_front(t, s, in, timed, exact, t->stream);
+  if (e != hipSuccess) return e;
+  return enqueue_back(t, s, in, timed);
+}
+
 // Learned routing and the static index: (re)build the directory, and retrain, when it is
 // missing or stale.
 // Also after split-heavy batches: once the leaves split since the last rebuild reach
@@ -656,31 +703,68 @@ int class_of(const stm_tree* t, uint32_t n) {
   return GRAPH_CLASSES - 1;  // max_batch <= 2^30 fits class 10
 }
 
-// Captures advance -> pipeline(desc_live, class bound) -> epilogue into an executable graph.
-int capture_class(stm_tree* t, int c) {
-  if (t->graphs[c]) return STM_OK;
-  Scratch& s = t->scr;
-  hipStream_t st = t->stream;
-  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  k_advance<<<1, 64, 0, st>>>(s.desc_table, s.cursor, s.desc_live, s.acc, t->view.hdr);
-  hipError_t e = hipGetLastError();
-  PipeIn in{nullptr, nullptr, nullptr, class_nmax(t, c), nullptr, nullptr, s.desc_live,
-            false, nullptr, nullptr, 0};
-  if (e == hipSuccess) e = enqueue_pipeline(t, s, in, false, t->graphs_exact);
-  if (e == hipSuccess) {
-    k_epilogue<<<1, 64, 0, st>>>(s.counters, s.desc_live, s.acc);
-    e = hipGetLastError();
+// The second scratch set, the front stream and the pipeline's events, made on first use.
+int ensure_pipeline(stm_tree* t) {
+  if (t->scr2_alloc) return STM_OK;
+  hipError_t e = t->scr2.alloc(t->cfg.max_batch, false);
+  if (e == hipSuccess) e = hipStreamCreate(&t->stream_front);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    e = hipEventCreateWithFlags(&t->ev_front[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ev_back[k], hipEventDisableTiming);
   }
-  hipGraph_t g = nullptr;
-  hipError_t e2 = hipStreamEndCapture(st, &g);
-  if (e == hipSuccess) e = e2;
-  if (e == hipSuccess) e = hipGraphInstantiate(&t->graphs[c], g, nullptr, nullptr, 0);
-  if (g) (void)hipGraphDestroy(g);
-  if (e != hipSuccess) {
-    t->graphs[c] = nullptr;
-    return fail(STM_ERR_DEVICE, "graph capture of size class %d: %s", c, hipGetErrorString(e));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ev_sync, hipEventDisableTiming);
+  if (e != hipSuccess) return fail(STM_ERR_NOMEM, "stream pipeline set-up: %s", hipGetErrorString(e));
+  t->scr2_alloc = true;
+  return STM_OK;
+}
+
+Scratch& slot_scratch(stm_tree* t, int k) { return k ? t->scr2 : t->scr; }
+
+// Captures the two halves of size class c on scratch set k:
+//   front: advance (next descriptor of the window into the set's live descriptor) -> sort -> fold
+//   back:  gate -> lookups -> upserts -> epilogue (counters into the stream's accumulator)
+// The descriptor window, its cursor and the accumulator are the first set's.
+int capture_pair(stm_tree* t, int c, int k) {
+  if (t->graphs_front[c][k] && t->graphs_back[c][k]) return STM_OK;
+  Scratch& s0 = t->scr;
+  Scratch& s = slot_scratch(t, k);
+  PipeIn in{nullptr, nullptr, nullptr, class_nmax(t, c), nullptr, nullptr, s.desc_live,
+            false, nullptr, nullptr, 0, nullptr};
+  for (int half = 0; half < 2; ++half) {
+    hipStream_t st = half ? t->stream : t->stream_front;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e;
+    if (half == 0) {
+      k_advance<<<1, 64, 0, st>>>(s0.desc_table, s0.cursor, s.desc_live, s0.acc, t->view.hdr);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = enqueue_front(t, s, in, false, t->graphs_exact, st);
+    } else {
+      k_gate<<<1, 64, 0, st>>>(s0.acc, t->view.hdr, s.desc_live, s.counters);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = enqueue_back(t, s, in, false);
+      if (e == hipSuccess) {
+        k_epilogue<<<1, 64, 0, st>>>(s.counters, s.desc_live, s0.acc);
+        e = hipGetLastError();
+      }
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess) e = e2;
+    hipGraphExec_t& x = half ? t->graphs_back[c][k] : t->graphs_front[c][k];
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      x = nullptr;
+      return fail(STM_ERR_DEVICE, "graph capture of size class %d: %s", c, hipGetErrorString(e));
+    }
   }
   return STM_OK;
+}
+
+int capture_class(stm_tree* t, int c) {
+  int rc = ensure_pipeline(t);
+  for (int k = 0; k < 2 && rc == STM_OK; ++k) rc = capture_pair(t, c, k);
+  return rc;
 }
 
 int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const uint8_t* ops,
@@ -710,23 +794,43 @@ int run_stream(stm_tree* t, const uint64_t* keys, const uint64_t* vals, const ui
     rc = capture_class(t, c);
     if (rc) return rc;
   }
+  // Batch i runs on scratch set i % 2: its front half on stream_front once the back half of
+  // batch i - 2 (the set's previous user) is done, its back half on the tree's stream once its
+  // front half is done.  Back halves run in batch order on one stream, so each sees the tree
+  // after the previous batch; the learned router retrains on that stream between them.
+  hipStream_t sf = t->stream_front;
+  bool used[2] = {false, false};
   HIP_TRY(hipMemsetAsync(s.acc, 0, sizeof(StreamAcc), st));
   for (size_t w0 = 0; w0 < descs.size(); w0 += DESC_TABLE) {
     const size_t w1 = std::min<size_t>(descs.size(), w0 + DESC_TABLE);
-    // Ordered after the replays of the previous window on the same stream.
+    // The window's table and cursor are replaced once every front half of the previous window
+    // has taken its descriptor.
+    HIP_TRY(hipEventRecord(t->ev_sync, sf));
+    HIP_TRY(hipStreamWaitEvent(st, t->ev_sync, 0));
     HIP_TRY(hipMemcpyAsync(s.desc_table, descs.data() + w0, (w1 - w0) * sizeof(BatchDesc),
                            hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemsetAsync(s.cursor, 0, sizeof(uint32_t), st));
+    HIP_TRY(hipEventRecord(t->ev_sync, st));
+    HIP_TRY(hipStreamWaitEvent(sf, t->ev_sync, 0));
     for (size_t i = w0; i < w1; ++i) {
+      const int k = (int)(i & 1);
+      if (used[k]) HIP_TRY(hipStreamWaitEvent(sf, t->ev_back[k], 0));
+      HIP_TRY(hipGraphLaunch(t->graphs_front[cls[i]][k], sf));
+      HIP_TRY(hipEventRecord(t->ev_front[k], sf));
       rc = maybe_retrain(t);
       if (rc) return rc;
-      HIP_TRY(hipGraphLaunch(t->graphs[cls[i]], st));
+      HIP_TRY(hipStreamWaitEvent(st, t->ev_front[k], 0));
+      HIP_TRY(hipGraphLaunch(t->graphs_back[cls[i]][k], st));
+      HIP_TRY(hipEventRecord(t->ev_back[k], st));
+      used[k] = true;
       t->st.batches += 1;
       t->st.ops += descs[i].n;
       t->batches_since_train += 1;
     }
   }
   StreamAcc acc;
+  HIP_TRY(hipEventRecord(t->ev_sync, sf));
+  HIP_TRY(hipStreamWaitEvent(st, t->ev_sync, 0));
   HIP_TRY(hipMemcpyAsync(&acc, s.acc, sizeof(acc), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   t->st.unique_ops += acc.unique;
@@ -1045,8 +1149,16 @@ int stm_destroy(stm_tree* t) {
   if (!t) return STM_OK;
   (void)hipSetDevice(t->cfg.device);
   if (t->stream) (void)hipStreamSynchronize(t->stream);
+  if (t->stream_front) (void)hipStreamSynchronize(t->stream_front);
   drop_graphs(t);
   t->scr.release();
+  if (t->scr2_alloc) t->scr2.release();
+  for (int k = 0; k < 2; ++k) {
+    if (t->ev_front[k]) (void)hipEventDestroy(t->ev_front[k]);
+    if (t->ev_back[k]) (void)hipEventDestroy(t->ev_back[k]);
+  }
+  if (t->ev_sync) (void)hipEventDestroy(t->ev_sync);
+  if (t->stream_front) (void)hipStreamDestroy(t->stream_front);
   void* ps[] = {t->view.keys, t->view.vals, t->view.meta, t->view.hdr, t->view.stats};
   for (void* p : ps)
     if (p) (void)hipFree(p);

@@ -209,7 +209,10 @@ int stm_run_batch_scan_device(stm_tree* tree, const uint64_t* d_keys, const uint
  * batch is captured once per size class (powers of 4 up to max_batch) as a hipGraph that reads
  * its batch from a device-side descriptor table, and replayed: the host reads nothing back
  * until the stream has run (a reserved key or a full node pool is reported at the end, the
- * tree then holds every batch up to and including the failing one).
+ * tree then holds every batch up to and including the failing one; the outputs of the batches
+ * after it are unspecified).  The graphs come in two halves per batch on two scratch sets, so
+ * batch b + 1's sort and fold run while batch b's lookups and upserts do (a second batch
+ * scratch and a second stream are made on first use).
  * Replaces the batch loop of GPU_STM_Tree::gpu_work_main (src/gpu-stm-tree-v38/bpt-gpustm.cpp:77-135:
  * transferData, launchKernelBefore, launchKernel, launchKernelAfter, transferAns per batch). */
 int stm_run_batches_device(stm_tree* tree, const uint64_t* d_keys, const uint64_t* d_vals,
