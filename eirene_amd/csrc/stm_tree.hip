@@ -1313,19 +1313,29 @@ STM_DEV void flush_counters(const TreeView& t, const TxCounters& c, BatchCounter
 
 // Lane-group fast path: NG = 64 / G merged INSERTs of one wavefront at once, each by G lanes
 // (slot e*G + g of the leaf in element e of lane g, so every load and store instruction covers
-// G consecutive slots).  It is the leaf-only transaction of the wave-wide path (test-and-set
-// of the leaf's lock word, update in place or shift-insert, unlock with version + 2) run for
-// NG ops in lock-step, started from the leaf, slot and entry count the lookup kernel found
-// for the key.  It never waits for a lock and never splits: an op whose leaf is locked, whose
-// hint is stale (the version changed since the lookup) or whose leaf is full is handed back
-// to the wave-wide path.  Only the slots that move are read: for an insert at position p of
-// a leaf with c entries, lane g loads slot x - 1 for each of its slots x in (p, c] and stores
-// it at x; a found key reads and writes its one value.  (Stores of whole 128-byte lines, two
-// slots per lane, measured no faster.)
+// G consecutive slots).  It is the leaf-only transaction of the wave-wide path (update in place
+// or shift-insert, commit with version + 2) run for NG ops in lock-step, started from the leaf,
+// slot and entry count the lookup kernel found for the key, and it runs only for the op that
+// owns its leaf in this launch: the first merged INSERT of the leaf in key order.  Groups are
+// sorted by key, so an op owns its leaf when none of the groups between the leaf's low fence and
+// its own key is an INSERT; lane g of the op's lane group reads group gi - 1 - g (key and merged
+// op), and ownership is proven when one of those G keys lies below the fence (or the batch
+// starts).  No other group of the launch then writes the leaf, and no writer ran since the
+// lookup kernel read it, so the hint is exact and no lock is taken.  An op that does not own
+// its leaf (another INSERT of the batch precedes it there, or more than G groups of the leaf do),
+// whose leaf is full or whose hint disagrees with the leaf's version, count or fences is handed
+// back to the wave-wide path, which runs after this launch.  Only the slots that move are read:
+// for an insert at position p of a leaf with c entries, lane g loads slot x - 1 for each of its
+// slots x in (p, c] and stores it at x; a found key reads and writes its one value.  (Measured
+// with the leaf's test-and-set instead of the ownership test: 131 µs per 2^22-op batch, of which
+// the test-and-set took ≈36 µs; stores of whole 128-byte lines, two slots per lane, measured no
+// faster.)
 // src = round lane of this group's op (-1: none).  Returns the round lanes whose op committed.
 template <int F, int G>
 STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uint64_t v_l,
-                                   uint32_t leaf_l, uint64_t ver_l, uint32_t up_l, uint32_t gs_l,
+                                   uint32_t leaf_l, uint64_t ver_l, uint32_t up_l, uint32_t gi_l,
+                                   const uint64_t* __restrict__ ukey,
+                                   const uint8_t* __restrict__ uop, uint32_t gs_l,
                                    uint32_t np_l, uint32_t p0_l, const uint32_t* __restrict__ sidx,
                                    uint64_t* __restrict__ out_vals,
                                    uint8_t* __restrict__ out_found, TxCounters& c) {
@@ -1341,6 +1351,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   const uint32_t np = (uint32_t)__shfl((int)np_l, sl, WAVE);
   const uint32_t p0 = (uint32_t)__shfl((int)p0_l, sl, WAVE);
   const uint32_t up = (uint32_t)__shfl((int)up_l, sl, WAVE);
+  const uint32_t gi = (uint32_t)__shfl((int)gi_l, sl, WAVE);
   const uint64_t ver = shfl64(ver_l, sl);
   const uint32_t pos = up & UPOS_SLOT;
   const bool hfound = (up & UPOS_FOUND) != 0;
@@ -1359,22 +1370,19 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
     sk[e] = 0;
     sv[e] = 0;
   }
-  // Test-and-set and the content loads go out together.  The content is the leaf's if the
-  // test-and-set returns exactly the version the lookup kernel saw (unlocked): every commit
-  // changes the version and an aborted writer writes nothing, so no group of this launch has
-  // written the leaf, and memory holds what the previous kernels left there (and what the
-  // lookup kernel's hint describes), which plain loads see across the kernel boundary.  Any
-  // other word (locked, or a version a group of this launch committed) hands the op back to
-  // the transactional kernel that runs after this launch.  So no group here reads keys or
-  // values another group here wrote: those stores need neither device-wide visibility nor
-  // ordering before the unlock; the launch boundary publishes them.  The lock and the
-  // count|level word share a line with other groups' atomics and keep device-coherent stores.
+  // The leaf's metadata line, the slots that move and the G groups before the op go out together.
+  const bool has_prev = act && (uint32_t)g < gi;
+  uint64_t pkey = 0;
+  uint8_t pop = UOP_SKIP;
   if (act) {
-    if (g == 0)
-      word = __hip_atomic_fetch_or(&m->lock, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    word = m->lock;
     info = m->info;
     low = m->low;
     high = m->high;
+    if (has_prev) {
+      pkey = ukey[gi - 1 - (uint32_t)g];
+      pop = uop[gi - 1 - (uint32_t)g];
+    }
     if (hfound) {
       if (g == 0) old = t.vals[base + pos];
     } else {
@@ -1388,9 +1396,13 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
       }
     }
   }
-  word = shfl64(word, gbase);
   old = shfl64(old, gbase);
-  const bool held = act && !(word & 1ull);
+  // Keys fall as g grows, so the groups inside the leaf's range are a prefix of the G read.
+  const bool in_leaf = has_prev && pkey >= low;
+  const uint64_t gmask = (G == WAVE ? ~0ull : ((1ull << G) - 1)) << gbase;
+  const uint64_t inside = ballot(in_leaf) & gmask;
+  const bool owner = (ballot(in_leaf && pop == OP_INSERT) & gmask) == 0 && inside != gmask;
+  const bool held = act && owner && !(word & 1ull);
   const bool fresh = held && word == ver;
   const uint32_t count = meta_count(info);
   const bool fits = fresh && meta_level(info) == 0 && count == hcount && key >= low && key < high;
@@ -1412,7 +1424,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
     }
   }
   if (insert && g == 0) st_relaxed(&m->info, make_info(count + 1, 0));
-  if (held && g == 0) st_relaxed(&m->lock, commit ? word + 2 : word);
+  if (commit && g == 0) st_relaxed(&m->lock, word + 2);
   if (commit) {
     for (uint32_t p = (uint32_t)g; p < np; p += G) {
       const uint32_t pos_o = p == 0 ? p0 : pos_of(sidx[gs + p]);
@@ -1469,6 +1481,7 @@ STM_DEV void fanout_answers(const uint64_t* __restrict__ fanout, uint64_t cap,
 template <int F, int G>
 __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t* __restrict__ ukey,
                                                      const uint64_t* __restrict__ uval,
+                                                     const uint8_t* __restrict__ uop,
                                                      const uint32_t* __restrict__ gstart,
                                                      const uint32_t* __restrict__ npend,
                                                      const uint32_t* __restrict__ sidx,
@@ -1533,8 +1546,9 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
         if (l >= 0) rest &= rest - 1;
         if (lane_id() / G == q) my_src = l;
       }
-      const uint64_t done = fast_leaf_upserts<F, G>(t, my_src, k_l, v_l, leaf_l, ver_l, up_l,
-                                                    gs_l, np_l, p0_l, sidx, out_vals, out_found, c);
+      const uint64_t done = fast_leaf_upserts<F, G>(t, my_src, k_l, v_l, leaf_l, ver_l, up_l, gi,
+                                                    ukey, uop, gs_l, np_l, p0_l, sidx, out_vals,
+                                                    out_found, c);
       upserts += (uint32_t)__popcll(done);
       todo &= ~done;
     }
@@ -1928,7 +1942,7 @@ static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hip
       std::min<uint32_t>(a.blocks, (uint32_t)resident.load(std::memory_order_relaxed)),
       FAST_WAVES_MAX / 4);
   waves = blocks * 4;  // one scount entry per wavefront
-  k_upsert_fast<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.gstart, a.npend, a.sidx,
+  k_upsert_fast<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
                                                 a.ilist, a.uleaf, a.uver, a.upos, a.counters, a.slist,
                                                 a.scount, a.out_vals, a.out_found, a.desc,
                                                 a.fanout, a.fanout_cap, a.gval);
