@@ -1220,17 +1220,35 @@ STM_DEV TxResult tx_leaf_merge(const TreeView& t, const WNode<F>& n, const WNode
 
 // One attempt of a merged upsert: routes key0 (the lowest remaining lane's key) to its leaf,
 // locks it and merges every remaining op of the round (todo) whose key the leaf's fences
-// hold, at most F of them.  On TX_DONE, done is the lanes whose ops committed.
+// hold, at most F of them.  On TX_DONE, done is the lanes whose ops committed.  With a leaf
+// hint (the leaf the lookup kernel found) and its level-1 hint the leaf is locked directly:
+// fast-path commits never split, so the leaf still holds the key unless a transaction of this
+// launch split it, which its fences show; a split then reads the hinted parent with its version
+// and checks that it points at the leaf.
 template <int F>
 STM_DEV TxResult tx_merge_attempt(const TreeView& t, uint64_t key0, uint64_t todo, uint64_t k_l,
                                   uint64_t v_l, TxCounters& c, ParentCache<F>& pc,
-                                  uint32_t par_hint, uint64_t& done, bool& found_l,
-                                  uint64_t& old_l) {
+                                  uint32_t par_hint, uint32_t leaf_hint, uint64_t& done,
+                                  bool& found_l, uint64_t& old_l) {
   WNode<F> par, cur;
-  bool has_par, par_fresh;
-  uint64_t hver;
-  TxResult r = tx_route_leaf<F>(t, key0, c, pc, par_hint, par, cur, has_par, par_fresh, hver);
-  if (r != TX_DONE) return r;
+  bool has_par = false, par_fresh = false, direct = false;
+  uint64_t hver = 0;
+  TxResult r;
+  if (leaf_hint < t.capacity && par_hint < t.capacity) {
+    c.node_reads++;
+    if (!tx_lock_leaf<F>(t, leaf_hint, cur)) return TX_ABORT;  // locked by another transaction
+    if (cur.level == 0 && cur.count <= (uint32_t)F && key0 >= cur.low && key0 < cur.high) {
+      direct = true;
+      has_par = true;
+      par.id = par_hint;
+    } else {
+      leaf_release(&t.meta[leaf_hint].lock, cur.ver);
+    }
+  }
+  if (!direct) {
+    r = tx_route_leaf<F>(t, key0, c, pc, par_hint, par, cur, has_par, par_fresh, hver);
+    if (r != TX_DONE) return r;
+  }
   uint64_t R = ballot(((todo >> lane_id()) & 1ull) && k_l >= cur.low && k_l < cur.high);
   if constexpr (F < WAVE) {
     // at most F ops, so that T <= 2F (the lowest lanes: key0's lane is the lowest of todo)
@@ -1578,6 +1596,7 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
                                                 const uint32_t* __restrict__ scount,
                                                 uint32_t fast_waves,
                                                 const uint32_t* __restrict__ upar,
+                                                const uint32_t* __restrict__ uleaf,
                                                 BatchCounters* cnt, uint32_t* __restrict__ dlist,
                                                 uint32_t max_attempts,
                                                 uint64_t* __restrict__ out_vals,
@@ -1592,20 +1611,40 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
   }
   const uint32_t nw = gridDim.x * (blockDim.x / WAVE);
   const uint32_t w = uniform32((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-  const uint32_t per = (U + nw - 1) / nw;
-  const uint32_t g0 = min(w * per, U);
-  const uint32_t g1 = min(g0 + per, U);
   TxCounters c;
   ParentCache<F> pc;
   uint32_t upserts = 0, deferred = 0;
-  for (uint32_t base = g0; base < g1; base += WAVE) {
+  // The wavefront's share, as ranges [s0, s1) of the group index (no slist) or of slist.
+  // Without slist: an equal contiguous part of the groups.  With slist: fast-path wavefront f
+  // handed back scount[f] ops at slist[f * per_f ...]; with r = nw / fast_waves wavefronts per
+  // such range each takes a contiguous r-th of it (so the hand-backs, which sit at the start of
+  // each range, spread over all wavefronts), with fewer wavefronts than ranges wavefront w takes
+  // ranges w, w + nw, ...
+  for (uint32_t f = w;; f += nw) {
+    uint32_t s0, s1;
+    if (!slist) {
+      if (f != w) break;
+      const uint32_t per = (U + nw - 1) / nw;
+      s0 = min(w * per, U);
+      s1 = min(s0 + per, U);
+    } else if (nw >= fast_waves) {
+      if (f != w) break;
+      const uint32_t r = nw / fast_waves;
+      const uint32_t fi = w / r, q = w % r;
+      if (fi >= fast_waves) break;
+      const uint32_t n_f = uniform32(scount[fi]);
+      const uint32_t chunk = (n_f + r - 1) / r;
+      s0 = fi * per_f + min(q * chunk, n_f);
+      s1 = fi * per_f + min((q + 1) * chunk, n_f);
+    } else {
+      if (f >= fast_waves) break;
+      s0 = f * per_f;
+      s1 = s0 + uniform32(scount[f]);
+    }
+  for (uint32_t base = s0; base < s1; base += WAVE) {
     // Op data of the round's upserts, loaded together and read lane by lane.
     const uint32_t e = base + lane_id();
-    bool in = e < g1;
-    if (slist && in) {
-      const uint32_t f = e / per_f;
-      in = e - f * per_f < scount[f];  // within what fast-path wavefront f handed back
-    }
+    const bool in = e < s1;
     const uint32_t gi = in ? (slist ? slist[e] : e) : 0;
     const bool mine = in && (slist || uop[gi] == OP_INSERT);
     uint64_t todo = ballot(mine);
@@ -1614,12 +1653,14 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
     const uint32_t gs_l = mine ? gstart[gi] : 0;
     const uint32_t np_l = mine ? npend[gi] : 0;
     const uint32_t par_l = (mine && upar) ? upar[gi] : NODE_NONE;
+    const uint32_t leaf_l = (mine && uleaf) ? uleaf[gi] : NODE_NONE;
     // Ops of the round whose keys share a leaf commit together (tx_merge_attempt): a hot
     // leaf takes one transaction per round instead of one per op.
     while (todo) {
       const int l = __builtin_ctzll(todo);
       const uint64_t key = readlane64(k_l, l);
       const uint32_t ph = __builtin_amdgcn_readlane(par_l, l);
+      const uint32_t lh = __builtin_amdgcn_readlane(leaf_l, l);
       uint32_t aborts = 0, restarts = 0;
       uint64_t done = 0;
       bool found_l = false;
@@ -1627,9 +1668,10 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
       bool gave_up = false;
       for (;;) {
         // The hint is only tried on the first attempt; retries descend from the root.
+        const bool first = aborts + restarts == 0;
         const TxResult r = tx_merge_attempt<F>(t, key, todo, k_l, v_l, c, pc,
-                                               aborts + restarts == 0 ? ph : NODE_NONE, done,
-                                               found_l, old_l);
+                                               first ? ph : NODE_NONE, first ? lh : NODE_NONE,
+                                               done, found_l, old_l);
         if (r == TX_DONE) break;
         if (r == TX_FAIL) {  // pool exhausted: error bit raised, the op is dropped
           done = 1ull << l;
@@ -1687,6 +1729,7 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
       todo &= ~done;
     }
   }
+  }  // ranges
   flush_counters(t, c, cnt);
   if (lane_id() == 0) {
     stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
@@ -1965,7 +2008,7 @@ static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStr
   }
   k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
                                         fast ? a.slist : nullptr, a.scount, fast_waves, a.upar,
-                                        a.counters, a.dlist, a.max_attempts, a.out_vals,
+                                        a.uleaf, a.counters, a.dlist, a.max_attempts, a.out_vals,
                                         a.out_found, a.desc);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
