@@ -223,43 +223,87 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Acc
     Acc total;
     // State before this wavefront's chunk (block_wave_prefix syncs s_nins too).
     Acc carry = acc_combine(tile_prefix[b], block_wave_prefix(chunk_acc(a, ch), s_w, total));
+    // The rounds in halves of HALF, each in three phases, so that the gathers of a half share
+    // two round trips: the vector memory counter also counts stores, so a gather after a
+    // round's output stores would wait for them, and one gather chain per round made the
+    // kernel latency-bound.  (All eight rounds at once needed 138 registers.)
+    constexpr int HALF = FOLD_ROUNDS / 2;
+    constexpr uint32_t F_VALID = 1, F_INS = 2, F_END = 4;
 #pragma unroll
-    for (int r = 0; r < FOLD_ROUNDS; ++r) {
+    for (int h = 0; h < FOLD_ROUNDS; h += HALF) {
+    // Phase 1, registers only: each round's flags, group index, head and latest earlier insert.
+    uint32_t fl[HALF], gg[HALF];
+    int32_t hd[HALF], lb[HALF];
+#pragma unroll
+    for (int q = 0; q < HALF; ++q) {
+      const int r = h + q;
       bool valid, is_head, is_ins;
       uint64_t next;
       round_flags(a, ch, r, valid, is_head, is_ins, next);
       const uint32_t rb = ch.c0 + (uint32_t)r * WAVE;
-      const int32_t i = (int32_t)(rb + lane_id());
+      const uint32_t i = rb + lane_id();
       const uint64_t hm = ballot(is_head), im = ballot(is_ins);
       // This lane's prefix: heads up to and including it, its head, the last insert before it.
       const uint32_t c_incl = carry.c + (uint32_t)__popcll(hm & le);
-      const int32_t head = last_set(hm & le, rb, carry.head);
+      hd[q] = last_set(hm & le, rb, carry.head);
       const int32_t ins_before = last_set(im & lt, rb, carry.ins);
+      lb[q] = ins_before >= hd[q] ? ins_before : -1;  // latest earlier insert of the key
+      gg[q] = c_incl - 1;
+      const bool group_end = valid && (i + 1 == a.n || next != ch.k[r]);
+      fl[q] = (valid ? F_VALID : 0u) | (is_ins ? F_INS : 0u) | (group_end ? F_END : 0u);
+      carry.c += (uint32_t)__popcll(hm);
+      carry.head = last_set(hm, rb, carry.head);
+      carry.ins = last_set(im, rb, carry.ins);
+    }
+    // Phase 2: input positions of the earlier inserts, then the values.  va = the latest
+    // earlier insert's value (an op's answer, or the merged value of a group ending in a
+    // non-insert), vb = the op's own value (the merged value of a group ending in an insert).
+    uint32_t ia[HALF];
+#pragma unroll
+    for (int q = 0; q < HALF; ++q)
+      ia[q] = ((fl[q] & F_VALID) && lb[q] >= 0) ? a.sidx[lb[q]] : 0u;
+    uint64_t va[HALF], vb[HALF];
+#pragma unroll
+    for (int q = 0; q < HALF; ++q) {
+      const int r = h + q;
+      const bool valid = fl[q] & F_VALID, is_ins = fl[q] & F_INS, end = fl[q] & F_END;
+      const bool real = ch.k[r] != KEY_EMPTY;
+      const bool need_a = valid && lb[q] >= 0 && (a.out_vals || (end && !is_ins && real));
+      const bool need_b = valid && end && is_ins && real;
+      va[q] = need_a ? a.vals[pos_of(ia[q])] : 0ull;
+      vb[q] = need_b ? a.vals[pos_of(ch.sx[r])] : 0ull;
+    }
+    // Phase 3: the outputs and the tile's insert list.
+#pragma unroll
+    for (int q = 0; q < HALF; ++q) {
+      const int r = h + q;
+      const bool valid = fl[q] & F_VALID, is_ins = fl[q] & F_INS, group_end = fl[q] & F_END;
+      const int32_t i = (int32_t)(ch.c0 + (uint32_t)r * WAVE + lane_id());
+      const int32_t head = hd[q];
+      const int32_t last_before = lb[q];
+      const uint32_t g = gg[q];
       bool append = false;
-      uint32_t g = c_incl - 1;
       if (valid) {
         const uint64_t key = ch.k[r];
-        const int32_t last_before = ins_before >= head ? ins_before : -1;  // latest earlier insert
         if (a.lastins) a.lastins[i] = is_ins ? i : last_before;
         if (a.out_vals && last_before >= 0) {
           const uint32_t pos = pos_of(ch.sx[r]);
-          a.out_vals[pos] = a.vals[pos_of(a.sidx[last_before])];
+          a.out_vals[pos] = va[q];
           a.out_found[pos] = 1;
         }
         if (a.gstart && is_ins && last_before < 0) a.npend[g] = (uint32_t)(i - head + 1);
-        const int32_t ins_incl = is_ins ? i : ins_before;
-        const bool group_end = (uint32_t)i + 1 == a.n || next != key;
         if (group_end) {
           a.ukey[g] = key;
           if (a.gstart) a.gstart[g] = (uint32_t)head;
-          const int32_t last_in = ins_incl >= head ? ins_incl : -1;
+          // the group's last insert: this op if it is one, else the latest earlier one
+          const int32_t last_in = is_ins ? i : last_before;
           uint8_t op;
           if (key == KEY_EMPTY) {
             op = UOP_SKIP;
             atomicAdd(&a.counters->n_bad, (uint32_t)(i - head + 1));
           } else if (last_in >= 0) {
             op = OP_INSERT;
-            a.uval[g] = a.vals[pos_of(a.sidx[last_in])];
+            a.uval[g] = is_ins ? vb[q] : va[q];
             append = true;
           } else {
             op = OP_GET;
@@ -268,15 +312,13 @@ __global__ __launch_bounds__(FOLD_BLOCK) void k_fold_apply(FoldArgs a, const Acc
           a.uop[g] = op;
         }
       }
-      carry.c += (uint32_t)__popcll(hm);
-      carry.head = last_set(hm, rb, carry.head);
-      carry.ins = last_set(im, rb, carry.ins);
       const uint64_t want = ballot(append);
       uint32_t wb = 0;
       if (lane_id() == 0 && want) wb = atomicAdd(&s_nins, (uint32_t)__popcll(want));  // LDS
       wb = (uint32_t)__shfl((int)wb, 0, WAVE);
       if (append) s_ins[wb + (uint32_t)__popcll(want & lt)] = g;
     }
+    }  // halves
     // One counter update per tile (a hot atomic per wavefront round would serialize the
     // kernel on one L2 channel), then the tile's INSERT groups go to the batch's list.
     __syncthreads();

@@ -80,12 +80,17 @@ constexpr uint32_t CTL_WRAPPED = CTL_EPOCH + 32;
 constexpr uint32_t CTL_STALL = CTL_WRAPPED + 32;
 constexpr uint32_t CTL_NEED_EXACT = CTL_STALL + 32;  // normalized sort: a long bucket holds distinct keys
 constexpr uint32_t CTL_BARRIER = CTL_NEED_EXACT + 32;  // grid-barrier arrivals of k_sort_exact
-constexpr uint32_t CTL_RANGE_DONE = CTL_BARRIER + 32;  // k_range workgroups finished (mod RANGE_BLOCKS)
-constexpr uint32_t CTL_NORM = CTL_RANGE_DONE + 32;     // u64 kmin, then u32 nshift (k_range's last workgroup)
+constexpr uint32_t CTL_NORM = CTL_BARRIER + 32;  // u64 kmin, then u32 nshift (first k_hist pass, workgroup 0)
 constexpr uint32_t GOFF_OFF = (CTL_NORM + 32 + 63) / 64 * 64;       // goff[MAX_PASSES][RADIX]
 constexpr uint32_t PART_OFF = GOFF_OFF + MAX_PASSES * RADIX;        // [HIST_GRID_CAP][MAX_PASSES][RADIX]
 constexpr uint32_t STATUS_OFF = PART_OFF + HIST_GRID_CAP * MAX_PASSES * RADIX;  // u64 [OS_MAX_TILES][RADIX]
-constexpr uint32_t RANGE_BLOCKS = 256;       // k_range workgroups: one (min, max) partial each
+// k_range: RANGE_BLOCKS workgroups of RANGE_THREADS, one (min, max) partial each (two rounds of
+// eight loads per thread at 2^22 keys).  The partials are reduced by every workgroup of the
+// first histogram pass: a last-workgroup reduction inside k_range (an agent-scope release and
+// an arrival atomic per workgroup, an acquire in the last) took 16 µs per 32 MiB of keys.
+constexpr uint32_t RANGE_BLOCKS = 256;
+constexpr uint32_t RANGE_THREADS = 1024;
+constexpr int RANGE_WAVES = RANGE_THREADS / WAVE;
 constexpr uint32_t RANGE_OFF = STATUS_OFF + 2 * OS_MAX_TILES * RADIX;  // u64 [RANGE_BLOCKS][2]
 constexpr uint32_t ONESWEEP_WORDS = RANGE_OFF + 4 * RANGE_BLOCKS;
 static_assert(STATUS_OFF % 2 == 0 && RANGE_OFF % 2 == 0 && ONESWEEP_WORDS % 64 == 0, "aligned regions");
@@ -134,7 +139,7 @@ struct NormKey {
   uint32_t shift;
 };
 
-// The normalization k_range's last workgroup stored for this sort.
+// The normalization the first histogram pass stored for this sort.
 STM_DEV NormKey norm_params(const uint32_t* words) {
   NormKey k;
   k.kmin = *reinterpret_cast<const uint64_t*>(words + CTL_NORM);
@@ -144,6 +149,40 @@ STM_DEV NormKey norm_params(const uint32_t* words) {
 
 STM_DEV uint32_t norm_of(uint64_t key, const NormKey& k) {
   return (uint32_t)((key - k.kmin) >> k.shift);
+}
+
+// The sort's normalization from k_range's per-workgroup (min, max) partials, reduced by a whole
+// SORT_BLOCK workgroup (every workgroup gets the same result).  NB = 8 * npass bits:
+// nshift = max(0, bitlen(kmax - kmin) - NB).
+STM_DEV NormKey norm_from_partials(const uint32_t* words, int npass) {
+  static_assert(RANGE_BLOCKS == SORT_BLOCK, "one partial per thread");
+  __shared__ uint64_t s_mm[2][SORT_WAVES];
+  const uint64_t* range = reinterpret_cast<const uint64_t*>(words + RANGE_OFF);
+  uint64_t mn = range[2 * threadIdx.x], mx = range[2 * threadIdx.x + 1];
+#pragma unroll
+  for (int off = WAVE / 2; off >= 1; off >>= 1) {
+    const uint64_t a = shfl64(mn, lane_id() ^ off), b = shfl64(mx, lane_id() ^ off);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  const int wave = threadIdx.x / WAVE;
+  if (lane_id() == 0) {
+    s_mm[0][wave] = mn;
+    s_mm[1][wave] = mx;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < SORT_WAVES; ++w) {
+    mn = s_mm[0][w] < mn ? s_mm[0][w] : mn;
+    mx = s_mm[1][w] > mx ? s_mm[1][w] : mx;
+  }
+  __syncthreads();
+  const uint64_t r = mx >= mn ? mx - mn : 0ull;
+  const uint32_t bl = bitlen64(r);
+  NormKey k;
+  k.kmin = mx >= mn ? mn : 0ull;
+  k.shift = bl > (uint32_t)(8 * npass) ? bl - (uint32_t)(8 * npass) : 0u;
+  return k;
 }
 
 struct SortRange {
@@ -174,7 +213,19 @@ STM_DEV void hist_body(const uint64_t* __restrict__ keys, const BatchDesc* __res
   const SortRange rg = sort_range(desc, n_host, keys, first_pass);
   uint64_t kmin = 0;
   if (norm) {
-    const NormKey nk = norm_params(norm);
+    // The first pass reduces k_range's partials itself (no cross-workgroup step inside
+    // k_range), and workgroup 0 stores the result for the later kernels of the sort.
+    NormKey nk;
+    if (first_pass) {
+      nk = norm_from_partials(norm, npass);
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t* w = const_cast<uint32_t*>(norm);
+        *reinterpret_cast<uint64_t*>(w + CTL_NORM) = nk.kmin;
+        w[CTL_NORM + 2] = nk.shift;
+      }
+    } else {
+      nk = norm_params(norm);
+    }
     kmin = nk.kmin;
     shift += (int)nk.shift;
   }
@@ -185,10 +236,18 @@ STM_DEV void hist_body(const uint64_t* __restrict__ keys, const BatchDesc* __res
     for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_h[0][0])[i] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)b * TILE;
-#pragma unroll 4
+    // Every key of the thread in one round of loads (the index clamped into the batch, the
+    // clamped keys not counted), then the LDS counts.
+    uint64_t kk[ITEMS];
+#pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
       const uint64_t e = base + (uint64_t)r * SORT_BLOCK + threadIdx.x;
-      if (e < rg.n) atomicAdd(&s_h[wave][digit_of(gload(kp + e) - kmin, shift)], 1u);
+      kk[r] = gload(kp + (e < rg.n ? e : (uint64_t)rg.n - 1));
+    }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint64_t e = base + (uint64_t)r * SORT_BLOCK + threadIdx.x;
+      if (e < rg.n) atomicAdd(&s_h[wave][digit_of(kk[r] - kmin, shift)], 1u);
     }
     __syncthreads();
     const int d = threadIdx.x;  // SORT_BLOCK == RADIX
@@ -751,15 +810,13 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
 // is already the sorted order when all its keys are equal (the ops of a hot key); if one holds
 // distinct keys, k_fix raises CTL_NEED_EXACT and k_sort_exact, one persistent launch that
 // otherwise returns at once, re-sorts the whole batch with the eight exact 8-bit passes.
-// Minimum and maximum of the batch's keys, one partial per workgroup.  Workgroup 0 also opens
-// the sort: the stall word, the exact-sort request and the barrier counter start at zero.
-__global__ __launch_bounds__(SORT_BLOCK) void k_range(const uint64_t* __restrict__ keys,
-                                                      const BatchDesc* __restrict__ desc,
-                                                      uint32_t n_host, uint32_t* __restrict__ words,
-                                                      int npass) {
-  static_assert(SORT_BLOCK == RANGE_BLOCKS, "the last workgroup reduces one partial per thread");
-  __shared__ uint64_t s_mm[2][SORT_WAVES];
-  __shared__ uint32_t s_last;
+// Minimum and maximum of the batch's keys, one partial per workgroup; the first histogram pass
+// reduces the partials into the normalization.  Workgroup 0 also opens the sort: the stall
+// word, the exact-sort request and the barrier counter start at zero.
+__global__ __launch_bounds__(RANGE_THREADS) void k_range(const uint64_t* __restrict__ keys,
+                                                         const BatchDesc* __restrict__ desc,
+                                                         uint32_t n_host, uint32_t* __restrict__ words) {
+  __shared__ uint64_t s_mm[2][RANGE_WAVES];
   const SortRange rg = sort_range(desc, n_host, keys, 1);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     words[CTL_STALL] = 0;
@@ -767,8 +824,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_range(const uint64_t* __restrict
     words[CTL_BARRIER] = 0;
   }
   uint64_t mn = ~0ull, mx = 0ull;
-  const uint64_t stride = (uint64_t)gridDim.x * SORT_BLOCK;
-  uint64_t i = (uint64_t)blockIdx.x * SORT_BLOCK + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * RANGE_THREADS;
+  uint64_t i = (uint64_t)blockIdx.x * RANGE_THREADS + threadIdx.x;
   // Eight loads in flight per thread.
   for (; i + 7 * stride < rg.n; i += 8 * stride) {
     uint64_t k[8];
@@ -798,47 +855,13 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_range(const uint64_t* __restrict
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < SORT_WAVES; ++w) {
+    for (int w = 1; w < RANGE_WAVES; ++w) {
       mn = s_mm[0][w] < mn ? s_mm[0][w] : mn;
       mx = s_mm[1][w] > mx ? s_mm[1][w] : mx;
     }
     uint64_t* range = reinterpret_cast<uint64_t*>(words + RANGE_OFF);
     range[2 * blockIdx.x] = mn;
     range[2 * blockIdx.x + 1] = mx;
-    // The workgroup that finishes last (arrival count RANGE_BLOCKS - 1 modulo RANGE_BLOCKS: the
-    // counter is never reset) reduces the partials into the sort's normalization.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const uint32_t done = __hip_atomic_fetch_add(words + CTL_RANGE_DONE, 1u, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (done % RANGE_BLOCKS) == RANGE_BLOCKS - 1 ? 1u : 0u;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  const uint64_t* range = reinterpret_cast<const uint64_t*>(words + RANGE_OFF);
-  mn = range[2 * threadIdx.x];
-  mx = range[2 * threadIdx.x + 1];
-#pragma unroll
-  for (int off = WAVE / 2; off >= 1; off >>= 1) {
-    const uint64_t a = shfl64(mn, lane_id() ^ off), b = shfl64(mx, lane_id() ^ off);
-    mn = a < mn ? a : mn;
-    mx = b > mx ? b : mx;
-  }
-  __syncthreads();
-  if (lane_id() == 0) {
-    s_mm[0][wave] = mn;
-    s_mm[1][wave] = mx;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < SORT_WAVES; ++w) {
-      mn = s_mm[0][w] < mn ? s_mm[0][w] : mn;
-      mx = s_mm[1][w] > mx ? s_mm[1][w] : mx;
-    }
-    const uint64_t r = mx >= mn ? mx - mn : 0ull;
-    const uint32_t bl = bitlen64(r);
-    *reinterpret_cast<uint64_t*>(words + CTL_NORM) = mx >= mn ? mn : 0ull;
-    words[CTL_NORM + 2] = bl > (uint32_t)(8 * npass) ? bl - (uint32_t)(8 * npass) : 0u;
   }
 }
 
@@ -863,10 +886,24 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_fix(const uint64_t* __restrict__
     const uint32_t t0 = t * FIX_TILE, t1 = min(n, t0 + FIX_TILE);
     const uint32_t w0 = t0 > FIX_SPAN ? t0 - FIX_SPAN : 0u;
     const uint32_t w1 = min(n, t1 + FIX_SPAN);
-    for (uint32_t i = w0 + threadIdx.x; i < w1; i += SORT_BLOCK) {
-      const uint32_t x = i - w0;
-      s_key[x] = gload(keys_in + i);
-      s_pl[x] = gload(idx_in + i);
+    // The window in one round of loads per thread (indexes clamped into the window, the
+    // clamped elements not stored).
+    constexpr uint32_t FIX_LOADS = (FIX_WIN + SORT_BLOCK - 1) / SORT_BLOCK;
+    uint64_t wk[FIX_LOADS];
+    uint32_t wp[FIX_LOADS];
+#pragma unroll
+    for (uint32_t r = 0; r < FIX_LOADS; ++r) {
+      const uint32_t i = min(w0 + r * SORT_BLOCK + threadIdx.x, w1 - 1);
+      wk[r] = gload(keys_in + i);
+      wp[r] = gload(idx_in + i);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < FIX_LOADS; ++r) {
+      const uint32_t x = r * SORT_BLOCK + threadIdx.x;
+      if (w0 + x < w1) {
+        s_key[x] = wk[r];
+        s_pl[x] = wp[r];
+      }
     }
     __syncthreads();
     const uint32_t wn = w1 - w0;
@@ -1027,7 +1064,7 @@ static hipError_t sort_normalized(const uint64_t* keys_in, const uint8_t* ops_in
   uint32_t xgrid = 0;
   hipError_t e = exact_grid<ITEMS>(xgrid);
   if (e != hipSuccess) return e;
-  k_range<<<RANGE_BLOCKS, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, words, npass);
+  k_range<<<RANGE_BLOCKS, RANGE_THREADS, 0, stream>>>(keys_in, desc, n_host, words);
   const uint64_t* kin = keys_in;
   const uint32_t* iin = nullptr;
   for (int p = 0; p < npass; ++p) {

@@ -427,25 +427,28 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
         par_l = par;
       }
     }
-    if (walk && !mine) {
+    // The sub-chunk's tail loads go out together, before any of its stores (the vector memory
+    // counter counts stores too, so a load issued after a store also waits for the store):
+    // the version and entry count of an INSERT group's leaf, the value a GET group returns.
+    const bool hint = walk && !mine;
+    uint64_t ver = 1ull, info = 0;
+    if (hint && leaf_l < t.capacity) {
+      ver = t.meta[leaf_l].lock;
+      info = t.meta[leaf_l].info;
+    }
+    const uint64_t val = (mine && found_l) ? t.vals[vidx_l] : 0;
+    if (hint) {
       // Hints for the upsert kernels: the leaf, its version and entry count now (no writer
       // runs during this kernel), the key's slot in it and the level-1 node last used for
       // routing.
       uleaf[j] = leaf_l;
-      uint64_t ver = 1ull;
-      uint32_t lcount = 0;
-      if (leaf_l < t.capacity) {
-        ver = t.meta[leaf_l].lock;
-        lcount = meta_count(t.meta[leaf_l].info);
-      }
       uver[j] = ver;
       upos[j] = (uint32_t)(vidx_l - (uint64_t)leaf_l * F) | (found_l ? UPOS_FOUND : 0u) |
-                (min(lcount, 255u) << UPOS_COUNT_SHIFT);
+                (min(meta_count(info), 255u) << UPOS_COUNT_SHIFT);
       upar[j] = par_l;
     }
-    // All values of the sub-chunk in one parallel round trip, then the answers.
+    // The answers.
     if (mine) {
-      const uint64_t val = found_l ? t.vals[vidx_l] : 0;
       if (np_l > FANOUT_DIRECT) {
         // A hot key: its waiting ops are answered by the whole grid (fanout_answers), in
         // chunks queued here, instead of one store after another from this lane.
