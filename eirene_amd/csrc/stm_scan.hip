@@ -204,14 +204,80 @@ STM_DEV uint32_t wave_leaf_below(const TreeView& t, uint32_t leaf, uint64_t key)
   return cnt;
 }
 
+// The entries of leaf `leaf` (the first `count` slots; a leaf's entries are unordered) in key
+// order: position p = j * WAVE + lane holds key sk[j] from slot ss[j] (KEY_EMPTY past count).
+// Each entry's rank is the number of entries below it; a second pass hands each slot to the
+// lane of its rank.  Both passes are wave-uniform loops over the entries.
+template <int F>
+STM_DEV void leaf_in_key_order(const TreeView& t, uint32_t leaf, uint32_t count,
+                               uint64_t (&sk)[(F + WAVE - 1) / WAVE],
+                               uint32_t (&ss)[(F + WAVE - 1) / WAVE]) {
+  constexpr int SPL = (F + WAVE - 1) / WAVE;
+  const int lane = lane_id();
+  uint64_t k[SPL];
+  uint32_t r[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    const uint32_t sl = (uint32_t)(j * WAVE + lane);
+    k[j] = sl < count ? t.keys[(uint64_t)leaf * F + sl] : KEY_EMPTY;
+    r[j] = 0;
+    sk[j] = KEY_EMPTY;
+    ss[j] = 0;
+  }
+  for (uint32_t i = 0; i < count; ++i) {
+    uint64_t ki = 0;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j)
+      if ((i >> 6) == (uint32_t)j) ki = readlane64(k[j], (int)(i & 63));
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) r[j] += ki < k[j] ? 1u : 0u;
+  }
+  for (uint32_t i = 0; i < count; ++i) {
+    uint64_t ki = 0;
+    uint32_t ri = 0;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      if ((i >> 6) == (uint32_t)j) {
+        ki = readlane64(k[j], (int)(i & 63));
+        ri = (uint32_t)__builtin_amdgcn_readlane((int)r[j], (int)(i & 63));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      if (ri == (uint32_t)(j * WAVE + lane)) {
+        sk[j] = ki;
+        ss[j] = i;
+      }
+    }
+  }
+}
+
+// Entry at key-order position p (lane-dependent, p < F) of a leaf_in_key_order view.
+template <int F, typename T>
+STM_DEV T at_position(const T (&x)[(F + WAVE - 1) / WAVE], uint32_t p) {
+  T r = x[0];
+#pragma unroll
+  for (int j = 0; j < (F + WAVE - 1) / WAVE; ++j) {
+    T v;
+    if constexpr (sizeof(T) == 8) v = shfl64(x[j], (int)(p & 63));
+    else v = (T)__shfl((int)x[j], (int)(p & 63), WAVE);
+    if ((p >> 6) == (uint32_t)j) r = v;
+  }
+  return r;
+}
+
 // One scan op, run by the whole wavefront; returns the pairs written.
 template <int F>
 STM_DEV uint32_t run_scan(const TreeView& t, const ScanArgs& a, const ScanIn& in, uint32_t U,
                           uint64_t k, uint32_t i, uint32_t L, uint64_t off) {
   const int lane = lane_id();
   const uint32_t cap = t.capacity;
+  constexpr int SPL = (F + WAVE - 1) / WAVE;
   uint32_t tl = wave_find_leaf<F>(t, k);
   uint32_t tcount = min(meta_count(uniform64(t.meta[tl].info)), (uint32_t)F);
+  uint64_t tsk[SPL];  // the current leaf in key order
+  uint32_t tss[SPL];
+  leaf_in_key_order<F>(t, tl, tcount, tsk, tss);
   uint32_t ts = wave_leaf_below<F>(t, tl, k);
   bool t_done = false;
   // First group with key >= k (groups are in ascending key order).
@@ -236,14 +302,19 @@ STM_DEV uint32_t run_scan(const TreeView& t, const ScanArgs& a, const ScanIn& in
       } else {
         tl = (uint32_t)nx;
         tcount = min(meta_count(uniform64(t.meta[tl].info)), (uint32_t)F);
+        leaf_in_key_order<F>(t, tl, tcount, tsk, tss);
         ts = 0;
       }
     }
     if (t_done && bg >= U) break;
     const uint32_t tn = t_done ? 0u : min(tcount - ts, (uint32_t)WAVE);
     const bool tv = (uint32_t)lane < tn;
-    const uint64_t tslot = (uint64_t)tl * F + ts + lane;
-    const uint64_t tk = tv ? t.keys[tslot] : KEY_EMPTY;
+    // Key-order position ts + lane, read from every lane (the shuffles need their sources
+    // active) and then selected.
+    const uint32_t tp = min(ts + (uint32_t)lane, (uint32_t)F - 1);
+    const uint64_t tk_at = at_position<F>(tsk, tp);
+    const uint64_t tslot = (uint64_t)tl * F + at_position<F>(tss, tp);
+    const uint64_t tk = tv ? tk_at : KEY_EMPTY;
     const bool t_more = !t_done && (ts + tn < tcount || uniform64(t.meta[tl].next) != NODE_NONE);
     const uint64_t lastT = t_more ? readlane64(tk, (int)tn - 1) : KEY_EMPTY;
 

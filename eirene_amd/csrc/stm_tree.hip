@@ -112,6 +112,18 @@ STM_DEV int group_count_le(const uint64_t (&x)[F / G], uint64_t key) {
   return (int)group_sum<G>(c);
 }
 
+// Slot of key in the leaf held by the group (lane g holds slots g*K .. g*K+K-1, as load_slots
+// loads them), -1 if absent.  A leaf's entries are unordered and its free slots hold KEY_EMPTY,
+// which is never a key, so one equality test per slot finds it.
+template <int F, int G>
+STM_DEV int group_find(const uint64_t (&x)[F / G], uint64_t key, int g) {
+  constexpr int K = F / G;
+  uint32_t hit = 0;  // e + 1 of the slot that holds key in this lane
+#pragma unroll
+  for (int e = 0; e < K; ++e) hit = x[e] == key ? (uint32_t)(e + 1) : hit;
+  return (int)group_sum<G>(hit ? (uint32_t)(g * K) + hit : 0u) - 1;
+}
+
 template <int F, int G>
 STM_DEV void load_slots(const uint64_t* __restrict__ base, uint32_t node, int g,
                         uint64_t (&x)[F / G]) {
@@ -416,13 +428,11 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
           reads += 1;
         }
       }
-      const int s = group_count_le<F, G>(lk, key) - 1;
-      const uint64_t at = group_at<F, G>(lk, max(s, 0), gbase);
+      const int s = group_find<F, G>(lk, key, g);
       if (g == (int)jj) {
-        found_l = s >= 0 && at == key;
-        // The key's slot when found, else its insert position (F when every key precedes
-        // it: only read for a found key).
-        vidx_l = (uint64_t)leaf * F + (uint32_t)(found_l ? s : s + 1);
+        found_l = s >= 0;
+        // The key's slot when found (a new key is appended at the leaf's count).
+        vidx_l = (uint64_t)leaf * F + (uint32_t)(found_l ? s : 0);
         leaf_l = leaf;
         par_l = par;
       }
@@ -528,6 +538,44 @@ STM_DEV uint32_t wave_count_le(const uint64_t (&x)[WNode<F>::SPL], uint64_t key)
   uint32_t c = 0;
 #pragma unroll
   for (int j = 0; j < WNode<F>::SPL; ++j) c += (uint32_t)__popcll(ballot(x[j] <= key));
+  return c;
+}
+
+// Slot of key in a leaf held by the wavefront (entries unordered, free slots KEY_EMPTY, which
+// is never a key); NODE_NONE if absent.  Wave-uniform.
+template <int F>
+STM_DEV uint32_t wave_find(const uint64_t (&x)[WNode<F>::SPL], uint64_t key) {
+  uint32_t s = NODE_NONE;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) {
+    const uint64_t m = ballot(x[j] == key);
+    if (m) s = (uint32_t)(j * WAVE) + (uint32_t)__builtin_ctzll(m);
+  }
+  return s;
+}
+
+// Rank of each held key among the leaf's first `count` slots (the number of those keys below
+// it): its place in key order.  The keys are distinct, so the ranks of the count entries are a
+// permutation of [0, count).  One wave-uniform pass over the entries.
+template <int F>
+STM_DEV void wave_ranks(const uint64_t (&x)[WNode<F>::SPL], uint32_t count,
+                        uint32_t (&r)[WNode<F>::SPL]) {
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j) r[j] = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint64_t ki = wave_at<F>(x, i);
+#pragma unroll
+    for (int j = 0; j < WNode<F>::SPL; ++j) r[j] += ki < x[j] ? 1u : 0u;
+  }
+}
+
+// Number of the leaf's first `count` entries below key (wave-uniform).
+template <int F>
+STM_DEV uint32_t wave_count_below(const uint64_t (&x)[WNode<F>::SPL], uint32_t count, uint64_t key) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < WNode<F>::SPL; ++j)
+    c += (uint32_t)__popcll(ballot((uint32_t)(j * WAVE + lane_id()) < count && x[j] < key));
   return c;
 }
 
@@ -786,38 +834,35 @@ STM_DEV void leaf_release(uint64_t* lw, uint64_t ver) {
 }
 
 // Upsert of (key, val) into leaf n, which this wavefront has locked (tx_lock_leaf).  pn
-// is its parent when has_par (else n is the root).  Every return other than TX_DONE and
-// TX_NEED_PARENT has released the leaf; TX_NEED_PARENT keeps it locked.
+// is its parent when has_par (else n is the root).  A leaf's entries are unordered: a found key
+// takes its value in place, a new one goes to slot count; a full leaf splits by rank, the lower
+// half of the F + 1 entries (in key order) staying, the upper half moving to a new right
+// sibling, each written in key order.  Every return other than TX_DONE and TX_NEED_PARENT has
+// released the leaf; TX_NEED_PARENT keeps it locked.
 template <int F>
 STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNode<F>& pn,
                                 bool has_par, bool par_fresh, uint64_t hver, uint64_t key,
                                 uint64_t val, bool& found, uint64_t& old, TxCounters& c) {
   constexpr int SPL = WNode<F>::SPL;
-  const uint32_t cle = wave_count_le<F>(n.k, key);
-  const uint32_t s = cle ? cle - 1 : 0;
-  found = cle > 0 && wave_at<F>(n.k, s) == key;
+  const int lane = lane_id();
+  const uint32_t s = wave_find<F>(n.k, key);
+  found = s != NODE_NONE;
   old = found ? wave_at<F>(n.v, s) : 0;
   uint64_t* lw = &t.meta[n.id].lock;
+  uint64_t* kp = t.keys + (size_t)n.id * F;
+  uint64_t* vp = t.vals + (size_t)n.id * F;
   if (found) {
-#pragma unroll
-    for (int j = 0; j < SPL; ++j)
-      if (j * WAVE + lane_id() == (int)s) st_relaxed(t.vals + (size_t)n.id * F + s, val);
+    if (lane == 0) st_relaxed(vp + s, val);
     tx_unlock(lw, n.ver, nullptr, 0, nullptr, 0);
     c.commits++;
     return TX_DONE;
   }
-  const uint32_t pos = cle;  // insert position
   if (n.count < (uint32_t)F) {
-    uint64_t nk[SPL], nv[SPL];
-#pragma unroll
-    for (int j = 0; j < SPL; ++j) {
-      const int sl = j * WAVE + lane_id();
-      const uint64_t gk = wgather<F>(n.k, sl - 1, KEY_EMPTY);
-      const uint64_t gv = wgather<F>(n.v, sl - 1, 0);
-      nk[j] = sl < (int)pos ? n.k[j] : (sl == (int)pos ? key : gk);
-      nv[j] = sl < (int)pos ? n.v[j] : (sl == (int)pos ? val : gv);
+    if (lane == 0) {
+      st_relaxed(kp + n.count, key);
+      st_relaxed(vp + n.count, val);
     }
-    tx_write_node<F>(t, n.id, nk, nv, n.count + 1, 0, n.low, n.high, n.next);
+    if (lane == 1) st_relaxed(&t.meta[n.id].info, make_info(n.count + 1, 0));
     tx_unlock(lw, n.ver, nullptr, 0, nullptr, 0);
     c.commits++;
     c.new_keys++;
@@ -838,6 +883,24 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
       return TX_ABORT;
     }
   }
+  // Places in key order: an old entry's rank among the old ones, plus one above the new key;
+  // the new key's place is the number of old entries below it.
+  uint32_t r[SPL];
+  wave_ranks<F>(n.k, F, r);
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) r[j] += key < n.k[j] ? 1u : 0u;
+  const uint32_t rn = wave_count_below<F>(n.k, F, key);
+  // the separator: the entry of place h
+  uint64_t cand = 0;
+  bool has = false;
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) {
+    if (j * WAVE + lane < F && r[j] == h) {
+      cand = n.k[j];
+      has = true;
+    }
+  }
+  const uint64_t sep = rn == h ? key : readlane64(cand, __builtin_ctzll(ballot(has) | (1ull << 63)));
   uint64_t* pw = has_par ? &t.meta[pn.id].lock : nullptr;
   uint64_t* hw = has_par ? nullptr : &t.hdr->lock;
   if (!tx_lock(pw, has_par ? pn.ver : 0, hw, hver, nullptr, 0)) {
@@ -852,33 +915,37 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
   const uint32_t rid = base;
   const uint32_t qid = has_par ? NODE_NONE : base + 1;
   tx_claim_new(t, rid, qid);
-  // A(x) = merged sequence: old[x] before pos, the new pair at pos, old[x-1] after.
-  uint64_t lk[SPL], lv[SPL], rk[SPL], rv[SPL];
+  uint64_t* rkp = t.keys + (size_t)rid * F;
+  uint64_t* rvp = t.vals + (size_t)rid * F;
+  // Every entry to its place: [0, h) in this leaf, [h, F+1) in the right sibling.  The left
+  // leaf's slots from h up are freed; the right sibling's slots past its F+1-h entries are free.
 #pragma unroll
   for (int j = 0; j < SPL; ++j) {
-    const int sl = j * WAVE + lane_id();
-    // left slot sl < h holds A(sl)
-    const uint64_t km1 = wgather<F>(n.k, sl - 1, KEY_EMPTY);
-    const uint64_t vm1 = wgather<F>(n.v, sl - 1, 0);
-    const uint64_t ak = sl < (int)pos ? n.k[j] : (sl == (int)pos ? key : km1);
-    const uint64_t av = sl < (int)pos ? n.v[j] : (sl == (int)pos ? val : vm1);
-    lk[j] = sl < (int)h ? ak : KEY_EMPTY;
-    lv[j] = sl < (int)h ? av : 0;
-    // right slot sl < F+1-h holds A(sl + h)
-    const int x = sl + (int)h;
-    const uint64_t kx = wgather<F>(n.k, x, KEY_EMPTY);
-    const uint64_t vx = wgather<F>(n.v, x, 0);
-    const uint64_t kx1 = wgather<F>(n.k, x - 1, KEY_EMPTY);
-    const uint64_t vx1 = wgather<F>(n.v, x - 1, 0);
-    const uint64_t bk = x < (int)pos ? kx : (x == (int)pos ? key : kx1);
-    const uint64_t bv = x < (int)pos ? vx : (x == (int)pos ? val : vx1);
-    const bool r_ok = sl < (int)(F + 1 - h);
-    rk[j] = r_ok ? bk : KEY_EMPTY;
-    rv[j] = r_ok ? bv : 0;
+    const uint32_t sl = (uint32_t)(j * WAVE + lane);
+    if (sl < (uint32_t)F) {
+      const uint32_t to = r[j];
+      if (to >= h) {
+        st_relaxed(rkp + (to - h), n.k[j]);
+        st_relaxed(rvp + (to - h), n.v[j]);
+      } else if (to != sl) {
+        st_relaxed(kp + to, n.k[j]);
+        st_relaxed(vp + to, n.v[j]);
+      }
+      if (sl >= h) st_relaxed(kp + sl, KEY_EMPTY);
+      if (sl >= F + 1 - h) st_relaxed(rkp + sl, KEY_EMPTY);
+    }
   }
-  const uint64_t sep = wave_at<F>(rk, 0);
-  tx_write_node<F>(t, rid, rk, rv, F + 1 - h, 0, sep, n.high, n.next);
-  tx_write_node<F>(t, n.id, lk, lv, h, 0, n.low, sep, rid);
+  if (lane == 0) {
+    if (rn >= h) {
+      st_relaxed(rkp + (rn - h), key);
+      st_relaxed(rvp + (rn - h), val);
+    } else {
+      st_relaxed(kp + rn, key);
+      st_relaxed(vp + rn, val);
+    }
+  }
+  tx_write_meta(t, rid, F + 1 - h, 0, sep, n.high, n.next);
+  tx_write_meta(t, n.id, h, 0, n.low, sep, rid);
   if (has_par) {
     uint64_t pk[SPL], pv[SPL];
     parent_with_child<F>(pn, ps, sep, rid, pk, pv);
@@ -889,22 +956,22 @@ STM_DEV TxResult tx_leaf_upsert(const TreeView& t, const WNode<F>& n, const WNod
     uint64_t qk[SPL], qv[SPL];
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
-      const int sl = j * WAVE + lane_id();
-      qk[j] = sl == 0 ? n.low : (sl == 1 ? sep : KEY_EMPTY);
-      qv[j] = sl == 0 ? (uint64_t)n.id : (sl == 1 ? (uint64_t)rid : 0);
+      const int q = j * WAVE + lane;
+      qk[j] = q == 0 ? n.low : (q == 1 ? sep : KEY_EMPTY);
+      qv[j] = q == 0 ? (uint64_t)n.id : (q == 1 ? (uint64_t)rid : 0);
     }
     tx_write_node<F>(t, qid, qk, qv, 2, 1, n.low, n.high, NODE_NONE);
-    if (lane_id() == 0) {
+    if (lane == 0) {
       st_relaxed(&t.hdr->root, (uint64_t)qid);
       st_relaxed(&t.hdr->height, 2ull);
     }
     tx_unlock(lw, n.ver, &t.meta[rid].lock, 0ull, &t.hdr->lock, hver);
-    if (lane_id() == 0) st_relaxed(&t.meta[qid].lock, 2ull);
+    if (lane == 0) st_relaxed(&t.meta[qid].lock, 2ull);
     c.root_splits++;
   }
-  if (t.rank_of && lane_id() == 0) {
-    const uint32_t r = t.rank_of[n.id];
-    if (r != NODE_NONE) t.dir[r].pflags |= DIR_SPLIT;
+  if (t.rank_of && lane == 0) {
+    const uint32_t rr = t.rank_of[n.id];
+    if (rr != NODE_NONE) t.dir[rr].pflags |= DIR_SPLIT;
   }
   c.leaf_splits++;
   c.commits++;
@@ -1032,79 +1099,60 @@ STM_DEV TxResult tx_upsert_attempt(const TreeView& t, uint64_t key, uint64_t val
   return r;
 }
 
-// Merged upsert of the round's ops whose keys lie in the locked leaf n (lane mask R, each
-// lane's key k_l and value v_l; keys are distinct): one transaction writes them all.  Lane j
-// finds its key's place cle (keys <= k_l) by a binary search over the leaf, a new key goes to
-// cle + (new keys of R below it), an old slot s to s + (new keys of R below its key); a key
-// already present takes its new value in place.  With T = count + new keys <= F the leaf is
-// rewritten in place; with F < T <= 2F it splits into T/2 + (T - T/2) entries and one
-// separator goes into the parent, as the single-op split does.  Returns as tx_leaf_upsert;
-// found_l / old_l are each lane's answer.
+// Merged upsert of the ops of lanes R into leaf n, which this wavefront has locked (each lane's
+// key k_l and value v_l; keys are distinct): one transaction writes them all.  A leaf's entries
+// are unordered: each op's key is looked up by equality over the leaf's slots, a key already
+// present takes its new value in place, and the new keys are appended from slot count in lane
+// order.  With T = count + new keys <= F the leaf is rewritten in place; with F < T <= 2F it
+// splits by rank, the lower T/2 entries (in key order) staying, the rest moving to a new right
+// sibling, each half written in key order, and one separator goes into the parent, as the
+// single-op split does.  Returns as tx_leaf_upsert; found_l / old_l are each lane's answer.
 template <int F>
 STM_DEV TxResult tx_leaf_merge(const TreeView& t, const WNode<F>& n, const WNode<F>& pn,
                                bool has_par, bool par_fresh, uint64_t hver, uint64_t key0,
                                uint64_t R, uint64_t k_l, uint64_t v_l, bool& found_l,
                                uint64_t& old_l, TxCounters& c) {
   constexpr int SPL = WNode<F>::SPL;
-  constexpr int STEPS = F <= 32 ? 6 : (F <= 64 ? 7 : 8);  // binary search over <= F keys
   const int lane = lane_id();
   const bool mine = (R >> lane) & 1ull;
   uint64_t* lw = &t.meta[n.id].lock;
-  uint32_t lo = 0, hi = n.count;
-#pragma unroll
-  for (int it = 0; it < STEPS; ++it) {
-    const uint32_t mid = (lo + hi) >> 1;
-    const uint64_t km = wgather<F>(n.k, (int)mid, KEY_EMPTY);
-    if (lo < hi) {
-      if (km <= k_l) lo = mid + 1;
-      else hi = mid;
-    }
-  }
-  const uint32_t cle = lo;
-  const uint64_t at_k = wgather<F>(n.k, (int)cle - 1, KEY_EMPTY);
-  const uint64_t at_v = wgather<F>(n.v, (int)cle - 1, 0);
-  const bool fnd = mine && cle > 0 && at_k == k_l;
-  const uint64_t newm = ballot(mine && !fnd);
-  const uint32_t ins = (uint32_t)__popcll(newm);
-  const uint32_t T = n.count + ins;
-  // rank of a new key among the new keys, shift of each old slot, new values of found keys
-  uint32_t rnk = 0;
-  uint32_t sh[SPL];
+  // The ops' keys in the leaf, one op at a time (wave-uniform): uv = the values after the found
+  // ops' updates, fmask = the found ops, each found lane's old value.
   uint64_t uv[SPL];
 #pragma unroll
-  for (int j = 0; j < SPL; ++j) {
-    sh[j] = 0;
-    uv[j] = n.v[j];
-  }
+  for (int j = 0; j < SPL; ++j) uv[j] = n.v[j];
+  uint64_t fmask = 0;
+  uint64_t oldv = 0;
   for (uint64_t rest = R; rest; rest &= rest - 1) {
     const int i = __builtin_ctzll(rest);
     const uint64_t ki = readlane64(k_l, i);
-    if ((newm >> i) & 1ull) {
-      rnk += ki < k_l ? 1u : 0u;
-#pragma unroll
-      for (int j = 0; j < SPL; ++j) sh[j] += ki < n.k[j] ? 1u : 0u;
-    } else {
+    const uint32_t s = wave_find<F>(n.k, ki);
+    if (s != NODE_NONE) {
       const uint64_t vi = readlane64(v_l, i);
+      const uint64_t ov = wave_at<F>(n.v, s);
 #pragma unroll
       for (int j = 0; j < SPL; ++j)
-        if (n.k[j] == ki) uv[j] = vi;
+        if ((uint32_t)(j * WAVE + lane) == s) uv[j] = vi;
+      fmask |= 1ull << i;
+      if (lane == i) oldv = ov;
     }
   }
-  const uint32_t P = cle + rnk;  // final place of this lane's key when it is new
+  const bool fnd = (fmask >> lane) & 1ull;
+  const uint64_t newm = R & ~fmask;
+  const uint32_t ins = (uint32_t)__popcll(newm);
+  const uint32_t T = n.count + ins;
   found_l = fnd;
-  old_l = fnd ? at_v : 0;
+  old_l = fnd ? oldv : 0;
   uint64_t* kp = t.keys + (size_t)n.id * F;
   uint64_t* vp = t.vals + (size_t)n.id * F;
   if (T <= (uint32_t)F) {
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
       const uint32_t sl = (uint32_t)(j * WAVE + lane);
-      if (sl < n.count && (sh[j] || uv[j] != n.v[j])) {
-        st_relaxed(kp + sl + sh[j], n.k[j]);
-        st_relaxed(vp + sl + sh[j], uv[j]);
-      }
+      if (sl < n.count && uv[j] != n.v[j]) st_relaxed(vp + sl, uv[j]);
     }
     if (mine && !fnd) {
+      const uint32_t P = n.count + (uint32_t)__popcll(newm & lanemask_lt());
       st_relaxed(kp + P, k_l);
       st_relaxed(vp + P, v_l);
     }
@@ -1114,7 +1162,7 @@ STM_DEV TxResult tx_leaf_merge(const TreeView& t, const WNode<F>& n, const WNode
     c.new_keys += ins;
     return TX_DONE;
   }
-  // Split: entries at final places [0, h) stay, [h, T) move to a new right sibling.
+  // Split: entries at places [0, h) in key order stay, [h, T) move to a new right sibling.
   if (has_par && !par_fresh) return TX_NEED_PARENT;
   const uint32_t h = T / 2;
   uint32_t ps = 0;
@@ -1129,18 +1177,36 @@ STM_DEV TxResult tx_leaf_merge(const TreeView& t, const WNode<F>& n, const WNode
       return TX_ABORT;
     }
   }
-  // the separator: the entry whose final place is h
+  // Places in key order: for an old entry, the old entries below it plus the new keys below it;
+  // for a new key, the same.
+  uint32_t r[SPL];
+#pragma unroll
+  for (int j = 0; j < SPL; ++j) r[j] = 0;
+  uint32_t rn = 0;  // per lane: entries below k_l
+  for (uint32_t i = 0; i < n.count; ++i) {
+    const uint64_t ki = wave_at<F>(n.k, i);
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) r[j] += ki < n.k[j] ? 1u : 0u;
+    rn += ki < k_l ? 1u : 0u;
+  }
+  for (uint64_t rest = newm; rest; rest &= rest - 1) {
+    const uint64_t ki = readlane64(k_l, __builtin_ctzll(rest));
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) r[j] += ki < n.k[j] ? 1u : 0u;
+    rn += ki < k_l ? 1u : 0u;
+  }
+  // the separator: the entry whose place is h
   uint64_t cand = 0;
   bool has = false;
 #pragma unroll
   for (int j = 0; j < SPL; ++j) {
     const uint32_t sl = (uint32_t)(j * WAVE + lane);
-    if (sl < n.count && sl + sh[j] == h) {
+    if (sl < n.count && r[j] == h) {
       cand = n.k[j];
       has = true;
     }
   }
-  if (mine && !fnd && P == h) {
+  if (mine && !fnd && rn == h) {
     cand = k_l;
     has = true;
   }
@@ -1161,15 +1227,17 @@ STM_DEV TxResult tx_leaf_merge(const TreeView& t, const WNode<F>& n, const WNode
   tx_claim_new(t, rid, qid);
   uint64_t* rkp = t.keys + (size_t)rid * F;
   uint64_t* rvp = t.vals + (size_t)rid * F;
+  // Every entry to its place; the left leaf's old slots from h up are freed, and the right
+  // sibling's slots past its T - h entries are free.
 #pragma unroll
   for (int j = 0; j < SPL; ++j) {
     const uint32_t sl = (uint32_t)(j * WAVE + lane);
     if (sl < n.count) {
-      const uint32_t to = sl + sh[j];
+      const uint32_t to = r[j];
       if (to >= h) {
         st_relaxed(rkp + (to - h), n.k[j]);
         st_relaxed(rvp + (to - h), uv[j]);
-      } else if (sh[j] || uv[j] != n.v[j]) {
+      } else if (to != sl || uv[j] != n.v[j]) {
         st_relaxed(kp + to, n.k[j]);
         st_relaxed(vp + to, uv[j]);
       }
@@ -1178,12 +1246,12 @@ STM_DEV TxResult tx_leaf_merge(const TreeView& t, const WNode<F>& n, const WNode
     if (sl >= T - h && sl < (uint32_t)F) st_relaxed(rkp + sl, KEY_EMPTY);
   }
   if (mine && !fnd) {
-    if (P >= h) {
-      st_relaxed(rkp + (P - h), k_l);
-      st_relaxed(rvp + (P - h), v_l);
+    if (rn >= h) {
+      st_relaxed(rkp + (rn - h), k_l);
+      st_relaxed(rvp + (rn - h), v_l);
     } else {
-      st_relaxed(kp + P, k_l);
-      st_relaxed(vp + P, v_l);
+      st_relaxed(kp + rn, k_l);
+      st_relaxed(vp + rn, v_l);
     }
   }
   tx_write_meta(t, rid, T - h, 0, sep, n.high, n.next);
@@ -1332,25 +1400,23 @@ STM_DEV void flush_counters(const TreeView& t, const TxCounters& c, BatchCounter
   flush_counters(t, c);
 }
 
-// Lane-group fast path: NG = 64 / G merged INSERTs of one wavefront at once, each by G lanes
-// (slot e*G + g of the leaf in element e of lane g, so every load and store instruction covers
-// G consecutive slots).  It is the leaf-only transaction of the wave-wide path (update in place
-// or shift-insert, commit with version + 2) run for NG ops in lock-step, started from the leaf,
-// slot and entry count the lookup kernel found for the key, and it runs only for the op that
-// owns its leaf in this launch: the first merged INSERT of the leaf in key order.  Groups are
-// sorted by key, so an op owns its leaf when none of the groups between the leaf's low fence and
-// its own key is an INSERT; lane g of the op's lane group reads group gi - 1 - g (key and merged
-// op), and ownership is proven when one of those G keys lies below the fence (or the batch
-// starts).  No other group of the launch then writes the leaf, and no writer ran since the
-// lookup kernel read it, so the hint is exact and no lock is taken.  An op that does not own
-// its leaf (another INSERT of the batch precedes it there, or more than G groups of the leaf do),
-// whose leaf is full or whose hint disagrees with the leaf's version, count or fences is handed
-// back to the wave-wide path, which runs after this launch.  Only the slots that move are read:
-// for an insert at position p of a leaf with c entries, lane g loads slot x - 1 for each of its
-// slots x in (p, c] and stores it at x; a found key reads and writes its one value.  (Measured
-// with the leaf's test-and-set instead of the ownership test: 131 µs per 2^22-op batch, of which
-// the test-and-set took ≈36 µs; stores of whole 128-byte lines, two slots per lane, measured no
-// faster.)
+// Lane-group fast path: NG = 64 / G merged INSERTs of one wavefront at once, each by G lanes.
+// It is the leaf-only transaction of the wave-wide path (update in place, or append at the
+// leaf's count: a leaf's entries are unordered), commit with version + 2, run for NG ops in
+// lock-step, started from the leaf, slot and entry count the lookup kernel found for the key,
+// and it runs only for the op that owns its leaf in this launch: the first merged INSERT of the
+// leaf in key order.  Groups are sorted by key, so an op owns its leaf when none of the groups
+// between the leaf's low fence and its own key is an INSERT; lane g of the op's lane group reads
+// group gi - 1 - g (key and merged op), and ownership is proven when one of those G keys lies
+// below the fence (or the batch starts).  No other group of the launch then writes the leaf, and
+// no writer ran since the lookup kernel read it, so the hint is exact and no lock is taken.  An
+// op that does not own its leaf (another INSERT of the batch precedes it there, or more than G
+// groups of the leaf do), whose leaf is full or whose hint disagrees with the leaf's version,
+// count or fences is handed back to the wave-wide path, which runs after this launch.  A found
+// key reads and writes its one value; a new key writes one key and one value.  (Measured with
+// sorted leaves, whose inserts shifted the slots above the key: 97 µs per 2^22-op batch, ≈56 of
+// them the shifted slots' stores; with the leaf's test-and-set instead of the ownership test
+// before that, 131 µs.)
 // src = round lane of this group's op (-1: none).  Returns the round lanes whose op committed.
 template <int F, int G>
 STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uint64_t v_l,
@@ -1360,7 +1426,6 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
                                    uint32_t np_l, uint32_t p0_l, const uint32_t* __restrict__ sidx,
                                    uint64_t* __restrict__ out_vals,
                                    uint8_t* __restrict__ out_found, TxCounters& c) {
-  constexpr int K = F / G;
   const int lane = lane_id();
   const int g = lane & (G - 1);
   const int gbase = lane - g;
@@ -1374,24 +1439,17 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   const uint32_t up = (uint32_t)__shfl((int)up_l, sl, WAVE);
   const uint32_t gi = (uint32_t)__shfl((int)gi_l, sl, WAVE);
   const uint64_t ver = shfl64(ver_l, sl);
-  const uint32_t pos = up & UPOS_SLOT;
+  const uint32_t pos = up & UPOS_SLOT;  // the key's slot when found
   const bool hfound = (up & UPOS_FOUND) != 0;
   const uint32_t hcount = up >> UPOS_COUNT_SHIFT;
   // A hint out of range (never written by the lookup kernel) is not acted on.
   const bool act = src >= 0 && leaf < t.capacity && hcount <= (uint32_t)F &&
-                   (hfound ? pos < hcount : pos <= hcount);
+                   (!hfound || pos < hcount);
   const size_t base = (size_t)(act ? leaf : 0u) * F;
   NodeMeta* m = t.meta + (act ? leaf : 0u);
-  uint64_t* kp = t.keys + base + g;
-  uint64_t* vp = t.vals + base + g;
-  uint64_t sk[K], sv[K];  // slot e*G + g - 1, for the slots in (pos, hcount]
   uint64_t info = 0, low = 0, high = 0, word = 0, old = 0;
-#pragma unroll
-  for (int e = 0; e < K; ++e) {
-    sk[e] = 0;
-    sv[e] = 0;
-  }
-  // The leaf's metadata line, the slots that move and the G groups before the op go out together.
+  // The leaf's metadata line, the found key's value and the G groups before the op go out
+  // together.
   const bool has_prev = act && (uint32_t)g < gi;
   uint64_t pkey = 0;
   uint8_t pop = UOP_SKIP;
@@ -1404,18 +1462,7 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
       pkey = ukey[gi - 1 - (uint32_t)g];
       pop = uop[gi - 1 - (uint32_t)g];
     }
-    if (hfound) {
-      if (g == 0) old = t.vals[base + pos];
-    } else {
-#pragma unroll
-      for (int e = 0; e < K; ++e) {
-        const uint32_t x = (uint32_t)(e * G + g);
-        if (x > pos && x <= hcount) {
-          sk[e] = kp[e * G - 1];
-          sv[e] = vp[e * G - 1];
-        }
-      }
-    }
+    if (hfound && g == 0) old = t.vals[base + pos];
   }
   old = shfl64(old, gbase);
   // Keys fall as g grows, so the groups inside the leaf's range are a prefix of the G read.
@@ -1435,16 +1482,11 @@ STM_DEV uint64_t fast_leaf_upserts(const TreeView& t, int src, uint64_t k_l, uin
   } else {
     old = 0;
   }
-  // Shift insert at pos: slot x takes slot x - 1 for x in (pos, count], the new pair at pos.
-#pragma unroll
-  for (int e = 0; e < K; ++e) {
-    const uint32_t x = (uint32_t)(e * G + g);
-    if (insert && x >= pos && x <= count) {
-      kp[e * G] = x == pos ? key : sk[e];
-      vp[e * G] = x == pos ? val : sv[e];
-    }
+  if (insert && g == 0) {  // slot count is free (KEY_EMPTY)
+    t.keys[base + count] = key;
+    t.vals[base + count] = val;
   }
-  if (insert && g == 0) st_relaxed(&m->info, make_info(count + 1, 0));
+  if (insert && g == 1 % G) st_relaxed(&m->info, make_info(count + 1, 0));
   if (commit && g == 0) st_relaxed(&m->lock, word + 2);
   if (commit) {
     for (uint32_t p = (uint32_t)g; p < np; p += G) {
@@ -1823,8 +1865,14 @@ __global__ void k_dump_copy(TreeView t, const uint32_t* __restrict__ sorted_ids,
   const uint64_t o1 = offsets[r + 1];
   if (o0 + s >= o1) return;
   const uint32_t id = sorted_ids[r];
-  out_k[o0 + s] = t.keys[(size_t)id * F + s];
-  out_v[o0 + s] = t.vals[(size_t)id * F + s];
+  // A leaf's entries are unordered: slot s goes to its rank among the leaf's count entries.
+  const uint64_t* kp = t.keys + (size_t)id * F;
+  const uint64_t k = kp[s];
+  const uint32_t count = (uint32_t)(o1 - o0);
+  uint32_t rank = 0;
+  for (uint32_t u = 0; u < count; ++u) rank += kp[u] < k ? 1u : 0u;
+  out_k[o0 + rank] = k;
+  out_v[o0 + rank] = t.vals[(size_t)id * F + s];
 }
 
 // One wavefront per node; counts violated invariants into *bad.
@@ -1854,7 +1902,13 @@ __global__ __launch_bounds__(256) void k_check(TreeView t, uint32_t n_nodes,
       if (k != KEY_EMPTY) errs++;
       continue;
     }
-    if (s > 0 && t.keys[(size_t)id * F + s - 1] >= k) errs++;
+    if (level > 0) {
+      if (s > 0 && t.keys[(size_t)id * F + s - 1] >= k) errs++;  // inner nodes in key order
+    } else {
+      // leaf entries are unordered but distinct
+      for (uint32_t u = 0; u < count; ++u)
+        if (u != (uint32_t)s && t.keys[(size_t)id * F + u] == k) errs++;
+    }
     if (k < m.low || k >= m.high) errs++;
     if (level > 0) {
       if (s == 0 && k != m.low) errs++;
