@@ -120,12 +120,13 @@ struct TreeView {
 // order it against the transactional kernels.
 // One exception inside the transactional kernels: the lane-group fast path of k_upsert_fast
 // (fast_leaf_upserts in stm_tree.hip) reads and writes a leaf's fences, keys and values with
-// plain loads and stores.  That is safe only because a group uses the content solely when its
-// test-and-set of the leaf's lock word returns exactly the version the lookup kernel recorded:
-// then no group of the same launch has committed to that leaf, so the plain loads return what
-// earlier kernels left, and no group of the launch reads what another group of it wrote (the
-// launch boundary publishes the plain stores).  A change that lets a group of that kernel re-read
-// a leaf it did not lock with the recorded version must use the agent-scope accesses instead.
+// plain loads and stores.  That is safe only because a group touches a leaf solely when it owns
+// the leaf in that launch (no other merged INSERT of the batch lies between the leaf's low fence
+// and its key, which the batch's sorted order proves) and the leaf's lock word still holds the
+// version the lookup kernel recorded: then no other group of the launch reads or writes that
+// leaf, so the plain loads return what earlier kernels left, and the launch boundary publishes
+// the plain stores.  A change that lets a group of that kernel touch a leaf it does not own
+// must use the agent-scope accesses and the lock word instead, as k_upsert does.
 // Plain load through a pointer known to address device global memory.  Pointers read from
 // memory (a batch descriptor's) are generic to the compiler, which then emits flat loads; this
 // lets it use global loads with a scalar base and immediate offsets.

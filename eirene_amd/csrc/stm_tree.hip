@@ -1624,6 +1624,34 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
   if (lane_id() == 0) stat_add(t.stats, ST_UPSERTS, wave_stripe(), upserts);
 }
 
+// Answers of the committed ops of a round (lanes with ans): a lane writes its own group's few
+// waiting ops (np of them from sorted position gs); a group with many (a hot key) is written by
+// the whole wavefront.  Called by every lane of the wavefront.
+STM_DEV void write_upsert_answers(bool ans, bool found, uint64_t old, uint32_t gs, uint32_t np,
+                                  const uint32_t* __restrict__ sidx,
+                                  uint64_t* __restrict__ out_vals, uint8_t* __restrict__ out_found) {
+  const bool many = ans && np > 8;
+  if (ans && !many) {
+    for (uint32_t p = 0; p < np; ++p) {
+      const uint32_t pos = pos_of(sidx[gs + p]);
+      out_vals[pos] = old;
+      out_found[pos] = found ? 1 : 0;
+    }
+  }
+  for (uint64_t bm = ballot(many); bm; bm &= bm - 1) {
+    const int b = __builtin_ctzll(bm);
+    const uint32_t gsb = __builtin_amdgcn_readlane(gs, b);
+    const uint32_t npb = __builtin_amdgcn_readlane(np, b);
+    const uint64_t ob = readlane64(old, b);
+    const uint8_t fb = (uint8_t)__builtin_amdgcn_readlane(found ? 1 : 0, b);
+    for (uint32_t p = lane_id(); p < npb; p += WAVE) {
+      const uint32_t pos = pos_of(sidx[gsb + p]);
+      out_vals[pos] = ob;
+      out_found[pos] = fb;
+    }
+  }
+}
+
 // Wave-wide transactions for the ops the fast path handed back, or for every merged INSERT
 // of the batch when slist is null.  The handed-back ops lie in the insert list's index range
 // [0, n_upserts): fast-path wavefront f owned [f * per_f, (f + 1) * per_f) and wrote its
@@ -1642,6 +1670,7 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
                                                 uint32_t fast_waves,
                                                 const uint32_t* __restrict__ upar,
                                                 const uint32_t* __restrict__ uleaf,
+                                                const uint32_t* __restrict__ upos,
                                                 BatchCounters* cnt, uint32_t* __restrict__ dlist,
                                                 uint32_t max_attempts,
                                                 uint64_t* __restrict__ out_vals,
@@ -1699,6 +1728,58 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
     const uint32_t np_l = mine ? npend[gi] : 0;
     const uint32_t par_l = (mine && upar) ? upar[gi] : NODE_NONE;
     const uint32_t leaf_l = (mine && uleaf) ? uleaf[gi] : NODE_NONE;
+    const uint32_t up_l = (mine && upos) ? upos[gi] : 0u;
+    if (upos) {
+      // Every lane at once: its own leaf transaction on the hinted leaf, for an op that needs no
+      // split (leaf entries are unordered, so a new key is one append and a found key one value
+      // store).  Test-and-set of the leaf's lock word; under the lock the leaf must still be a
+      // leaf holding the key in its fences (a split by a transaction of this launch moves keys
+      // out), a found key must still sit at its hinted slot (a split rewrites the slots in key
+      // order) and a new key needs a free slot.  The same lock word serializes these with the
+      // wave-wide transactions below, which take every op that fails here.
+      bool held = false, ok = false, fnd = false;
+      uint64_t w = 0, info = 0, low = 0, high = 0, kat = 0, old = 0;
+      const uint32_t pos = up_l & UPOS_SLOT;
+      const bool hfound = (up_l & UPOS_FOUND) != 0;
+      NodeMeta* m = t.meta + (leaf_l < t.capacity ? leaf_l : 0u);
+      const size_t nb = (size_t)(leaf_l < t.capacity ? leaf_l : 0u) * F;
+      if (mine && leaf_l < t.capacity && (!hfound || pos < (uint32_t)F)) {
+        w = __hip_atomic_fetch_or(&m->lock, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        held = !(w & 1ull);
+      }
+      wave_order();  // the leaf's content is read only after its lock is held
+      if (held) {
+        info = ld_relaxed(&m->info);
+        low = ld_relaxed(&m->low);
+        high = ld_relaxed(&m->high);
+        if (hfound) {
+          kat = ld_relaxed(t.keys + nb + pos);
+          old = ld_relaxed(t.vals + nb + pos);
+        }
+      }
+      const uint32_t count = meta_count(info);
+      if (held && meta_level(info) == 0 && count <= (uint32_t)F && k_l >= low && k_l < high) {
+        if (hfound) {
+          ok = kat == k_l;
+          fnd = ok;
+          if (ok) st_relaxed(t.vals + nb + pos, v_l);
+        } else if (count < (uint32_t)F) {
+          ok = true;
+          st_relaxed(t.keys + nb + count, k_l);
+          st_relaxed(t.vals + nb + count, v_l);
+          st_relaxed(&m->info, make_info(count + 1, 0));
+        }
+      }
+      wave_order();  // the content stores are acknowledged before the lock word changes
+      if (held) st_relaxed(&m->lock, ok ? w + 2 : w);
+      const uint64_t adone = ballot(ok);
+      write_upsert_answers(ok, fnd, fnd ? old : 0ull, gs_l, np_l, sidx, out_vals, out_found);
+      c.commits += (uint32_t)__popcll(adone);
+      c.new_keys += (uint32_t)__popcll(ballot(ok && !fnd));
+      c.node_reads += (uint32_t)__popcll(ballot(held));
+      upserts += (uint32_t)__popcll(adone);
+      todo &= ~adone;
+    }
     // Ops of the round whose keys share a leaf commit together (tx_merge_attempt): a hot
     // leaf takes one transaction per round instead of one per op.
     while (todo) {
@@ -1747,29 +1828,8 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
         todo &= ~(1ull << l);
         continue;
       }
-      // Answers: a lane writes its own group's few waiting ops; a group with many (a hot
-      // key) is written by the whole wavefront.
-      const bool ans = (done >> lane_id()) & 1ull;
-      const bool many = ans && np_l > 8;
-      if (ans && !many) {
-        for (uint32_t p = 0; p < np_l; ++p) {
-          const uint32_t pos = pos_of(sidx[gs_l + p]);
-          out_vals[pos] = old_l;
-          out_found[pos] = found_l ? 1 : 0;
-        }
-      }
-      for (uint64_t bm = ballot(many); bm; bm &= bm - 1) {
-        const int b = __builtin_ctzll(bm);
-        const uint32_t gsb = __builtin_amdgcn_readlane(gs_l, b);
-        const uint32_t npb = __builtin_amdgcn_readlane(np_l, b);
-        const uint64_t ob = readlane64(old_l, b);
-        const uint8_t fb = (uint8_t)__builtin_amdgcn_readlane(found_l ? 1 : 0, b);
-        for (uint32_t p = lane_id(); p < npb; p += WAVE) {
-          const uint32_t pos = pos_of(sidx[gsb + p]);
-          out_vals[pos] = ob;
-          out_found[pos] = fb;
-        }
-      }
+      write_upsert_answers(((done >> lane_id()) & 1ull) != 0, found_l, old_l, gs_l, np_l, sidx,
+                           out_vals, out_found);
       upserts += (uint32_t)__popcll(done);
       todo &= ~done;
     }
@@ -2065,7 +2125,8 @@ static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStr
   }
   k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
                                         fast ? a.slist : nullptr, a.scount, fast_waves, a.upar,
-                                        a.uleaf, a.counters, a.dlist, a.max_attempts, a.out_vals,
+                                        a.uleaf, fast ? a.upos : nullptr, a.counters, a.dlist,
+                                        a.max_attempts, a.out_vals,
                                         a.out_found, a.desc);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
