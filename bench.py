@@ -4,12 +4,18 @@ Workload (the 2^26-key, 95 % lookup / 5 % insert configuration the headline metr
 quoted on): bulk-load 2^26 unique keys drawn from Gamma(k=2, theta=2^56), values
 splitmix64(key); then batches of 2^22 ops, 95 % point lookups (90 % of them hits sampled
 uniformly from the loaded keys, 10 % fresh misses) and 5 % inserts of fresh keys of the
-same distribution.  A step is one batch.  All batches are generated on the host, copied
-to HBM before the timed region, and run back to back through stm_run_batch_device.
+same distribution.  A step is one batch.  All batches are generated on the host and copied
+to HBM before the timed region.  Each route (plain descent, learned routing, the static leaf
+index; at the library defaults and at the launch parameters of the tuning record) runs them
+back to back through stm_run_batch_device, one synchronous call per batch; the learned and
+descent routes also run them as one device-resident stream (stm_run_batches_device, graph
+replay).
 
-Prints one JSON line.  `value` is Mops/s over the K timed batches (wall clock from the
-first launch to the last batch's completion).  Per-kernel device times come from HIP
-events the library records on its own stream around every phase (stm_config.timing).
+Prints one JSON line.  `value` is Mops/s over the K timed batches of the learned route at
+the tuned parameters, as a stream (wall clock from the first launch to the last batch's
+completion), the mean of two such timed streams (the stream routes alternate).  Per-kernel
+device times come from HIP events the library records on its own stream around every phase
+of the synchronous runs (stm_config.timing).
 
 A second section ("uneven") runs the uneven-batching workload: 2^26 Normal(2^62, 2^58)
 keys and a stream of batches with sizes log-uniform in [2^10, 2^22], replayed from captured
@@ -65,6 +71,9 @@ def parse():
                     help="approximate CPU-baseline budget (batches run until it is spent)")
     ap.add_argument("--no-stream", dest="stream", action="store_false",
                     help="time only one synchronous call per batch (no stream runs)")
+    ap.add_argument("--stream-rounds", type=int, default=2,
+                    help="timed streams per route, the routes alternating; a route's figure "
+                         "(and `value`) is their mean")
     ap.add_argument("--no-headline", action="store_true",
                     help="skip the learned-vs-plain section (profiling the other sections alone)")
     ap.add_argument("--serve-requests", type=int, default=8000,
@@ -572,13 +581,23 @@ def main():
     streams = {}
     if args.stream and head_name.startswith("learned"):
         plain = head_name.replace("learned", "descent")
-        # the learned route at the defaults too, so the tuning's gain is measured on the stream
-        for r in dict.fromkeys((head_name, plain, "learned")):
-            if r in res:
-                streams[r] = run_stream(r)
-                x = streams[r]
+        # the learned route at the defaults too, so the tuning's gain is measured on the stream;
+        # every route twice, the routes alternating (a route's figure is the mean of its two
+        # timed streams, so clock drift over the run does not favour the route timed last)
+        order = [r for r in dict.fromkeys((head_name, plain, "learned")) if r in res]
+        runs = {r: [] for r in order}
+        for _ in range(args.stream_rounds):
+            for r in order:
+                x = run_stream(r)
                 x["identical_to_sync"] = bool(np.array_equal(x["tail"][0], res[r]["tail"][0]) and
                                               np.array_equal(x["tail"][1], res[r]["tail"][1]))
+                runs[r].append(x)
+        for r in order:
+            xs = runs[r]
+            streams[r] = {"mops": float(np.mean([x["mops"] for x in xs])),
+                          "elapsed": float(np.mean([x["elapsed"] for x in xs])),
+                          "runs_mops": [round(x["mops"], 2) for x in xs],
+                          "identical_to_sync": all(x["identical_to_sync"] for x in xs)}
     if head_name in streams:
         mops, elapsed = streams[head_name]["mops"], streams[head_name]["elapsed"]
     else:
@@ -704,7 +723,7 @@ def main():
         "latency_ms": {"p50_batch": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99_batch": round(float(np.percentile(lat, 99)) * 1e3, 3)},
         "routes": {r: summary(r) for r in res},  # one synchronous call per batch
-        "streams": {r: {"mops": round(x["mops"], 2),
+        "streams": {r: {"mops": round(x["mops"], 2), "runs_mops": x["runs_mops"],
                         "ms_per_batch": round(x["elapsed"] / args.steps * 1e3, 3),
                         "last_batches_identical_to_sync": x["identical_to_sync"]}
                     for r, x in streams.items()},

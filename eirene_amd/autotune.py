@@ -17,9 +17,12 @@ same keys, so every trial starts from the same tree.  Each timed tree captures i
 time of stm_run_batches_device, which returns after the stream has completed, and a trial's
 figure is the median of three such timed streams.
 
-The result is a JSON tuning file:
+After the search the defaults and the best configuration are timed alternately (`final`:
+three replays each, one at a time), since the search's own figures come from trials minutes
+apart.  The result is a JSON tuning file:
     {"device": ..., "workload": ..., "best": {<StmConfig fields>}, "best_mops": ...,
-     "default_mops": ..., "trials": [{"config": {...}, "mops": ...}, ...]}
+     "default_mops": ..., "final": {"default_median": ..., "best_median": ..., ...},
+     "trials": [{"config": {...}, "mops": ...}, ...]}
 read back with `load_tuning()`.
 
     python -m eirene_amd.autotune --out tuning.json [--keys-log2 22] [--batches 48]
@@ -95,7 +98,8 @@ def _trial(sample: Sample, cfg: StmConfig, repeats: int = 3) -> float:
 
 
 def tune(sample: Sample, base: StmConfig, fanouts=FANOUTS, sweeps: int = 2,
-         min_gain: float = 0.01, log: Optional[Callable[[str], None]] = None) -> Dict:
+         min_gain: float = 0.01, log: Optional[Callable[[str], None]] = None,
+         final_rounds: int = 3) -> Dict:
     """Coordinate descent per fanout; returns the tuning record (see module doc)."""
     log = log or (lambda s: None)
     trials: List[Dict] = []
@@ -133,11 +137,24 @@ def tune(sample: Sample, base: StmConfig, fanouts=FANOUTS, sweeps: int = 2,
                 break
         if cur_mops > best_mops:
             best_cfg, best_mops = cur, cur_mops
+    # The search's figures come from trials minutes apart (the defaults' trial is the process's
+    # first): the final comparison times the defaults and the best alternately, one replay each,
+    # `final_rounds` times, and records both medians.
+    final = {"default": [], "best": []}
+    for _ in range(final_rounds):
+        final["default"].append(_trial(sample, base, repeats=1))
+        final["best"].append(_trial(sample, best_cfg, repeats=1))
+    log(f"final: defaults {final['default']} best {final['best']}")
     return {
         "best": {f: getattr(best_cfg, f) for f in TUNED_FIELDS},
         "best_mops": round(best_mops, 2),
         "default": {f: getattr(base, f) for f in TUNED_FIELDS},
         "default_mops": round(default_mops, 2),
+        "final": {"rounds": final_rounds,
+                  "default_mops": [round(x, 2) for x in final["default"]],
+                  "best_mops": [round(x, 2) for x in final["best"]],
+                  "default_median": round(float(np.median(final["default"])), 2),
+                  "best_median": round(float(np.median(final["best"])), 2)},
         "trials": trials,
     }
 
