@@ -43,11 +43,13 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mops/s (95% lookup/5% insert, 2^26 keys): learned vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per op with a 64-slot u64 accounting node (key 8 + result 8 + leaf
-# keys 512 + value 8 for a lookup; key, value, old value, leaf read + write and the lock
-# word for an insert).
+# Algorithmic bytes per op with a 64-slot u64 accounting node: a lookup reads its key (8),
+# the leaf's keys (512) and the value (8) and writes the result (8); an insert reads its key
+# and value (16) and the leaf's keys (512, routed by the lookup kernel) and its metadata line
+# (64), appends one key and value (16), writes the count and version (16) and the result (8)
+# (leaf entries are unordered, so nothing else of the leaf is read or written).
 BYTES_LOOKUP = 536
-BYTES_INSERT = 2088
+BYTES_INSERT = 632
 
 
 def parse():
