@@ -206,47 +206,58 @@ STM_DEV uint32_t wave_leaf_below(const TreeView& t, uint32_t leaf, uint64_t key)
 
 // The entries of leaf `leaf` (the first `count` slots; a leaf's entries are unordered) in key
 // order: position p = j * WAVE + lane holds key sk[j] from slot ss[j] (KEY_EMPTY past count).
-// Each entry's rank is the number of entries below it; a second pass hands each slot to the
-// lane of its rank.  Both passes are wave-uniform loops over the entries.
+// A bitonic sorting network over the wavefront's N = WAVE * SPL elements (element j * WAVE +
+// lane in register j): 21 compare-exchange steps for N = 64, 28 for 128, each a lane shuffle of
+// the key and slot (or, at distance 64, the lane's other register).  The keys are distinct and
+// KEY_EMPTY pads past count, so the order is exact.  (Ranking each entry with two wave-uniform
+// passes over the entries took a serving batch's scan kernel from 49 to 115 µs.)
 template <int F>
 STM_DEV void leaf_in_key_order(const TreeView& t, uint32_t leaf, uint32_t count,
                                uint64_t (&sk)[(F + WAVE - 1) / WAVE],
                                uint32_t (&ss)[(F + WAVE - 1) / WAVE]) {
   constexpr int SPL = (F + WAVE - 1) / WAVE;
-  const int lane = lane_id();
-  uint64_t k[SPL];
-  uint32_t r[SPL];
+  constexpr uint32_t N = (uint32_t)(WAVE * SPL);
+  static_assert(SPL == 1 || SPL == 2, "one or two slots per lane");
+  const uint32_t lane = (uint32_t)lane_id();
 #pragma unroll
   for (int j = 0; j < SPL; ++j) {
-    const uint32_t sl = (uint32_t)(j * WAVE + lane);
-    k[j] = sl < count ? t.keys[(uint64_t)leaf * F + sl] : KEY_EMPTY;
-    r[j] = 0;
-    sk[j] = KEY_EMPTY;
-    ss[j] = 0;
+    const uint32_t sl = (uint32_t)j * WAVE + lane;
+    sk[j] = sl < count ? t.keys[(uint64_t)leaf * F + sl] : KEY_EMPTY;
+    ss[j] = sl;
   }
-  for (uint32_t i = 0; i < count; ++i) {
-    uint64_t ki = 0;
 #pragma unroll
-    for (int j = 0; j < SPL; ++j)
-      if ((i >> 6) == (uint32_t)j) ki = readlane64(k[j], (int)(i & 63));
+  for (uint32_t size = 2; size <= N; size <<= 1) {
 #pragma unroll
-    for (int j = 0; j < SPL; ++j) r[j] += ki < k[j] ? 1u : 0u;
-  }
-  for (uint32_t i = 0; i < count; ++i) {
-    uint64_t ki = 0;
-    uint32_t ri = 0;
+    for (uint32_t d = size >> 1; d > 0; d >>= 1) {
+      if (d >= (uint32_t)WAVE) {
+        // SPL == 2, d == 64: elements lane and 64 + lane, both in this lane
+        if constexpr (SPL == 2) {
+          const bool up = (lane & size) == 0;  // ascending block (size >= 128 here)
+          const bool sw = up ? sk[1] < sk[0] : sk[0] < sk[1];
+          const uint64_t k0 = sk[0];
+          const uint32_t s0 = ss[0];
+          if (sw) {
+            sk[0] = sk[1];
+            ss[0] = ss[1];
+            sk[1] = k0;
+            ss[1] = s0;
+          }
+        }
+      } else {
 #pragma unroll
-    for (int j = 0; j < SPL; ++j) {
-      if ((i >> 6) == (uint32_t)j) {
-        ki = readlane64(k[j], (int)(i & 63));
-        ri = (uint32_t)__builtin_amdgcn_readlane((int)r[j], (int)(i & 63));
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < SPL; ++j) {
-      if (ri == (uint32_t)(j * WAVE + lane)) {
-        sk[j] = ki;
-        ss[j] = i;
+        for (int j = 0; j < SPL; ++j) {
+          const uint32_t e = (uint32_t)j * WAVE + lane;
+          const uint64_t pk = shfl64(sk[j], (int)(lane ^ d));
+          const uint32_t ps = (uint32_t)__shfl((int)ss[j], (int)(lane ^ d), WAVE);
+          const bool up = (e & size) == 0;
+          const bool lower = (e & d) == 0;
+          // the lower element of an ascending pair keeps the smaller key, and so on
+          const bool take = (lower == up) ? pk < sk[j] : pk > sk[j];
+          if (take) {
+            sk[j] = pk;
+            ss[j] = ps;
+          }
+        }
       }
     }
   }
