@@ -240,8 +240,9 @@ int stm_size(stm_tree* tree, uint64_t* n_keys);
  * *n receives the number of keys; STM_ERR_INVALID if cap < *n. */
 int stm_dump(stm_tree* tree, uint64_t* keys, uint64_t* vals, size_t cap, size_t* n);
 
-/* Structural check of the device tree: sorted keys, fences, levels, sibling chain,
- * child ranges, all locks released.  *bad receives the number of violations. */
+/* Structural check of the device tree: inner nodes in key order, leaf entries distinct (a
+ * leaf's entries are unordered), keys within their fences, free slots empty, levels, sibling
+ * chain, child ranges, all locks released.  *bad receives the number of violations. */
 int stm_check(stm_tree* tree, uint64_t* bad);
 
 /* Counters and per-phase device times accumulated since create or stm_reset_stats. */
