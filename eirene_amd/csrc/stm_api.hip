@@ -138,13 +138,10 @@ struct Scratch {
   uint64_t *ukey = nullptr, *uval = nullptr;
   uint8_t* uop = nullptr;
   uint32_t *gstart = nullptr, *npend = nullptr, *dlist = nullptr;
-  uint32_t* uleaf = nullptr;  // leaf hint of every merged INSERT
   uint32_t* slist = nullptr;  // merged INSERTs the fast path hands back
   uint32_t* scount = nullptr;  // per fast-path wavefront: how many it handed back
   uint32_t* ilist = nullptr;  // merged INSERTs of the batch
-  uint32_t* upar = nullptr;   // level-1 node hint of every merged INSERT
-  uint64_t* uver = nullptr;   // version of the hinted leaf
-  uint32_t* upos = nullptr;   // slot hint in the hinted leaf
+  InsHint* ihint = nullptr;   // the lookup kernel's record of every merged INSERT
   uint64_t* fanout = nullptr; // answer chunks of hot GET groups (fanout_capacity(n))
   BatchCounters* counters = nullptr;
   BatchCounters* h_counters = nullptr;  // pinned host copy for the per-batch readback
@@ -175,13 +172,10 @@ struct Scratch {
     fold_tiles = ft;
     A_(ukey, cap); A_(uval, cap); A_(uop, cap); A_(gstart, cap); A_(npend, cap);
     A_(dlist, cap);
-    A_(uleaf, cap);
+    A_(ihint, cap);
     A_(slist, cap);
     A_(scount, FAST_WAVES_MAX);
     A_(ilist, cap);
-    A_(upar, cap);
-    A_(uver, cap);
-    A_(upos, cap);
     A_(fanout, fanout_capacity(cap));
     A_(counters, 1);
     A_(desc_live, 1);
@@ -214,7 +208,7 @@ struct Scratch {
   }
   void release() {
     void* ps[] = {in_keys, in_vals, out_vals, in_ops, out_found, skeys_tmp, skeys, sidx_tmp, sidx,
-                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, uleaf, slist, scount, ilist, upar, uver, upos, fanout, counters,
+                  hist, fold_tiles, ukey, uval, uop, gstart, npend, dlist, ihint, slist, scount, ilist, fanout, counters,
                   desc_table, desc_live, cursor, acc, soff, lastins, scan_tiles};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -560,10 +554,7 @@ hipError_t enqueue_back(stm_tree* t, Scratch& s, const PipeIn& in, bool timed) {
   la.out_vals = in.out_vals;
   la.out_found = in.out_found;
   la.desc = in.desc;
-  la.uleaf = s.uleaf;
-  la.uver = s.uver;
-  la.upos = s.upos;
-  la.upar = s.upar;
+  la.ihint = s.ihint;
   la.gval = s.uval;  // GET groups carry no merged value: a hot group's value goes there
   la.fanout = s.fanout;
   la.fanout_cap = fanout_capacity(in.n);
@@ -595,10 +586,7 @@ hipError_t enqueue_back(stm_tree* t, Scratch& s, const PipeIn& in, bool timed) {
   ua.out_vals = in.out_vals;
   ua.out_found = in.out_found;
   ua.desc = in.desc;
-  ua.uleaf = s.uleaf;
-  ua.uver = s.uver;
-  ua.upos = s.upos;
-  ua.upar = s.upar;
+  ua.ihint = s.ihint;
   ua.ilist = s.ilist;
   ua.slist = s.slist;
   ua.scount = s.scount;

@@ -273,10 +273,7 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
                                                 const uint32_t* __restrict__ lids,
                                                 const SegModel* __restrict__ segs,
                                                 const BatchDesc* __restrict__ desc,
-                                                uint32_t* __restrict__ uleaf,
-                                                uint64_t* __restrict__ uver,
-                                                uint32_t* __restrict__ upos,
-                                                uint32_t* __restrict__ upar,
+                                                InsHint* __restrict__ ihint,
                                                 uint64_t* __restrict__ gval,
                                                 uint64_t* __restrict__ fanout,
                                                 uint64_t fanout_cap) {
@@ -335,7 +332,7 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
     const uint32_t np_l = valid ? np_raw : 0;
     const bool mine = op_l == OP_GET;
     // Merged INSERTs are routed too: their leaf is the upsert kernel's starting hint.
-    const bool walk = mine || (uleaf && op_l == OP_INSERT);
+    const bool walk = mine || (ihint && op_l == OP_INSERT);
     const uint32_t sx_raw = sidx[gs_l];  // gs_l is a sorted position of the batch
     const uint32_t pos_l = (mine && np_l) ? pos_of(sx_raw) : 0;
     uint64_t vidx_l = 0;  // value slot of op base + g
@@ -448,14 +445,18 @@ __global__ __launch_bounds__(256, (lookup_min_waves<F, G>())) void k_lookup(Tree
     }
     const uint64_t val = (mine && found_l) ? t.vals[vidx_l] : 0;
     if (hint) {
-      // Hints for the upsert kernels: the leaf, its version and entry count now (no writer
-      // runs during this kernel), the key's slot in it and the level-1 node last used for
-      // routing.
-      uleaf[j] = leaf_l;
-      uver[j] = ver;
-      upos[j] = (uint32_t)(vidx_l - (uint64_t)leaf_l * F) | (found_l ? UPOS_FOUND : 0u) |
-                (min(meta_count(info), 255u) << UPOS_COUNT_SHIFT);
-      upar[j] = par_l;
+      // The record for the upsert kernels: the op's answer range and leaf, the leaf's version
+      // and entry count now (no writer runs during this kernel), the key's slot in it and the
+      // level-1 node last used for routing.
+      // field by field, so that the fields need not be live together (registers)
+      InsHint* h = ihint + j;
+      h->ver = ver;
+      h->leaf = leaf_l;
+      h->pos = (uint32_t)(vidx_l - (uint64_t)leaf_l * F) | (found_l ? UPOS_FOUND : 0u) |
+               (min(meta_count(info), 255u) << UPOS_COUNT_SHIFT);
+      h->par = par_l;
+      h->gs = gs_l;
+      h->np = np_l;
     }
     // The answers.
     if (mine) {
@@ -1549,9 +1550,7 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
                                                      const uint32_t* __restrict__ npend,
                                                      const uint32_t* __restrict__ sidx,
                                                      const uint32_t* __restrict__ ilist,
-                                                     const uint32_t* __restrict__ uleaf,
-                                                     const uint64_t* __restrict__ uver,
-                                                     const uint32_t* __restrict__ upos,
+                                                     const InsHint* __restrict__ ihint,
                                                      BatchCounters* cnt, uint32_t* __restrict__ slist,
                                                      uint32_t* __restrict__ scount,
                                                      uint64_t* __restrict__ out_vals,
@@ -1582,13 +1581,14 @@ __global__ __launch_bounds__(256) void k_upsert_fast(TreeView t, const uint64_t*
     // afterwards, so the gathers issue together.
     const uint32_t gi = ilist[mine ? e : e1 - 1];
     uint64_t todo = ballot(mine);
-    const uint64_t k_raw = ukey[gi];
+    const InsHint h = ihint[gi];
     const uint64_t v_raw = uval[gi];
-    const uint32_t gs_raw = gstart[gi];
-    const uint32_t np_raw = npend[gi];
-    const uint32_t leaf_raw = uleaf[gi];
-    const uint64_t ver_raw = uver[gi];
-    const uint32_t up_raw = upos[gi];
+    const uint64_t k_raw = ukey[gi];
+    const uint32_t gs_raw = h.gs;
+    const uint32_t np_raw = h.np;
+    const uint32_t leaf_raw = h.leaf;
+    const uint64_t ver_raw = h.ver;
+    const uint32_t up_raw = h.pos;
     const uint64_t k_l = mine ? k_raw : 0;
     const uint64_t v_l = mine ? v_raw : 0;
     const uint32_t gs_l = mine ? gs_raw : 0;
@@ -1668,9 +1668,7 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
                                                 const uint32_t* __restrict__ slist,
                                                 const uint32_t* __restrict__ scount,
                                                 uint32_t fast_waves,
-                                                const uint32_t* __restrict__ upar,
-                                                const uint32_t* __restrict__ uleaf,
-                                                const uint32_t* __restrict__ upos,
+                                                const InsHint* __restrict__ ihint,
                                                 BatchCounters* cnt, uint32_t* __restrict__ dlist,
                                                 uint32_t max_attempts,
                                                 uint64_t* __restrict__ out_vals,
@@ -1722,14 +1720,30 @@ __global__ __launch_bounds__(256) void k_upsert(TreeView t, const uint64_t* __re
     const uint32_t gi = in ? (slist ? slist[e] : e) : 0;
     const bool mine = in && (slist || uop[gi] == OP_INSERT);
     uint64_t todo = ballot(mine);
-    const uint64_t k_l = mine ? ukey[gi] : 0;
-    const uint64_t v_l = mine ? uval[gi] : 0;
-    const uint32_t gs_l = mine ? gstart[gi] : 0;
-    const uint32_t np_l = mine ? npend[gi] : 0;
-    const uint32_t par_l = (mine && upar) ? upar[gi] : NODE_NONE;
-    const uint32_t leaf_l = (mine && uleaf) ? uleaf[gi] : NODE_NONE;
-    const uint32_t up_l = (mine && upos) ? upos[gi] : 0u;
-    if (upos) {
+    // The op: from the lookup kernel's record when there is one, else from the fold's arrays
+    // (then without leaf hints).
+    uint64_t k_l = 0, v_l = 0;
+    uint32_t gs_l = 0, np_l = 0, par_l = NODE_NONE, leaf_l = NODE_NONE, up_l = 0;
+    if (ihint) {
+      const InsHint h = ihint[gi];
+      const uint64_t k = ukey[gi];
+      const uint64_t v = uval[gi];
+      if (mine) {
+        k_l = k;
+        v_l = v;
+        gs_l = h.gs;
+        np_l = h.np;
+        par_l = h.par;
+        leaf_l = h.leaf;
+        up_l = h.pos;
+      }
+    } else if (mine) {
+      k_l = ukey[gi];
+      v_l = uval[gi];
+      gs_l = gstart[gi];
+      np_l = npend[gi];
+    }
+    if (ihint) {
       // Every lane at once: its own leaf transaction on the hinted leaf, for an op that needs no
       // split (leaf entries are unordered, so a new key is one append and a found key one value
       // store).  Test-and-set of the leaf's lock word; under the lock the leaf must still be a
@@ -2050,19 +2064,19 @@ static hipError_t launch_lookup_fg(const TreeView& t, const LookupArgs& a, hipSt
     k_lookup<F, G, true, true><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                        a.counters, a.chunk, a.out_vals, a.out_found,
                                                        a.route_hdr, a.fences, a.lids, a.segs, a.desc,
-                                                       a.uleaf, a.uver, a.upos, a.upar, a.gval, a.fanout,
+                                                       a.ihint, a.gval, a.fanout,
                                                        a.fanout_cap);
   else if (a.route_hdr)
     k_lookup<F, G, true, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                         a.counters, a.chunk, a.out_vals, a.out_found,
                                                         a.route_hdr, a.fences, a.lids, a.segs, a.desc,
-                                                        a.uleaf, a.uver, a.upos, a.upar, a.gval, a.fanout,
+                                                        a.ihint, a.gval, a.fanout,
                                                         a.fanout_cap);
   else
     k_lookup<F, G, false, false><<<blocks, 256, 0, s>>>(t, a.ukey, a.uop, a.gstart, a.npend, a.sidx,
                                                          a.counters, a.chunk, a.out_vals, a.out_found,
                                                          nullptr, nullptr, nullptr, nullptr, a.desc,
-                                                         a.uleaf, a.uver, a.upos, a.upar, a.gval, a.fanout,
+                                                         a.ihint, a.gval, a.fanout,
                                                          a.fanout_cap);
   return hipGetLastError();
 }
@@ -2103,7 +2117,7 @@ static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hip
       FAST_WAVES_MAX / 4);
   waves = blocks * 4;  // one scount entry per wavefront
   k_upsert_fast<F, G><<<blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
-                                                a.ilist, a.uleaf, a.uver, a.upos, a.counters, a.slist,
+                                                a.ilist, a.ihint, a.counters, a.slist,
                                                 a.scount, a.out_vals, a.out_found, a.desc,
                                                 a.fanout, a.fanout_cap, a.gval);
   return hipGetLastError();
@@ -2112,7 +2126,7 @@ static hipError_t launch_upsert_fast(const TreeView& t, const UpsertArgs& a, hip
 template <int F>
 static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStream_t s) {
   hipError_t e = hipSuccess;
-  const bool fast = a.uleaf && a.uver && a.upos && a.ilist && a.slist && a.scount;
+  const bool fast = a.ihint && a.ilist && a.slist && a.scount;
   uint32_t fast_waves = 0;
   if (fast) {
     switch (a.lanes) {
@@ -2124,8 +2138,8 @@ static hipError_t launch_upsert_f(const TreeView& t, const UpsertArgs& a, hipStr
     if (e != hipSuccess) return e;
   }
   k_upsert<F><<<a.blocks, 256, 0, s>>>(t, a.ukey, a.uval, a.uop, a.gstart, a.npend, a.sidx,
-                                        fast ? a.slist : nullptr, a.scount, fast_waves, a.upar,
-                                        a.uleaf, fast ? a.upos : nullptr, a.counters, a.dlist,
+                                        fast ? a.slist : nullptr, a.scount, fast_waves,
+                                        fast ? a.ihint : nullptr, a.counters, a.dlist,
                                         a.max_attempts, a.out_vals,
                                         a.out_found, a.desc);
   e = hipGetLastError();

@@ -18,13 +18,27 @@ constexpr uint32_t FANOUT_DIRECT = 32;
 constexpr uint32_t FANOUT_CHUNK = 1024;
 inline uint64_t fanout_capacity(uint64_t n) { return n / FANOUT_CHUNK + n / FANOUT_DIRECT + 1; }
 
-// Slot hint of a merged INSERT (one u32 per group, written by the lookup kernel): bits 0..7
-// hold the key's slot in the hinted leaf when the key is there (UPOS_FOUND set), else its
-// insert position; bits 16..23 the leaf's entry count.  Valid while the leaf keeps the
-// version the lookup kernel read.
+// Slot hint of a merged INSERT (written by the lookup kernel): bits 0..7 hold the key's slot
+// in the hinted leaf when the key is there (UPOS_FOUND set); bits 16..23 the leaf's entry
+// count.  Valid while the leaf keeps the version the lookup kernel read.
 constexpr uint32_t UPOS_SLOT = 0xffu;
 constexpr uint32_t UPOS_FOUND = 0x100u;
 constexpr uint32_t UPOS_COUNT_SHIFT = 16;
+
+// What the lookup kernel records for a merged INSERT (at its group index): everything the
+// upsert kernels need of the op apart from its key and value, in one 32-byte record.  The
+// inserts are a sparse subset of the groups, so each array they gathered from instead cost a
+// line per insert.
+struct alignas(32) InsHint {
+  uint64_t ver;   // the hinted leaf's lock word when the lookup kernel ran
+  uint32_t leaf;  // the leaf whose fences hold the key
+  uint32_t pos;   // slot hint (UPOS_*)
+  uint32_t par;   // level-1 node that routed it (NODE_NONE if none)
+  uint32_t gs;    // sorted position of the group's first op
+  uint32_t np;    // ops of the group waiting for the tree's answer
+  uint32_t pad;
+};
+static_assert(sizeof(InsHint) == 32, "two 16-byte words");
 
 struct LookupArgs {
   const uint64_t* ukey;
@@ -45,10 +59,7 @@ struct LookupArgs {
   const SegModel* segs;
   uint32_t index;         // learned routing: 1 = search the directory instead of the model
   const BatchDesc* desc;  // optional: out_vals / out_found are taken from here
-  uint32_t* uleaf;        // optional: leaf of every merged INSERT (the upsert kernels' hint),
-  uint64_t* uver;         //   that leaf's lock word when the lookup kernel ran,
-  uint32_t* upos;         //   the key's slot hint in it (UPOS_*),
-  uint32_t* upar;         //   and the level-1 node that routed it (NODE_NONE if none)
+  InsHint* ihint;         // optional: the record of every merged INSERT (the upsert kernels' hint)
   uint64_t* gval;         // a hot GET group's value (per group; GET groups leave uval unused)
   uint64_t* fanout;       // queued answer chunks: group | first op << 32 | found << 63
   uint64_t fanout_cap;    // fanout_capacity(n_max)
@@ -68,10 +79,8 @@ struct UpsertArgs {
   uint64_t* out_vals;
   uint8_t* out_found;
   const BatchDesc* desc;
-  const uint32_t* uleaf;  // leaf hints from the lookup kernel; with slist: the fast path runs
-  const uint64_t* uver;   // the hinted leaf's lock word when the lookup kernel ran
-  const uint32_t* upos;   // the key's slot hint in that leaf (UPOS_*)
-  const uint32_t* upar;   // level-1 node hint for the wave-wide path
+  const InsHint* ihint;   // the lookup kernel's records of the merged INSERTs; with slist: the
+                          //   fast path runs
   const uint32_t* ilist;  // the merged INSERT groups (counters->n_upserts)
   uint32_t* slist;        // ops the fast path hands back (capacity: groups), in the handing
                           //   wavefront's own range of the insert list
