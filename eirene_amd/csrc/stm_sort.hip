@@ -5,9 +5,12 @@
 // Stability is what makes the per-key fold reproduce input order
 // (src/gpu-stm-tree-v38/bpt-gpustm-kernel-aux.cuh:16-44), so every pass is stable.
 //
-// One pass = 8 key bits.  Two forms, chosen by the largest batch a launch sequence serves:
+// One pass = 8 key bits.  Full 64-bit keys are sorted by an order-preserving normalized image
+// of 16 or 24 bits (k_range, then k_fix orders each bucket by exact key; see the normalized-key
+// form below), with the eight exact passes as the fallback.  Two forms of the passes, chosen by
+// the largest batch a launch sequence serves:
 //
-// Up to ONESWEEP_MAX_N keys, single sweep (10 kernels per sort):
+// Up to ONESWEEP_MAX_N keys, single sweep (2 or 3 normalized passes, 8 exact):
 //   k_digit_hist  one read of the keys builds the histograms of every digit position
 //                 (per-workgroup partials)
 //   k_digit_scan  sums the partials and scans them into each pass's global digit offsets
@@ -509,10 +512,23 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_scatter(
 __global__ __launch_bounds__(SORT_BLOCK) void k_digit_hist(const uint64_t* __restrict__ keys,
                                                            const BatchDesc* __restrict__ desc,
                                                            uint32_t n_host, int begin_bit,
-                                                           int passes, uint32_t* __restrict__ words) {
+                                                           int passes, uint32_t* __restrict__ words,
+                                                           int norm) {
   __shared__ uint32_t s_h[SORT_WAVES][MAX_PASSES][RADIX];
   const SortRange rg = sort_range(desc, n_host, keys, 1);
   const int wave = threadIdx.x / WAVE;
+  // norm: the digits are those of the normalized key (key - kmin) >> nshift, 8 * passes bits,
+  // from k_range's partials (reduced here by every workgroup; workgroup 0 stores the result).
+  uint64_t kmin = 0;
+  if (norm) {
+    const NormKey nk = norm_from_partials(words, passes);
+    kmin = nk.kmin;
+    begin_bit += (int)nk.shift;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *reinterpret_cast<uint64_t*>(words + CTL_NORM) = nk.kmin;
+      words[CTL_NORM + 2] = nk.shift;
+    }
+  }
   for (int i = threadIdx.x; i < SORT_WAVES * MAX_PASSES * RADIX; i += SORT_BLOCK)
     (&s_h[0][0][0])[i] = 0;
   if (blockIdx.x == 0) {
@@ -543,7 +559,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_digit_hist(const uint64_t* __res
       if (base + (uint64_t)r * SORT_BLOCK + threadIdx.x < b1) {
 #pragma unroll
         for (int p = 0; p < MAX_PASSES; ++p)
-          if (p < passes) atomicAdd(&s_h[wave][p][digit_of(k[r], begin_bit + 8 * p)], 1u);
+          if (p < passes) atomicAdd(&s_h[wave][p][digit_of(k[r] - kmin, begin_bit + 8 * p)], 1u);
       }
     }
   }
@@ -632,7 +648,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ idx_in,
     const uint8_t* __restrict__ ops_in,
     const BatchDesc* __restrict__ desc, uint32_t n_host, int first_pass, int pass, int shift,
-    uint32_t* __restrict__ words, uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out) {
+    uint32_t* __restrict__ words, uint64_t* __restrict__ keys_out, uint32_t* __restrict__ idx_out,
+    int norm) {
   constexpr int TILE = SORT_BLOCK * ITEMS;
   constexpr int SEG = TILE / SORT_WAVES;  // contiguous elements owned by one wavefront
   __shared__ uint32_t s_wcnt[SORT_WAVES][RADIX];  // running per-wave digit counts, then wave bases
@@ -651,6 +668,12 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
   const uint8_t* ops = (first_pass && desc) ? desc->ops : ops_in;  // read on the first pass only
   const int lane = lane_id();
   const int wave = threadIdx.x / WAVE;
+  uint64_t kmin = 0;  // norm: digits of the normalized key (k_digit_hist stored kmin, nshift)
+  if (norm) {
+    const NormKey nk = norm_params(words);
+    kmin = nk.kmin;
+    shift += (int)nk.shift;
+  }
   const uint32_t cls = blockIdx.x & (TICKET_CLASSES - 1);
   uint32_t* ticket = words + CTL_TICKETS + (pass * TICKET_CLASSES + cls) * 32;
   uint32_t* stall = words + CTL_STALL;
@@ -693,7 +716,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
       const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
-      const uint32_t d = digit_of(k[r], shift);
+      const uint32_t d = digit_of(k[r] - kmin, shift);
       dk[r / 4] |= d << (8 * (r % 4));
       // Lanes holding the same digit: intersect 8 bit-ballots.
       uint64_t peers = ballot(valid);
@@ -777,7 +800,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
       const uint32_t p = (uint32_t)r * SORT_BLOCK + tid;
       if (p < tile_n) {
         const uint64_t kk = s_stage[p];
-        const uint32_t d = digit_of(kk, shift);
+        const uint32_t d = digit_of(kk - kmin, shift);
         dg[r / 4] |= d << (8 * (r % 4));
         keys_out[s_gbase[d] + (p - s_dbase[d])] = kk;
       }
@@ -1110,30 +1133,55 @@ static hipError_t sort_passes(const uint64_t* keys_in, const uint8_t* ops_in, co
   const uint64_t* kin = keys_in;
   const uint32_t* iin = nullptr;
   if (n_max <= ONESWEEP_MAX_N) {
+    // Single sweep.  Full 64-bit keys outside the exact form sort their normalized image
+    // (k_range, then the first kernel reduces its partials) in two passes up to 2^14 keys and
+    // three above, then k_fix orders each bucket by exact key and k_sort_exact takes over when a
+    // long bucket holds distinct keys, as in sort_normalized; the passes ping-pong so that the
+    // last one lands in tmp.  (Eight exact passes took 120 µs per 2^17-op batch.)
+    const bool normalized = !exact && begin_bit == 0 && end_bit == 64;
+    const int npass = normalized ? (n_max <= (1u << 14) ? 2 : 3) : passes;
+    uint32_t xgrid = 0;
+    if (normalized) {
+      hipError_t e = exact_grid<ITEMS>(xgrid);
+      if (e != hipSuccess) return e;
+      k_range<<<RANGE_BLOCKS, RANGE_THREADS, 0, stream>>>(keys_in, desc, n_host, words);
+    }
     // Every ticket class needs workgroups: the grid is a multiple of TICKET_CLASSES.
     const uint32_t grid = (std::max<uint32_t>(stride, 1) + TICKET_CLASSES - 1) / TICKET_CLASSES *
                           TICKET_CLASSES;
     const uint32_t hist_grid = std::max<uint32_t>(
         1, (n_max + SORT_BLOCK * HIST_ITEMS - 1) / (SORT_BLOCK * HIST_ITEMS));
-    k_digit_hist<<<hist_grid, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, begin_bit, passes,
-                                                       words);
-    k_digit_scan<<<passes, SORT_BLOCK, 0, stream>>>(words, hist_grid);
+    k_digit_hist<<<hist_grid, SORT_BLOCK, 0, stream>>>(keys_in, desc, n_host, begin_bit, npass,
+                                                       words, normalized ? 1 : 0);
+    k_digit_scan<<<npass, SORT_BLOCK, 0, stream>>>(words, hist_grid);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    for (int p = 0; p < passes; ++p) {
+    for (int p = 0; p < npass; ++p) {
       const int shift = begin_bit + 8 * p;
-      const bool to_out = ((passes - 1 - p) % 2) == 0;
+      const bool last_to_out = ((npass - 1 - p) % 2) == 0;
+      const bool to_out = normalized ? !last_to_out : last_to_out;
       uint64_t* kout = to_out ? keys_out : keys_tmp;
       uint32_t* iout = to_out ? idx_out : idx_tmp;
       const int first = p == 0 ? 1 : 0;
       k_onesweep<ITEMS><<<grid, SORT_BLOCK, 0, stream>>>(kin, iin, first ? ops_in : nullptr, desc,
-                                                         n_host, first, p, shift, words, kout, iout);
+                                                         n_host, first, p, shift, words, kout, iout,
+                                                         normalized ? 1 : 0);
       e = hipGetLastError();
       if (e != hipSuccess) return e;
       kin = kout;
       iin = iout;
     }
-    return hipSuccess;
+    if (normalized) {
+      const uint32_t fgrid = std::max<uint32_t>(
+          1, std::min<uint32_t>((n_max + FIX_TILE - 1) / FIX_TILE, SORT_GRID_CAP));
+      k_fix<<<fgrid, SORT_BLOCK, 0, stream>>>(keys_tmp, idx_tmp, desc, n_host, keys_out, idx_out,
+                                              words, npass);
+      k_sort_exact<ITEMS><<<xgrid, SORT_BLOCK, 0, stream>>>(keys_in, ops_in, desc, n_host,
+                                                           keys_tmp, keys_out, idx_tmp, idx_out,
+                                                           words, stride);
+      e = hipGetLastError();
+    }
+    return e;
   }
   if (!exact && begin_bit == 0 && end_bit == 64)
     return sort_normalized<ITEMS>(keys_in, ops_in, desc, n_host, n_max, keys_tmp, keys_out,
