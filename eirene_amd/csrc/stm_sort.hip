@@ -16,7 +16,7 @@
 //   k_digit_scan  sums the partials and scans them into each pass's global digit offsets
 //   k_onesweep    one launch per pass: ranking, the tile's offset among earlier tiles by
 //                 decoupled look-back, staging and coalesced writes (as k_scatter below)
-// Larger, three kernels per pass (24 per sort):
+// Larger, three kernels per pass (3 normalized passes, 4 above 2^23 keys; 8 exact):
 //   k_hist      tile digit histograms           -> hist[digit][tile]
 //   k_scan_hist exclusive scan per digit over tiles, digit totals
 //   k_scatter   stable in-tile ranking with 64-lane ballots (a wavefront owns a contiguous
