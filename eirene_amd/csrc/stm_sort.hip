@@ -88,8 +88,8 @@ constexpr uint32_t GOFF_OFF = (CTL_NORM + 32 + 63) / 64 * 64;       // goff[MAX_
 constexpr uint32_t PART_OFF = GOFF_OFF + MAX_PASSES * RADIX;        // [HIST_GRID_CAP][MAX_PASSES][RADIX]
 constexpr uint32_t STATUS_OFF = PART_OFF + HIST_GRID_CAP * MAX_PASSES * RADIX;  // u64 [OS_MAX_TILES][RADIX]
 // k_range: RANGE_BLOCKS workgroups of RANGE_THREADS, one (min, max) partial each (two rounds of
-// eight loads per thread at 2^22 keys).  The partials are reduced by every workgroup of the
-// first histogram pass: a last-workgroup reduction inside k_range (an agent-scope release and
+// eight loads per thread at 2^22 keys; one round of sixteen measured the same).  The partials
+// are reduced by every workgroup of the first histogram pass: a last-workgroup reduction inside k_range (an agent-scope release and
 // an arrival atomic per workgroup, an acquire in the last) took 16 µs per 32 MiB of keys.
 constexpr uint32_t RANGE_BLOCKS = 256;
 constexpr uint32_t RANGE_THREADS = 1024;
@@ -889,70 +889,112 @@ __global__ __launch_bounds__(RANGE_THREADS) void k_range(const uint64_t* __restr
 }
 
 // Buckets (runs of equal nk) into key order.  A step covers FIX_TILE sorted positions with
-// FIX_SPAN positions of context on each side in LDS.  Position i's bucket is found by scanning
-// at most FIX_SPAN positions each way: a bucket of at most FIX_SPAN elements lies wholly in the
-// window, and its element i goes to bucket start + its rank by (key, position).  Elements of
-// longer buckets stay where they are; a longer bucket with two different adjacent keys raises
-// CTL_NEED_EXACT.
+// FIX_SPAN positions of context on each side in LDS.  A bucket of at most FIX_SPAN elements lies
+// wholly in the window, and its element i goes to bucket start + its rank by (key, position).
+// Elements of longer buckets stay where they are; a longer bucket with two different adjacent
+// keys raises CTL_NEED_EXACT.
+// Bucket bounds: the 64 lanes of a wavefront hold 64 consecutive positions, so one ballot of
+// "first of its bucket" and one of "last of its bucket" give every lane its bucket's start and
+// end when they lie among those positions; only a bucket that reaches past them is walked in LDS
+// (at most FIX_SPAN steps each way).  Walking every bucket took 40.5 µs per 2^22 keys, the
+// ballots 30.6.
+// Only keys go through LDS (17 KB per workgroup); a thread keeps the payloads of its own tile
+// positions in registers, so eight workgroups fit on a CU, the most its waves allow, and a
+// 2^22-key batch is fixed in one round of workgroups.
+constexpr uint32_t FIX_ITEMS = FIX_TILE / SORT_BLOCK;
 __global__ __launch_bounds__(SORT_BLOCK) void k_fix(const uint64_t* __restrict__ keys_in,
                                                     const uint32_t* __restrict__ idx_in,
                                                     const BatchDesc* __restrict__ desc,
                                                     uint32_t n_host, uint64_t* __restrict__ keys_out,
                                                     uint32_t* __restrict__ idx_out,
                                                     uint32_t* __restrict__ words, int npass) {
-  __shared__ uint32_t s_pl[FIX_WIN];
-  __shared__ uint64_t s_key[FIX_WIN];
+  __shared__ uint64_t s_key[FIX_WIN];  // s_key[x]: key of sorted position t0 - FIX_SPAN + x
   const uint32_t n = desc ? desc->n : n_host;
   const NormKey nkp = norm_params(words);
+  const uint32_t tid = threadIdx.x;
   bool need = false;
   for (uint32_t t = blockIdx.x; (uint64_t)t * FIX_TILE < n; t += gridDim.x) {
     const uint32_t t0 = t * FIX_TILE, t1 = min(n, t0 + FIX_TILE);
-    const uint32_t w0 = t0 > FIX_SPAN ? t0 - FIX_SPAN : 0u;
-    const uint32_t w1 = min(n, t1 + FIX_SPAN);
-    // The window in one round of loads per thread (indexes clamped into the window, the
-    // clamped elements not stored).
-    constexpr uint32_t FIX_LOADS = (FIX_WIN + SORT_BLOCK - 1) / SORT_BLOCK;
-    uint64_t wk[FIX_LOADS];
-    uint32_t wp[FIX_LOADS];
+    // Window positions [lo, hi) are in the batch; LDS index x holds position x + t0 - FIX_SPAN.
+    const uint32_t lo_x = t0 >= FIX_SPAN ? 0u : FIX_SPAN - t0;
+    const uint32_t hi_x = min(n, t1 + FIX_SPAN) - t0 + FIX_SPAN;
+    // One round of loads: the thread's tile positions (key and payload; indexes clamped into the
+    // tile, the clamped elements not used) and, for 2 * FIX_SPAN threads, one context key.
+    uint64_t k[FIX_ITEMS];
+    uint32_t pl[FIX_ITEMS];
 #pragma unroll
-    for (uint32_t r = 0; r < FIX_LOADS; ++r) {
-      const uint32_t i = min(w0 + r * SORT_BLOCK + threadIdx.x, w1 - 1);
-      wk[r] = gload(keys_in + i);
-      wp[r] = gload(idx_in + i);
+    for (uint32_t r = 0; r < FIX_ITEMS; ++r) {
+      const uint32_t i = min(t0 + r * SORT_BLOCK + tid, t1 - 1);
+      k[r] = gload(keys_in + i);
+      pl[r] = gload(idx_in + i);
     }
+    // Context slot: x = tid on the left (positions t0 - FIX_SPAN ..), x = t1 - t0 + tid on the
+    // right (positions t1 ..); tid - FIX_SPAN of the right slots.
+    const uint32_t cx = tid < FIX_SPAN ? tid : t1 - t0 + tid;
+    const bool cvalid = tid < 2 * FIX_SPAN && cx >= lo_x && cx < hi_x;
+    uint64_t ck = 0;
+    if (cvalid) ck = gload(keys_in + (t0 + cx - FIX_SPAN));
 #pragma unroll
-    for (uint32_t r = 0; r < FIX_LOADS; ++r) {
-      const uint32_t x = r * SORT_BLOCK + threadIdx.x;
-      if (w0 + x < w1) {
-        s_key[x] = wk[r];
-        s_pl[x] = wp[r];
-      }
+    for (uint32_t r = 0; r < FIX_ITEMS; ++r) {
+      const uint32_t x = FIX_SPAN + r * SORT_BLOCK + tid;
+      if (t0 + r * SORT_BLOCK + tid < t1) s_key[x] = k[r];
     }
+    if (cvalid) s_key[cx] = ck;
     __syncthreads();
-    const uint32_t wn = w1 - w0;
-    for (uint32_t i = t0 + threadIdx.x; i < t1; i += SORT_BLOCK) {
-      const uint32_t x = i - w0;
+    // Not unrolled (the walks would be copied eight times): the key comes from LDS and the
+    // payload out of the registers by a chain of selects.
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint64_t upto = (2ull << lane) - 1;  // lanes 0..lane
+    const uint64_t from = ~0ull << lane;       // lanes lane..63
+#pragma unroll 1
+    for (uint32_t r = 0; r < FIX_ITEMS; ++r) {
+      if (t0 + r * SORT_BLOCK >= t1) break;
+      const uint32_t i = t0 + r * SORT_BLOCK + tid;
+      const bool valid = i < t1;
+      const uint32_t x = FIX_SPAN + r * SORT_BLOCK + tid;
+      const bool has_prev = x > lo_x, has_next = x + 1 < hi_x;
       const uint64_t key = s_key[x];
+      const uint64_t kp = s_key[has_prev ? x - 1 : x];
+      const uint64_t kn = s_key[has_next ? x + 1 : x];
       const uint32_t v = norm_of(key, nkp);
-      uint32_t l = x;
-      while (l > 0 && x - l < FIX_SPAN && norm_of(s_key[l - 1], nkp) == v) --l;
-      uint32_t r = x + 1;
-      while (r < wn && r - x <= FIX_SPAN && norm_of(s_key[r], nkp) == v) ++r;
-      if (x - l >= FIX_SPAN || r - x > FIX_SPAN || r - l > FIX_SPAN) {
-        // A long bucket: keep input order; it is sorted already unless its keys differ.
-        keys_out[i] = key;
-        idx_out[i] = s_pl[x];
-        if (x > 0 && norm_of(s_key[x - 1], nkp) == v && s_key[x - 1] != key) need = true;
-        continue;
+      const bool first = !has_prev || norm_of(kp, nkp) != v;
+      const bool last = !has_next || norm_of(kn, nkp) != v;
+      const uint64_t fmask = ballot(valid && first) & upto;
+      const uint64_t lmask = ballot(valid && last) & from;
+      if (!valid) continue;
+      uint32_t dst = i;
+      if (!(first && last)) {
+        const uint32_t cx = x - lane;  // LDS index of the wavefront's first position
+        uint32_t l, e;
+        if (fmask) {
+          l = cx + 63u - (uint32_t)__clzll(fmask);
+        } else {
+          l = cx;
+          while (l > lo_x && x - l < FIX_SPAN && norm_of(s_key[l - 1], nkp) == v) --l;
+        }
+        if (lmask) {
+          e = cx + (uint32_t)__builtin_ctzll(lmask) + 1u;
+        } else {
+          e = cx + WAVE;
+          while (e < hi_x && e - x <= FIX_SPAN && norm_of(s_key[e], nkp) == v) ++e;
+        }
+        if (x - l >= FIX_SPAN || e - x > FIX_SPAN || e - l > FIX_SPAN) {
+          // A long bucket: keep input order; it is sorted already unless its keys differ.
+          if (!first && kp != key) need = true;
+        } else {
+          uint32_t rank = 0;
+          for (uint32_t j = l; j < e; ++j) {
+            const uint64_t kj = s_key[j];
+            rank += (kj < key || (kj == key && j < x)) ? 1u : 0u;
+          }
+          dst = t0 + l + rank - FIX_SPAN;
+        }
       }
-      uint32_t rank = 0;
-      for (uint32_t j = l; j < r; ++j) {
-        const uint64_t kj = s_key[j];
-        rank += (kj < key || (kj == key && j < x)) ? 1u : 0u;
-      }
-      const uint32_t dst = w0 + l + rank;
+      uint32_t pv = pl[0];
+#pragma unroll
+      for (uint32_t q = 1; q < FIX_ITEMS; ++q) pv = r == q ? pl[q] : pv;
       keys_out[dst] = key;
-      idx_out[dst] = s_pl[x];
+      idx_out[dst] = pv;
     }
     __syncthreads();
   }

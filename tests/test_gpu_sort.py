@@ -35,6 +35,13 @@ def keys_of(dist, n, rng):
         k[:m] = np.uint64(1 << 55)
         k[m: 2 * m] = np.uint64(1 << 55) + rng.integers(1, 1 << 30, m).astype(np.uint64)
         return k
+    if dist == "medium":  # groups of 1..64 keys (some equal), each alone in a 2^39-wide bucket
+        sizes = rng.integers(1, 65, max(2, n // 16))
+        slots = rng.choice(1 << 23, sizes.size, replace=False).astype(np.uint64) * np.uint64(2)
+        slots[:2] = [0, (1 << 24) - 2]  # the range spans 2^63: normalized buckets 2^39 wide
+        base = (slots << np.uint64(39)) + rng.integers(0, 1 << 38, sizes.size).astype(np.uint64)
+        k = np.repeat(base, sizes) + rng.integers(0, 1 << 20, int(sizes.sum())).astype(np.uint64)
+        return k[:n]
     if dist == "cluster":  # a dense cluster of distinct keys inside a 2^63-wide range
         k = rng.integers(0, 1 << 63, n, dtype=np.uint64)
         m = n // 2
@@ -52,7 +59,7 @@ def check(keys):
     assert np.array_equal(got_k, keys[want_i])
 
 
-DISTS = ["uniform63", "full64", "gamma", "narrow", "equal", "hot", "cluster"]
+DISTS = ["uniform63", "full64", "gamma", "narrow", "equal", "hot", "medium", "cluster"]
 
 
 @pytest.mark.parametrize("dist", DISTS)
@@ -64,7 +71,7 @@ def test_sort_matches_stable_argsort(dist, n):
     check(keys)
 
 
-@pytest.mark.parametrize("dist", ["uniform63", "gamma", "hot"])
+@pytest.mark.parametrize("dist", ["uniform63", "gamma", "hot", "medium"])
 @pytest.mark.parametrize("n", [1 << 22, (1 << 23) + 5])
 def test_sort_large(dist, n):
     """2^22 keys (three 8-bit normalized passes) and just above 2^23 (four)."""
