@@ -130,6 +130,24 @@ STM_DEV uint32_t digit_of(uint64_t k, int shift) {
   return w & (RADIX - 1);
 }
 
+// The valid lanes of the wavefront that hold the same 8-bit digit d as this lane.  Lane M
+// differs from this lane at bit b exactly where ballot(bit b) disagrees with this lane's own
+// bit b: in ballot(bit b) XOR (all ones if the own bit is set).  OR-ing those over the 8 bits and
+// complementing gives the peers.  The own bit is sign-extended into a word by one bit-field
+// extract and the masks are kept as 32-bit halves: about 5 VALU per bit, against 9 for
+// intersecting ballot(bit) or its complement chosen by a select.
+STM_DEV uint64_t digit_peers(uint32_t d, bool valid) {
+  uint32_t dlo = 0, dhi = 0;
+#pragma unroll
+  for (int bit = 0; bit < 8; ++bit) {
+    const uint64_t bb = ballot((d >> bit) & 1u);
+    const uint32_t own = (uint32_t)((int32_t)(d << (31 - bit)) >> 31);
+    dlo |= (uint32_t)bb ^ own;
+    dhi |= (uint32_t)(bb >> 32) ^ own;
+  }
+  return ballot(valid) & ~(((uint64_t)dhi << 32) | dlo);
+}
+
 constexpr uint32_t FIX_TILE = 2048;  // sorted positions per k_fix step
 constexpr uint32_t FIX_SPAN = 64;    // longest bucket k_fix orders itself
 constexpr uint32_t FIX_WIN = FIX_TILE + 2 * FIX_SPAN;
@@ -363,7 +381,6 @@ STM_DEV void scatter_body(
     uint32_t tot;
     s_tbase[threadIdx.x] = block_exclusive_scan_256(totals[threadIdx.x], s_tmp, tot);
   }
-  const uint64_t lt = lanemask_lt();
   for (uint32_t b = blockIdx.x; b < ntiles; b += gridDim.x) {
     const uint64_t tile0 = (uint64_t)b * TILE;
     const uint32_t tile_n = (uint32_t)min<uint64_t>(TILE, rg.n - tile0);
@@ -399,21 +416,18 @@ STM_DEV void scatter_body(
       const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
       const uint32_t d = digit_of(k[r] - kmin, shift);
       dk[r / 4] |= d << (8 * (r % 4));
-      // Lanes holding the same digit: intersect 8 bit-ballots.
-      uint64_t peers = ballot(valid);
-#pragma unroll
-      for (int bit = 0; bit < 8; ++bit) {
-        const bool on = (d >> bit) & 1u;
-        const uint64_t bb = ballot(on);
-        peers &= on ? bb : ~bb;
-      }
+      const uint64_t peers = digit_peers(d, valid);
       uint32_t base = valid ? s_wcnt[wave][d] : 0u;
-      rk[r] = base + (uint32_t)__popcll(peers & lt);
+      // Peers below this lane (mbcnt: masked popcount against the lane's own position) and in
+      // all; the last of them, the one with no peer above, advances the digit's count.
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+      const uint32_t cnt = (uint32_t)__popcll(peers);
+      rk[r] = base + below;
       // Materialise the rank here: left to itself the compiler keeps the 64-bit peer mask of
       // every item alive across the barrier below, which costs occupancy.
       asm volatile("" : "+v"(rk[r]));
-      const bool leader = valid && (lane == 63 - __clzll(peers));
-      if (leader) s_wcnt[wave][d] = base + (uint32_t)__popcll(peers);
+      if (valid && below + 1u == cnt) s_wcnt[wave][d] = base + cnt;
     }
     __syncthreads();
     {
@@ -680,7 +694,6 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
   uint64_t* status = reinterpret_cast<uint64_t*>(words + STATUS_OFF);
   const uint32_t tag = (words[CTL_EPOCH] << 3) | (uint32_t)pass;
   const uint32_t goff = words[GOFF_OFF + pass * RADIX + threadIdx.x];  // digit threadIdx.x
-  const uint64_t lt = lanemask_lt();
   for (;;) {
     for (int i = threadIdx.x; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wcnt[0][0])[i] = 0;
     if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
@@ -718,21 +731,18 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
       const bool valid = lane0 + (uint32_t)r * WAVE < tile_n;
       const uint32_t d = digit_of(k[r] - kmin, shift);
       dk[r / 4] |= d << (8 * (r % 4));
-      // Lanes holding the same digit: intersect 8 bit-ballots.
-      uint64_t peers = ballot(valid);
-#pragma unroll
-      for (int bit = 0; bit < 8; ++bit) {
-        const bool on = (d >> bit) & 1u;
-        const uint64_t bb = ballot(on);
-        peers &= on ? bb : ~bb;
-      }
+      const uint64_t peers = digit_peers(d, valid);
       uint32_t base = valid ? s_wcnt[wave][d] : 0u;
-      rk[r] = base + (uint32_t)__popcll(peers & lt);
+      // Peers below this lane (mbcnt: masked popcount against the lane's own position) and in
+      // all; the last of them, the one with no peer above, advances the digit's count.
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+      const uint32_t cnt = (uint32_t)__popcll(peers);
+      rk[r] = base + below;
       // Materialise the rank here: left to itself the compiler keeps the 64-bit peer mask of
       // every item alive across the barrier below, which costs occupancy.
       asm volatile("" : "+v"(rk[r]));
-      const bool leader = valid && (lane == 63 - __clzll(peers));
-      if (leader) s_wcnt[wave][d] = base + (uint32_t)__popcll(peers);
+      if (valid && below + 1u == cnt) s_wcnt[wave][d] = base + cnt;
     }
     __syncthreads();
     uint64_t* row = status + (uint64_t)b * RADIX;
